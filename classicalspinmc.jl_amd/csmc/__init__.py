@@ -1,0 +1,21 @@
+"""csmc -- MI355X (gfx950) engine for the sweep hot path of ClassicalSpinMC.jl.
+
+The reference API is mirrored with Julia's `!` dropped from mutating names
+(addBilinear! -> addBilinear, simulated_annealing! -> simulated_annealing).
+Everything that touches spins runs in libcsmc on the GPU; there is no CPU
+fallback.
+"""
+from ._lib import (CONE, CONE_ADAPTIVE, DETERMINISTIC, FP32, FP64, METROPOLIS, OVERRELAX,
+                   BoundsError, Context, CsmcError, UndefError, default_context, device_count, lib)
+from .batch import SpinBatch
+from .lattice import (Lattice, energy, energy_density, get_local_field, get_magnetization, get_spin,
+                      local_fields, random_spin_orientation, set_spin, site_energies, total_energy)
+from .montecarlo import (MCParamsBuffer, Metropolis, MetropolisAdaptive, MetropolisConstraint,
+                         MetropolisConstraintAdaptive, MetropolisFixedCone, MonteCarlo,
+                         SimulationParameters, deterministic_updates, overrelaxation,
+                         parallel_tempering, simulated_annealing)
+from .unitcell import (FCC, BreathingPyrochlore, Honeycomb, IMToMatrix, InteractionMatrix, Pyrochlore,
+                       Square, Triangular, UnitCell, addBasisSite, addBilinear, addCubic, addOnSite,
+                       addQuartic, addZeemanCoupling, transposeJ)
+
+__all__ = [n for n in dir() if not n.startswith("_")]

@@ -1,0 +1,171 @@
+"""SpinBatch: a batch of systems (replicas of one lattice, or independent
+lattices of uneven size) resident on one GPU, driven through libcsmc.
+
+This is what replaces one MPI rank per replica (monte_carlo.jl:246-262):
+every system of the batch sweeps in the same persistent launch, and
+parallel tempering exchanges temperature labels on the device.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+class SpinBatch:
+    def __init__(self, lattices, precision=32, ctx=None):
+        self.ctx = ctx or _lib.default_context()
+        self.lattices = list(lattices)
+        if not self.lattices:
+            raise ValueError("empty batch")
+        self.precision = int(precision)
+        ids = np.array([lat.device_id(self.ctx) for lat in self.lattices], dtype=np.int32)
+        bid = C.c_int32()
+        _lib.check(_lib.lib().csmc_batch_create(self.ctx.h, len(ids), _lib.ptr(ids), self.precision,
+                                                C.byref(bid)))
+        self.id = bid.value
+        self.n = len(ids)
+        self.sizes = np.array([lat.size for lat in self.lattices], dtype=np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)])
+
+    @property
+    def _h(self):
+        return self.ctx.h
+
+    def close(self):
+        if self.id is not None and self.ctx.h:
+            _lib.lib().csmc_batch_destroy(self.ctx.h, self.id)
+        self.id = None
+
+    # -- spins ---------------------------------------------------------------------
+    def set_spins(self, i, spins):
+        s = _lib.as_f64(spins)
+        if s.shape == (3, self.sizes[i]):
+            s = np.ascontiguousarray(s.T)
+        _lib.check(_lib.lib().csmc_set_spins(self._h, self.id, i, _lib.ptr(s.reshape(-1, 3))))
+
+    def get_spins(self, i):
+        out = np.zeros((int(self.sizes[i]), 3))
+        _lib.check(_lib.lib().csmc_get_spins(self._h, self.id, i, _lib.ptr(out)))
+        return out
+
+    def load_lattices(self):
+        for i, lat in enumerate(self.lattices):
+            self.set_spins(i, lat._spins)
+
+    def randomize(self, seed):
+        _lib.check(_lib.lib().csmc_randomize_spins(self._h, self.id, int(seed)))
+
+    # -- evaluation ---------------------------------------------------------------
+    def total_energy(self, parts=False):
+        E = np.zeros(self.n)
+        P = np.zeros((self.n, 5))
+        _lib.check(_lib.lib().csmc_total_energy(self._h, self.id, _lib.ptr(E), _lib.ptr(P)))
+        return (E, P) if parts else E
+
+    def magnetization(self):
+        M = np.zeros((self.n, 3))
+        _lib.check(_lib.lib().csmc_magnetization(self._h, self.id, _lib.ptr(M)))
+        return M
+
+    def local_field(self, i):
+        H = np.zeros((int(self.sizes[i]), 3))
+        _lib.check(_lib.lib().csmc_local_field(self._h, self.id, i, _lib.ptr(H)))
+        return H
+
+    def site_energy(self, i):
+        e = np.zeros(int(self.sizes[i]))
+        _lib.check(_lib.lib().csmc_site_energy(self._h, self.id, i, _lib.ptr(e)))
+        return e
+
+    # -- sweeps ---------------------------------------------------------------------
+    def _vec(self, v, default=None):
+        if v is None:
+            return None if default is None else np.full(self.n, float(default))
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (self.n,)).copy())
+
+    def sweep(self, kind, T=None, sigma=None, n_sweeps=1, seed=0, plain=False):
+        """n_sweeps coloured sweeps of `kind`; returns accepted counts (and
+        updates `sigma` in place for the adaptive kind if an array is given)."""
+        Tv = self._vec(T)
+        sg = sigma if isinstance(sigma, np.ndarray) and sigma.dtype == np.float64 and sigma.shape == (self.n,) \
+            else self._vec(sigma)
+        acc = np.zeros(self.n)
+        fn = _lib.lib().csmc_sweep_plain if plain else _lib.lib().csmc_sweep
+        _lib.check(fn(self._h, self.id, int(kind), _lib.ptr(Tv), _lib.ptr(sg), int(n_sweeps), int(seed),
+                      _lib.ptr(acc)))
+        return acc
+
+    def counters(self):
+        c = np.zeros(self.n, dtype=np.uint32)
+        _lib.check(_lib.lib().csmc_get_counters(self._h, self.id, _lib.ptr(c)))
+        return c
+
+    def set_counters(self, c):
+        c = np.ascontiguousarray(np.broadcast_to(np.asarray(c, dtype=np.uint32), (self.n,)).copy())
+        _lib.check(_lib.lib().csmc_set_counters(self._h, self.id, _lib.ptr(c)))
+
+    def anneal(self, temps, t_thermalization, overrelaxation_rate, alg, sigma0=60.0, seed=0):
+        temps = _lib.as_f64(temps).reshape(-1)
+        acc = np.zeros((self.n, len(temps)))
+        _lib.check(_lib.lib().csmc_anneal(self._h, self.id, _lib.ptr(temps), len(temps),
+                                          int(t_thermalization), int(overrelaxation_rate), int(alg),
+                                          float(sigma0), int(seed), _lib.ptr(acc)))
+        return acc
+
+    # -- parallel tempering -------------------------------------------------------
+    def pt_init(self, temps, params, alg, sigma0=60.0, seed=0, measure=True):
+        temps = _lib.as_f64(temps).reshape(-1)
+        if temps.shape != (self.n,):
+            raise ValueError("one temperature per system")
+        p = _lib.PTParams()
+        for k in ("t_thermalization", "t_measurement", "probe_rate", "swap_rate",
+                  "overrelaxation_rate", "report_interval", "checkpoint_rate"):
+            setattr(p, k, int(params[k]))
+        p.alg = int(alg)
+        p.measure = 1 if measure else 0
+        self.temps = temps
+        _lib.check(_lib.lib().csmc_pt_init(self._h, self.id, _lib.ptr(temps), C.byref(p), float(sigma0),
+                                           int(seed)))
+
+    def pt_advance(self, until=-1):
+        stopped = C.c_int64()
+        event = C.c_int32()
+        _lib.check(_lib.lib().csmc_pt_advance(self._h, self.id, int(until), C.byref(stopped),
+                                              C.byref(event)))
+        return stopped.value, event.value
+
+    def pt_state(self):
+        sos = np.zeros(self.n, dtype=np.int32)
+        E = np.zeros(self.n)
+        al = np.zeros(self.n)
+        ax = np.zeros(self.n)
+        sg = np.zeros(self.n)
+        _lib.check(_lib.lib().csmc_pt_state(self._h, self.id, _lib.ptr(sos), _lib.ptr(E), _lib.ptr(al),
+                                            _lib.ptr(ax), _lib.ptr(sg)))
+        return dict(system_of_slot=sos, E=E, accepted_local=al, accepted_exchange=ax, sigma=sg)
+
+    def pt_observables(self):
+        out = np.zeros((self.n, 8))
+        _lib.check(_lib.lib().csmc_pt_observables(self._h, self.id, _lib.ptr(out)))
+        keys = ("specific_heat", "specific_heat_err", "susceptibility", "susceptibility_err",
+                "energy", "energy_err", "magnetization", "magnetization_err")
+        return {k: out[:, i].copy() for i, k in enumerate(keys)}
+
+    # -- adjoint --------------------------------------------------------------------
+    def grad_spins(self):
+        g = np.zeros((int(self.offsets[-1]), 3))
+        _lib.check(_lib.lib().csmc_energy_grad_spins(self._h, self.id, _lib.ptr(g)))
+        return [g[self.offsets[i]:self.offsets[i + 1]] for i in range(self.n)]
+
+    def grad_couplings(self):
+        n = self.lattices[0].unit_cell.n_params()
+        g = np.zeros((self.n, n))
+        _lib.check(_lib.lib().csmc_energy_grad_couplings(self._h, self.id, _lib.ptr(g)))
+        return g
+
+    def gradient_descent(self, eta, n_steps=1):
+        _lib.check(_lib.lib().csmc_gradient_descent(self._h, self.id, float(eta), int(n_steps)))
+
+    def sync(self):
+        _lib.check(_lib.lib().csmc_ctx_sync(self._h))

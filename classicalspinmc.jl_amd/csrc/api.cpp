@@ -1,0 +1,1029 @@
+// api.cpp -- libcsmc host runtime and C ABI (include/csmc.h).
+//
+// A context owns one HIP stream on one device, the device table pools of
+// every lattice created on it, and the batches of systems.  Lattice tables
+// are append-only, so pool offsets stay valid when lattices are added; the
+// device pools are re-uploaded lazily before the next launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/csmc.h"
+#include "csmc_internal.h"
+#include "kernels.h"
+#include "lattice.h"
+
+using namespace csmc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(CSMC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void free_() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        free_();
+        n = count;
+        if (!count) return hipSuccess;
+        return hipMalloc((void **)&p, count * sizeof(T));
+    }
+    ~DevBuf() { free_(); }
+};
+
+struct PTHost {
+    bool init = false;
+    csmc_pt_params p{};
+    int64_t sweep = 0;
+    bool mid = false;
+    int64_t total = 0, dosweep = 1;
+    std::vector<double> temps;
+    DevBuf<double> temps_d, E, sigma, acc_local, acc_exch, bins;
+    DevBuf<int32_t> slot_of_sys, sys_of_slot;
+    DevBuf<uint32_t> done;
+    uint64_t seed = 0;
+};
+
+struct Batch {
+    int prec = 64;
+    int nsys = 0;
+    int maxN = 0;
+    int64_t total = 0;
+    bool self = false;
+    bool gen = false;  // some lattice has cubic or quartic terms
+    std::vector<int> lat;
+    std::vector<SysDev> sys;
+    DevBuf<SysDev> sys_d;
+    DevBuf<unsigned char> spins;
+    DevBuf<uint32_t> mcount;
+    DevBuf<double> T, sigma, acc;  // per-system scratch
+    DevBuf<uint32_t> accU;
+    DevBuf<double> stage;          // host<->device staging (total*3 doubles)
+    PTHost pt;
+    size_t spin_bytes() const { return prec == 32 ? 16 : 32; }
+};
+
+}  // namespace
+
+struct csmc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<HostLattice>> lats;
+    std::vector<LatDev> h_lat;
+    std::vector<int32_t> h_i32;
+    std::vector<int4> h_runs;
+    std::vector<double> h_coef;
+    DevBuf<LatDev> d_lat;
+    DevBuf<int32_t> d_i32;
+    DevBuf<int4> d_runs;
+    DevBuf<float> d_coefF;
+    DevBuf<double> d_coefD;
+    bool dirty = true;
+    std::map<int, std::unique_ptr<Batch>> batches;
+    int next_batch = 1;
+    int max_lds = 65536;
+    bool no_lds = false;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    double timed_ms = 0.0;
+    int64_t timed_launches = 0;
+};
+
+namespace {
+
+int ensure_pools(csmc_ctx *c) {
+    if (!c->dirty) return CSMC_OK;
+    HIPCHK(c->d_lat.alloc(c->h_lat.size()));
+    HIPCHK(c->d_i32.alloc(c->h_i32.size()));
+    HIPCHK(c->d_runs.alloc(c->h_runs.size()));
+    HIPCHK(c->d_coefF.alloc(c->h_coef.size()));
+    HIPCHK(c->d_coefD.alloc(c->h_coef.size()));
+    std::vector<float> cf(c->h_coef.begin(), c->h_coef.end());
+    if (c->h_lat.size())
+        HIPCHK(hipMemcpy(c->d_lat.p, c->h_lat.data(), c->h_lat.size() * sizeof(LatDev), hipMemcpyHostToDevice));
+    if (c->h_i32.size())
+        HIPCHK(hipMemcpy(c->d_i32.p, c->h_i32.data(), c->h_i32.size() * 4, hipMemcpyHostToDevice));
+    if (c->h_runs.size())
+        HIPCHK(hipMemcpy(c->d_runs.p, c->h_runs.data(), c->h_runs.size() * sizeof(int4), hipMemcpyHostToDevice));
+    if (cf.size()) {
+        HIPCHK(hipMemcpy(c->d_coefF.p, cf.data(), cf.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_coefD.p, c->h_coef.data(), c->h_coef.size() * 8, hipMemcpyHostToDevice));
+    }
+    c->dirty = false;
+    return CSMC_OK;
+}
+
+DevPools pools(csmc_ctx *c, Batch *b) {
+    DevPools P;
+    P.lats = c->d_lat.p;
+    P.i32 = c->d_i32.p;
+    P.runs = c->d_runs.p;
+    P.coefF = c->d_coefF.p;
+    P.coefD = c->d_coefD.p;
+    P.sys = b ? b->sys_d.p : nullptr;
+    return P;
+}
+
+// append one lattice's device tables to the host-side pools
+void append_lattice(csmc_ctx *c, const HostLattice &L) {
+    LatDev d{};
+    const int N = (int)L.N;
+    d.N = N;
+    d.nb = L.nb;
+    d.K2 = L.K2;
+    d.K3 = L.K3;
+    d.K4 = L.K4;
+    d.ncol = L.ncol;
+    d.nruns = (int)L.runs.size();
+    d.has_self = L.has_self;
+    d.S = L.S;
+    auto &I = c->h_i32;
+    auto dev = [&](int ref) { return ref < 0 ? -1 : L.dev_of_ref[ref]; };
+    // neighbour tables in device order, slot-major
+    d.nbr_off = (int64_t)I.size();
+    for (int k = 0; k < L.K2; ++k)
+        for (int i = 0; i < N; ++i) I.push_back(dev(L.nbr2[(size_t)L.ref_of_dev[i] * L.K2 + k]));
+    for (int k = 0; k < L.K3; ++k)
+        for (int q = 0; q < 2; ++q)
+            for (int i = 0; i < N; ++i) I.push_back(dev(L.nbr3[((size_t)L.ref_of_dev[i] * L.K3 + k) * 2 + q]));
+    for (int k = 0; k < L.K4; ++k)
+        for (int q = 0; q < 3; ++q)
+            for (int i = 0; i < N; ++i) I.push_back(dev(L.nbr4[((size_t)L.ref_of_dev[i] * L.K4 + k) * 3 + q]));
+    d.ref_off = (int64_t)I.size();
+    for (int i = 0; i < N; ++i) I.push_back(L.ref_of_dev[i]);
+    d.sub_off = (int64_t)I.size();
+    for (int i = 0; i < N; ++i) I.push_back(L.basis_of(L.ref_of_dev[i]));
+    d.colrun_off = (int64_t)I.size();
+    for (int v : L.colrun) I.push_back(v);
+    d.run_off = (int64_t)c->h_runs.size();
+    for (auto &r : L.runs) c->h_runs.push_back(make_int4(r[0], r[1], r[2], r[3]));
+    // incidence lists: participant -> (owner, kind, role, slot)
+    std::vector<std::vector<Inc>> inc(N);
+    for (int i = 0; i < N; ++i) {
+        int rs = L.ref_of_dev[i];
+        for (int k = 0; k < L.K2; ++k) {
+            int j = L.nbr2[(size_t)rs * L.K2 + k];
+            if (j < 0) continue;
+            inc[i].push_back({i, 0 | (0 << 2) | (k << 4)});
+            inc[dev(j)].push_back({i, 0 | (1 << 2) | (k << 4)});
+        }
+        for (int k = 0; k < L.K3; ++k) {
+            int j = L.nbr3[((size_t)rs * L.K3 + k) * 2];
+            if (j < 0) continue;
+            inc[i].push_back({i, 1 | (0 << 2) | (k << 4)});
+            for (int q = 0; q < 2; ++q)
+                inc[dev(L.nbr3[((size_t)rs * L.K3 + k) * 2 + q])].push_back({i, 1 | ((q + 1) << 2) | (k << 4)});
+        }
+        for (int k = 0; k < L.K4; ++k) {
+            int j = L.nbr4[((size_t)rs * L.K4 + k) * 3];
+            if (j < 0) continue;
+            inc[i].push_back({i, 2 | (0 << 2) | (k << 4)});
+            for (int q = 0; q < 3; ++q)
+                inc[dev(L.nbr4[((size_t)rs * L.K4 + k) * 3 + q])].push_back({i, 2 | ((q + 1) << 2) | (k << 4)});
+        }
+    }
+    d.inc_off = (int64_t)I.size();
+    int64_t acc = 0;
+    for (int i = 0; i < N; ++i) {
+        I.push_back((int32_t)acc);
+        acc += (int64_t)inc[i].size();
+    }
+    I.push_back((int32_t)acc);
+    d.inc_n = acc;
+    for (int i = 0; i < N; ++i)
+        for (auto &e : inc[i]) {
+            I.push_back(e.owner);
+            I.push_back(e.code);
+        }
+    // gradient tables
+    d.grad_off = (int64_t)I.size();
+    for (int v : L.slot2) I.push_back(v);
+    for (int v : L.role2) I.push_back(v);
+    for (int v : L.slot3) I.push_back(v);
+    for (int v : L.role3) I.push_back(v);
+    for (int v : L.slot4) I.push_back(v);
+    for (int v : L.role4) I.push_back(v);
+    for (int v : L.fsrc) I.push_back(v);
+    for (int v : L.osrc) I.push_back(v);
+    int nf = (int)L.fb.size(), no = (int)L.ob.size();
+    I.push_back(0);
+    I.push_back(3 * nf);
+    I.push_back(3 * nf + 9 * no);
+    I.push_back(3 * nf + 9 * no + 9 * L.n2);
+    I.push_back(3 * nf + 9 * no + 9 * L.n2 + 27 * L.n3);
+    d.grad_len = L.grad_len;
+    d.coef_off = (int64_t)c->h_coef.size();
+    c->h_coef.insert(c->h_coef.end(), L.coef.begin(), L.coef.end());
+    c->h_lat.push_back(d);
+    c->dirty = true;
+}
+
+Batch *get_batch(csmc_ctx *c, int id) {
+    auto it = c->batches.find(id);
+    return it == c->batches.end() ? nullptr : it->second.get();
+}
+
+#define CHECK_CTX(c) \
+    if (!(c)) return fail(CSMC_EINVAL, "null context")
+#define GET_BATCH(c, id, b)                                                     \
+    Batch *b = get_batch(c, id);                                                \
+    if (!b) return fail(CSMC_ESTATE, "unknown batch id " + std::to_string(id))
+
+bool use_lds(csmc_ctx *c, Batch *b) {
+    if (c->no_lds) return false;
+    size_t need = 1024 + (size_t)b->maxN * b->spin_bytes();
+    return need <= (size_t)c->max_lds;
+}
+
+WGArgs wg_base(csmc_ctx *c, Batch *b, uint64_t seed) {
+    WGArgs a{};
+    a.P = pools(c, b);
+    a.spins = b->spins.p;
+    a.k0 = (uint32_t)seed;
+    a.k1 = (uint32_t)(seed >> 32);
+    a.mcount = b->mcount.p;
+    return a;
+}
+
+// fold finished event pairs into the running total (keeps the list short)
+int drain_events(csmc_ctx *c, bool wait) {
+    size_t k = 0;
+    for (; k < c->events.size(); ++k) {
+        auto &ev = c->events[k];
+        if (wait) HIPCHK(hipEventSynchronize(ev.second));
+        else if (hipEventQuery(ev.second) != hipSuccess) break;
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev.first, ev.second));
+        c->timed_ms += ms;
+        ++c->timed_launches;
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    c->events.erase(c->events.begin(), c->events.begin() + k);
+    return CSMC_OK;
+}
+
+int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
+    bool lds = use_lds(c, b);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, c->stream));
+    }
+    HIPCHK(launch_wg(b->prec, lds, b->self, b->gen, b->nsys, lds ? (size_t)b->maxN * b->spin_bytes() : 0, a,
+                     c->stream));
+    if (c->timing) {
+        HIPCHK(hipEventRecord(e1, c->stream));
+        c->events.push_back({e0, e1});
+        if (c->events.size() > 256) return drain_events(c, false);
+    }
+    return CSMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t csmc_version(void) { return 1; }
+
+const char *csmc_last_error(void) { return g_err.c_str(); }
+
+int32_t csmc_device_count(int32_t *n) {
+    int v = 0;
+    hipError_t e = hipGetDeviceCount(&v);
+    if (e != hipSuccess) v = 0;
+    if (n) *n = v;
+    return CSMC_OK;
+}
+
+int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
+    if (!out) return fail(CSMC_EINVAL, "null out pointer");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(CSMC_EHIP, "no HIP device available: the engine has no CPU fallback");
+    if (device < 0 || device >= n) return fail(CSMC_EINVAL, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    auto *c = new csmc_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(CSMC_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    c->max_lds = max_lds_bytes();
+    const char *nl = getenv("CSMC_NO_LDS");
+    c->no_lds = nl && nl[0] == '1';
+    const char *vb = getenv("CSMC_VERBOSE");
+    if (vb && vb[0] == '1') {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+            fprintf(stderr, "csmc: device %d %s (%s), %d CUs, max LDS per block %d B%s\n", device,
+                    prop.name, prop.gcnArchName, prop.multiProcessorCount, c->max_lds,
+                    c->no_lds ? " (LDS residency off)" : "");
+    }
+    *out = c;
+    return CSMC_OK;
+}
+
+int32_t csmc_ctx_destroy(csmc_ctx *c) {
+    CHECK_CTX(c);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->batches.clear();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CSMC_OK;
+}
+
+int32_t csmc_ctx_sync(csmc_ctx *c) {
+    CHECK_CTX(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_timing(csmc_ctx *c, int32_t enable) {
+    CHECK_CTX(c);
+    int rc = drain_events(c, true);
+    if (rc) return rc;
+    c->timing = enable != 0;
+    c->timed_ms = 0.0;
+    c->timed_launches = 0;
+    return CSMC_OK;
+}
+
+int32_t csmc_timing_read(csmc_ctx *c, double *total_ms, int64_t *launches) {
+    CHECK_CTX(c);
+    int rc = drain_events(c, true);
+    if (rc) return rc;
+    if (total_ms) *total_ms = c->timed_ms;
+    if (launches) *launches = c->timed_launches;
+    return CSMC_OK;
+}
+
+static void fill_info(const HostLattice &L, csmc_lattice_info *info) {
+    info->n_sites = L.N;
+    info->dim = L.dim;
+    info->n_basis = L.nb;
+    info->n2 = L.n2;
+    info->n3 = L.n3;
+    info->n4 = L.n4;
+    info->k2 = L.K2;
+    info->k3 = L.K3;
+    info->k4 = L.K4;
+    info->n_colours = L.ncol;
+    info->has_self = L.has_self;
+    info->S = L.S;
+}
+
+int32_t csmc_build_tables(const csmc_unit_cell *uc, const int32_t *shape, int32_t periodic,
+                          double S, csmc_lattice_info *info, double *pos, double *field,
+                          double *onsite, int64_t *bsite, double *bmat, int64_t *csite,
+                          double *ctens, int64_t *qsite, double *qtens, int32_t *colour) {
+    HostLattice L;
+    std::string err;
+    int rc = build_lattice(uc, shape, periodic, S, L, err);
+    if (rc) return fail(rc, err);
+    if (info) fill_info(L, info);
+    export_tables(L, pos, field, onsite, bsite, bmat, csite, ctens, qsite, qtens);
+    if (colour)
+        for (long i = 0; i < L.N; ++i) colour[i] = L.colour[i];
+    return CSMC_OK;
+}
+
+int32_t csmc_lattice_create(csmc_ctx *c, const csmc_unit_cell *uc, const int32_t *shape,
+                            int32_t periodic, double S, int32_t *lattice_id) {
+    CHECK_CTX(c);
+    auto L = std::make_unique<HostLattice>();
+    std::string err;
+    int rc = build_lattice(uc, shape, periodic, S, *L, err);
+    if (rc) return fail(rc, err);
+    append_lattice(c, *L);
+    c->lats.push_back(std::move(L));
+    if (lattice_id) *lattice_id = (int32_t)c->lats.size() - 1;
+    return CSMC_OK;
+}
+
+int32_t csmc_lattice_info_get(csmc_ctx *c, int32_t id, csmc_lattice_info *info) {
+    CHECK_CTX(c);
+    if (id < 0 || id >= (int)c->lats.size()) return fail(CSMC_ESTATE, "unknown lattice id");
+    fill_info(*c->lats[id], info);
+    return CSMC_OK;
+}
+
+int32_t csmc_lattice_tables(csmc_ctx *c, int32_t id, double *pos, double *field, double *onsite,
+                            int64_t *bsite, double *bmat, int64_t *csite, double *ctens,
+                            int64_t *qsite, double *qtens) {
+    CHECK_CTX(c);
+    if (id < 0 || id >= (int)c->lats.size()) return fail(CSMC_ESTATE, "unknown lattice id");
+    export_tables(*c->lats[id], pos, field, onsite, bsite, bmat, csite, ctens, qsite, qtens);
+    return CSMC_OK;
+}
+
+int32_t csmc_lattice_colouring(csmc_ctx *c, int32_t id, int32_t *colour) {
+    CHECK_CTX(c);
+    if (id < 0 || id >= (int)c->lats.size()) return fail(CSMC_ESTATE, "unknown lattice id");
+    const HostLattice &L = *c->lats[id];
+    for (long i = 0; i < L.N; ++i) colour[i] = L.colour[i];
+    return CSMC_OK;
+}
+
+int32_t csmc_lattice_update_couplings(csmc_ctx *c, int32_t id, const csmc_unit_cell *uc) {
+    CHECK_CTX(c);
+    if (id < 0 || id >= (int)c->lats.size()) return fail(CSMC_ESTATE, "unknown lattice id");
+    HostLattice &L = *c->lats[id];
+    std::string err;
+    int rc = update_couplings(uc, L, err);
+    if (rc) return fail(rc, err);
+    std::copy(L.coef.begin(), L.coef.end(), c->h_coef.begin() + c->h_lat[id].coef_off);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->dirty = true;
+    return CSMC_OK;
+}
+
+int32_t csmc_batch_create(csmc_ctx *c, int32_t n, const int32_t *lat_ids, int32_t prec,
+                          int32_t *batch_id) {
+    CHECK_CTX(c);
+    if (n < 1 || !lat_ids) return fail(CSMC_EINVAL, "batch needs at least one system");
+    if (prec != 32 && prec != 64) return fail(CSMC_EINVAL, "precision must be 32 or 64");
+    auto b = std::make_unique<Batch>();
+    b->prec = prec;
+    b->nsys = n;
+    int64_t off = 0;
+    for (int s = 0; s < n; ++s) {
+        int id = lat_ids[s];
+        if (id < 0 || id >= (int)c->lats.size())
+            return fail(CSMC_ESTATE, "unknown lattice id in batch");
+        const HostLattice &L = *c->lats[id];
+        SysDev sd;
+        sd.lat = id;
+        sd.stream = s;
+        sd.spin_off = off;
+        b->sys.push_back(sd);
+        b->lat.push_back(id);
+        off += L.N;
+        b->maxN = std::max<int>(b->maxN, (int)L.N);
+        b->self |= (L.has_self != 0);
+        b->gen |= (L.K3 > 0 || L.K4 > 0);
+    }
+    b->total = off;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(b->sys_d.alloc(n));
+    HIPCHK(hipMemcpy(b->sys_d.p, b->sys.data(), n * sizeof(SysDev), hipMemcpyHostToDevice));
+    HIPCHK(b->spins.alloc((size_t)off * b->spin_bytes()));
+    HIPCHK(hipMemset(b->spins.p, 0, (size_t)off * b->spin_bytes()));
+    HIPCHK(b->mcount.alloc(n));
+    HIPCHK(hipMemset(b->mcount.p, 0, n * sizeof(uint32_t)));
+    HIPCHK(b->T.alloc(n));
+    HIPCHK(b->sigma.alloc(n));
+    HIPCHK(b->acc.alloc(n));
+    HIPCHK(b->accU.alloc(n));
+    HIPCHK(b->stage.alloc(std::max<size_t>((size_t)off * 3, (size_t)8 * n)));
+    int id = c->next_batch++;
+    c->batches[id] = std::move(b);
+    if (batch_id) *batch_id = id;
+    return CSMC_OK;
+}
+
+int32_t csmc_batch_destroy(csmc_ctx *c, int32_t id) {
+    CHECK_CTX(c);
+    (void)hipStreamSynchronize(c->stream);
+    if (!c->batches.erase(id)) return fail(CSMC_ESTATE, "unknown batch id");
+    return CSMC_OK;
+}
+
+int32_t csmc_set_spins(csmc_ctx *c, int32_t id, int32_t sys, const double *spins) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (sys < 0 || sys >= b->nsys || !spins) return fail(CSMC_EINVAL, "bad system index or pointer");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    int N = (int)c->lats[b->lat[sys]]->N;
+    double *st = b->stage.p + 3 * b->sys[sys].spin_off;
+    HIPCHK(hipMemcpyAsync(st, spins, (size_t)N * 3 * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_scatter_spins(b->prec, pools(c, b), b->spins.p, sys, N, st, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_get_spins(csmc_ctx *c, int32_t id, int32_t sys, double *spins) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (sys < 0 || sys >= b->nsys || !spins) return fail(CSMC_EINVAL, "bad system index or pointer");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    int N = (int)c->lats[b->lat[sys]]->N;
+    double *st = b->stage.p + 3 * b->sys[sys].spin_off;
+    HIPCHK(launch_gather_spins(b->prec, pools(c, b), b->spins.p, sys, N, st, c->stream));
+    HIPCHK(hipMemcpyAsync(spins, st, (size_t)N * 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_randomize_spins(csmc_ctx *c, int32_t id, uint64_t seed) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    HIPCHK(launch_randomize(b->prec, pools(c, b), b->spins.p, b->nsys, b->maxN, (uint32_t)seed,
+                            (uint32_t)(seed >> 32), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_local_field(csmc_ctx *c, int32_t id, int32_t sys, double *H) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (sys < 0 || sys >= b->nsys || !H) return fail(CSMC_EINVAL, "bad system index or pointer");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    int N = (int)c->lats[b->lat[sys]]->N;
+    double *st = b->stage.p + 3 * b->sys[sys].spin_off;
+    HIPCHK(launch_site_eval(b->prec, pools(c, b), b->spins.p, sys, N, st, nullptr, c->stream));
+    HIPCHK(hipMemcpyAsync(H, st, (size_t)N * 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_site_energy(csmc_ctx *c, int32_t id, int32_t sys, double *e) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (sys < 0 || sys >= b->nsys || !e) return fail(CSMC_EINVAL, "bad system index or pointer");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    int N = (int)c->lats[b->lat[sys]]->N;
+    double *st = b->stage.p + 3 * b->sys[sys].spin_off;
+    HIPCHK(launch_site_eval(b->prec, pools(c, b), b->spins.p, sys, N, nullptr, st, c->stream));
+    HIPCHK(hipMemcpyAsync(e, st, (size_t)N * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_total_energy(csmc_ctx *c, int32_t id, double *E, double *parts) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!E) return fail(CSMC_EINVAL, "null output");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    double *dE = b->stage.p, *dP = b->stage.p + b->nsys;
+    HIPCHK(launch_energy(b->prec, pools(c, b), b->spins.p, b->nsys, dE, dP, c->stream));
+    HIPCHK(hipMemcpyAsync(E, dE, b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
+    if (parts) HIPCHK(hipMemcpyAsync(parts, dP, b->nsys * 5 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_magnetization(csmc_ctx *c, int32_t id, double *M) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!M) return fail(CSMC_EINVAL, "null output");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    HIPCHK(launch_magnet(b->prec, pools(c, b), b->spins.p, b->nsys, b->stage.p, c->stream));
+    HIPCHK(hipMemcpyAsync(M, b->stage.p, b->nsys * 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+static bool valid_kind(int k) { return k >= CSMC_OVERRELAX && k <= CSMC_DETERMINISTIC; }
+
+int32_t csmc_sweep(csmc_ctx *c, int32_t id, int32_t kind, const double *T, double *sigma,
+                   int32_t n_sweeps, uint64_t seed, double *accepted) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!valid_kind(kind)) return fail(CSMC_EINVAL, "unknown sweep kind");
+    if (n_sweeps < 0) return fail(CSMC_EINVAL, "n_sweeps < 0");
+    bool metro = kind != CSMC_OVERRELAX && kind != CSMC_DETERMINISTIC;
+    if (metro && !T) return fail(CSMC_EINVAL, "Metropolis sweeps need temperatures");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    if (T) HIPCHK(hipMemcpyAsync(b->T.p, T, b->nsys * 8, hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(hipMemsetAsync(b->T.p, 0, b->nsys * 8, c->stream));
+    if (sigma) HIPCHK(hipMemcpyAsync(b->sigma.p, sigma, b->nsys * 8, hipMemcpyHostToDevice, c->stream));
+    WGArgs a = wg_base(c, b, seed);
+    a.mode = WG_SWEEPS;
+    a.kind = kind;
+    a.n_sweeps = n_sweeps;
+    a.T_sys = b->T.p;
+    a.sigma_sys = sigma ? b->sigma.p : nullptr;
+    a.acc_sys = b->acc.p;
+    rc = run_wg(c, b, a);
+    if (rc) return rc;
+    if (accepted) HIPCHK(hipMemcpyAsync(accepted, b->acc.p, b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
+    if (sigma) HIPCHK(hipMemcpyAsync(sigma, b->sigma.p, b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_sweep_plain(csmc_ctx *c, int32_t id, int32_t kind, const double *T, double *sigma,
+                         int32_t n_sweeps, uint64_t seed, double *accepted) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!valid_kind(kind)) return fail(CSMC_EINVAL, "unknown sweep kind");
+    bool metro = kind != CSMC_OVERRELAX && kind != CSMC_DETERMINISTIC;
+    if (metro && !T) return fail(CSMC_EINVAL, "Metropolis sweeps need temperatures");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    std::vector<uint32_t> ctr(b->nsys);
+    HIPCHK(hipMemcpyAsync(ctr.data(), b->mcount.p, b->nsys * 4, hipMemcpyDeviceToHost, c->stream));
+    if (T) HIPCHK(hipMemcpyAsync(b->T.p, T, b->nsys * 8, hipMemcpyHostToDevice, c->stream));
+    std::vector<double> sg(b->nsys, 60.0);
+    if (sigma) sg.assign(sigma, sigma + b->nsys);
+    HIPCHK(hipMemcpyAsync(b->sigma.p, sg.data(), b->nsys * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(b->accU.p, 0, b->nsys * 4, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<double> acc(b->nsys, 0.0);
+    PassArgs a{};
+    a.P = pools(c, b);
+    a.spins = b->spins.p;
+    a.kind = kind;
+    a.k0 = (uint32_t)seed;
+    a.k1 = (uint32_t)(seed >> 32);
+    a.ctr = nullptr;
+    a.T_sys = b->T.p;
+    a.sigma_sys = b->sigma.p;
+    a.acc = b->accU.p;
+    for (int s = 0; s < b->nsys; ++s) {
+        const HostLattice &L = *c->lats[b->lat[s]];
+        a.sys0 = s;
+        for (int sw = 0; sw < n_sweeps; ++sw) {
+            a.ctr0 = ctr[s];
+            for (int col = 0; col < L.ncol; ++col) {
+                int n_col = L.runs[L.colrun[col + 1] - 1][1] - L.runs[L.colrun[col]][0];
+                a.colour = col;
+                HIPCHK(launch_pass(b->prec, b->self, b->gen, 1, n_col, a, c->stream));
+            }
+            if (metro) {
+                ++ctr[s];
+                if (kind == CSMC_CONE_ADAPTIVE) {
+                    // metropolis_adaptive! width update after the sweep
+                    uint32_t got = 0;
+                    HIPCHK(hipMemcpyAsync(&got, b->accU.p + s, 4, hipMemcpyDeviceToHost, c->stream));
+                    HIPCHK(hipMemsetAsync(b->accU.p + s, 0, 4, c->stream));
+                    HIPCHK(hipStreamSynchronize(c->stream));
+                    acc[s] += got;
+                    double av = (double)got / (double)L.N;
+                    double f = 0.5 / std::max(1.0 - av, 0.05);
+                    sg[s] = std::min(std::max(sg[s] * f, 0.0), 100.0);
+                    HIPCHK(hipMemcpyAsync(b->sigma.p + s, &sg[s], 8, hipMemcpyHostToDevice, c->stream));
+                    HIPCHK(hipStreamSynchronize(c->stream));
+                }
+            }
+        }
+    }
+    std::vector<uint32_t> got(b->nsys);
+    HIPCHK(hipMemcpyAsync(got.data(), b->accU.p, b->nsys * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(b->mcount.p, ctr.data(), b->nsys * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < b->nsys; ++s) acc[s] += got[s];
+    if (accepted) std::copy(acc.begin(), acc.end(), accepted);
+    if (sigma) std::copy(sg.begin(), sg.end(), sigma);
+    return CSMC_OK;
+}
+
+int32_t csmc_set_counters(csmc_ctx *c, int32_t id, const uint32_t *counters) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    HIPCHK(hipMemcpyAsync(b->mcount.p, counters, b->nsys * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_get_counters(csmc_ctx *c, int32_t id, uint32_t *counters) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    HIPCHK(hipMemcpyAsync(counters, b->mcount.p, b->nsys * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_anneal(csmc_ctx *c, int32_t id, const double *temps, int32_t n_temps,
+                    int64_t t_th, int64_t or_rate, int32_t alg, double sigma0, uint64_t seed,
+                    double *accepted) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (n_temps < 0 || (n_temps && !temps)) return fail(CSMC_EINVAL, "bad temperature list");
+    if (alg < CSMC_METROPOLIS || alg > CSMC_CONE_ADAPTIVE) return fail(CSMC_EINVAL, "alg must be a Metropolis kind");
+    if (or_rate < 0) return fail(CSMC_EINVAL, "overrelaxation_rate < 0");
+    if (n_temps == 0) return CSMC_OK;
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    DevBuf<double> dt, dacc;
+    HIPCHK(dt.alloc(n_temps));
+    HIPCHK(dacc.alloc((size_t)n_temps * b->nsys));
+    HIPCHK(hipMemcpyAsync(dt.p, temps, n_temps * 8, hipMemcpyHostToDevice, c->stream));
+    WGArgs a = wg_base(c, b, seed);
+    a.mode = WG_ANNEAL;
+    a.kind = alg;
+    a.temps = dt.p;
+    a.n_temps = n_temps;
+    a.t_th = t_th;
+    a.or_rate = or_rate;
+    a.sigma0 = sigma0;
+    a.acc_T = dacc.p;
+    rc = run_wg(c, b, a);
+    if (rc) return rc;
+    if (accepted)
+        HIPCHK(hipMemcpyAsync(accepted, dacc.p, (size_t)n_temps * b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_pt_init(csmc_ctx *c, int32_t id, const double *temps, const csmc_pt_params *p,
+                     double sigma0, uint64_t seed) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!temps || !p) return fail(CSMC_EINVAL, "null temperatures or parameters");
+    if (p->alg < CSMC_METROPOLIS || p->alg > CSMC_CONE_ADAPTIVE)
+        return fail(CSMC_EINVAL, "alg must be a Metropolis kind");
+    if (p->overrelaxation_rate < 0 || p->t_thermalization < 0 || p->t_measurement < 0)
+        return fail(CSMC_EINVAL, "negative schedule parameter");
+    if (b->nsys > 1 && p->swap_rate <= 0)
+        return fail(CSMC_EINVAL, "swap_rate must be > 0 (the reference divides by it)");
+    if (p->measure && p->t_measurement > 0 && p->probe_rate <= 0)
+        return fail(CSMC_EINVAL, "probe_rate must be > 0 (the reference divides by it)");
+    for (int s = 1; s < b->nsys; ++s)
+        if (b->lat[s] != b->lat[0]) return fail(CSMC_EINVAL, "PT replicas must share one lattice");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    PTHost &pt = b->pt;
+    pt.p = *p;
+    pt.seed = seed;
+    pt.sweep = 0;
+    pt.mid = false;
+    pt.total = p->t_thermalization + p->t_measurement;
+    pt.dosweep = p->overrelaxation_rate == 0 ? 1 : p->overrelaxation_rate;
+    pt.temps.assign(temps, temps + b->nsys);
+    int R = b->nsys;
+    HIPCHK(pt.temps_d.alloc(R));
+    HIPCHK(pt.E.alloc(R));
+    HIPCHK(pt.sigma.alloc(R));
+    HIPCHK(pt.acc_local.alloc(R));
+    HIPCHK(pt.acc_exch.alloc(R));
+    HIPCHK(pt.bins.alloc((size_t)R * 2 * BIN_DOUBLES));
+    HIPCHK(pt.slot_of_sys.alloc(R));
+    HIPCHK(pt.sys_of_slot.alloc(R));
+    HIPCHK(pt.done.alloc(1));
+    std::vector<int32_t> ident(R);
+    for (int r = 0; r < R; ++r) ident[r] = r;
+    std::vector<double> sg(R, sigma0);
+    HIPCHK(hipMemcpyAsync(pt.temps_d.p, temps, R * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(pt.slot_of_sys.p, ident.data(), R * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(pt.sys_of_slot.p, ident.data(), R * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(pt.sigma.p, sg.data(), R * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(pt.acc_local.p, 0, R * 8, c->stream));
+    HIPCHK(hipMemsetAsync(pt.acc_exch.p, 0, R * 8, c->stream));
+    HIPCHK(hipMemsetAsync(pt.bins.p, 0, (size_t)R * 2 * BIN_DOUBLES * 8, c->stream));
+    HIPCHK(hipMemsetAsync(pt.done.p, 0, 4, c->stream));
+    // E = total_energy(mc.lattice) (monte_carlo.jl:265)
+    HIPCHK(launch_energy(b->prec, pools(c, b), b->spins.p, R, pt.E.p, nullptr, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    pt.init = true;
+    return CSMC_OK;
+}
+
+int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped, int32_t *event) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    PTHost &pt = b->pt;
+    if (!pt.init) return fail(CSMC_ESTATE, "csmc_pt_init was not called for this batch");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    const csmc_pt_params &p = pt.p;
+    const int R = b->nsys;
+    if (until < 0 || until > pt.total) until = pt.total;
+    auto exchange_at = [&](int64_t s) { return R > 1 && s % pt.dosweep == 0 && s % p.swap_rate == 0; };
+    auto checkpoint_at = [&](int64_t s) {
+        return p.checkpoint_rate > 0 && s >= p.t_thermalization && s % p.checkpoint_rate == 0;
+    };
+    auto report_at = [&](int64_t s1) { return p.report_interval > 0 && s1 % p.report_interval == 0; };
+    const int64_t max_seg = 2048;
+    WGArgs a = wg_base(c, b, pt.seed);
+    a.mode = WG_PT;
+    a.pt.R = R;
+    a.pt.alg = p.alg;
+    a.pt.t_th = p.t_thermalization;
+    a.pt.t_meas = p.t_measurement;
+    a.pt.probe = p.probe_rate > 0 ? p.probe_rate : 1;
+    a.pt.swap = p.swap_rate > 0 ? p.swap_rate : 1;
+    a.pt.or_rate = p.overrelaxation_rate;
+    a.pt.dosweep = pt.dosweep;
+    a.pt.measure = p.measure && p.probe_rate > 0;
+    a.pt.seed = pt.seed;
+    a.pt.temps = pt.temps_d.p;
+    a.pt.slot_of_sys = pt.slot_of_sys.p;
+    a.pt.sys_of_slot = pt.sys_of_slot.p;
+    a.pt.E = pt.E.p;
+    a.pt.sigma = pt.sigma.p;
+    a.pt.acc_local = pt.acc_local.p;
+    a.pt.acc_exch = pt.acc_exch.p;
+    a.pt.binners = pt.bins.p;
+    a.pt.done = pt.done.p;
+    int32_t ev = 0;
+    while (true) {
+        if (pt.sweep >= pt.total) {
+            ev = 3;
+            break;
+        }
+        if (pt.sweep >= until && !pt.mid) {
+            ev = 0;
+            break;
+        }
+        // plan one segment
+        int64_t s = pt.sweep;
+        bool begin_mid = pt.mid, end_mid = false, exch = false;
+        while (true) {
+            bool head = !(s == pt.sweep && begin_mid);
+            if (head && (exchange_at(s) || checkpoint_at(s))) {
+                end_mid = true;
+                exch = exchange_at(s);
+                break;
+            }
+            if (report_at(s + 1) || s + 1 >= pt.total || s + 1 >= until || s - pt.sweep + 1 >= max_seg) break;
+            ++s;
+        }
+        a.seg_begin = pt.sweep;
+        a.seg_end = s;
+        a.begin_mid = begin_mid;
+        a.end_mid = end_mid;
+        a.exchange = exch;
+        rc = run_wg(c, b, a);
+        if (rc) return rc;
+        if (end_mid) {
+            pt.sweep = s;
+            pt.mid = true;
+            if (checkpoint_at(s)) {
+                ev = 1;
+                break;
+            }
+        } else {
+            pt.sweep = s + 1;
+            pt.mid = false;
+            if (report_at(s + 1)) {
+                ev = 2;
+                break;
+            }
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (stopped) *stopped = pt.sweep;
+    if (event) *event = (pt.sweep >= pt.total && !pt.mid) ? 3 : ev;
+    return CSMC_OK;
+}
+
+int32_t csmc_pt_state(csmc_ctx *c, int32_t id, int32_t *sys_of_slot, double *E, double *acc_local,
+                      double *acc_exch, double *sigma) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    PTHost &pt = b->pt;
+    if (!pt.init) return fail(CSMC_ESTATE, "csmc_pt_init was not called for this batch");
+    int R = b->nsys;
+    std::vector<int32_t> sos(R);
+    std::vector<double> Es(R);
+    HIPCHK(hipMemcpyAsync(sos.data(), pt.sys_of_slot.p, R * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(Es.data(), pt.E.p, R * 8, hipMemcpyDeviceToHost, c->stream));
+    if (acc_local) HIPCHK(hipMemcpyAsync(acc_local, pt.acc_local.p, R * 8, hipMemcpyDeviceToHost, c->stream));
+    if (acc_exch) HIPCHK(hipMemcpyAsync(acc_exch, pt.acc_exch.p, R * 8, hipMemcpyDeviceToHost, c->stream));
+    if (sigma) HIPCHK(hipMemcpyAsync(sigma, pt.sigma.p, R * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (sys_of_slot) std::copy(sos.begin(), sos.end(), sys_of_slot);
+    if (E)
+        for (int r = 0; r < R; ++r) E[r] = Es[sos[r]];
+    return CSMC_OK;
+}
+
+namespace {
+// log-binning readout: level l holds count, sum[2], sum2[2][2], pend[2], has
+double bin_var(const double *st, int lvl, const double g[2]) {
+    const double *q = st + lvl * 10;
+    double n = q[0];
+    if (n < 2.0) return 0.0;
+    double m[2] = {q[1] / n, q[2] / n};
+    double v = 0.0;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            double cij = (q[3 + 2 * i + j] / n - m[i] * m[j]) * n / (n - 1.0);
+            v += g[i] * cij * g[j];
+        }
+    return v;
+}
+int bin_level(const double *st) {
+    int l = 0;
+    for (int k = 0; k < BIN_LVLS; ++k)
+        if (st[k * 10] >= 32.0) l = k;
+    return l;
+}
+double bin_err(const double *st, const double g[2]) {
+    int l = bin_level(st);
+    double n = st[l * 10];
+    if (n <= 0) return 0.0;
+    return std::sqrt(std::fabs(bin_var(st, l, g) / n));
+}
+}  // namespace
+
+int32_t csmc_pt_observables(csmc_ctx *c, int32_t id, double *out) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    PTHost &pt = b->pt;
+    if (!pt.init) return fail(CSMC_ESTATE, "csmc_pt_init was not called for this batch");
+    int R = b->nsys;
+    std::vector<double> bins((size_t)R * 2 * BIN_DOUBLES);
+    HIPCHK(hipMemcpyAsync(bins.data(), pt.bins.p, bins.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int r = 0; r < R; ++r) {
+        const double *be = &bins[(size_t)r * 2 * BIN_DOUBLES], *bm = be + BIN_DOUBLES;
+        double N = (double)c->lats[b->lat[0]]->N, T = pt.temps[r];
+        double ne = be[0], nm = bm[0];
+        double me[2] = {ne > 0 ? be[1] / ne : 0.0, ne > 0 ? be[2] / ne : 0.0};
+        double mm[2] = {nm > 0 ? bm[1] / nm : 0.0, nm > 0 ? bm[2] / nm : 0.0};
+        // specific_heat / susceptibility (observables.jl:36-63)
+        double C = 1.0 / (T * T) * (me[1] - me[0] * me[0]) / N;
+        double gC[2] = {-2.0 * 1.0 / (T * T) * me[0] / N, 1.0 / (T * T) / N};
+        double X = 1.0 / T * (mm[1] - mm[0] * mm[0]) / N;
+        double gX[2] = {-2.0 * 1.0 / T * mm[0] / N, 1.0 / T / N};
+        double g0[2] = {1.0, 0.0};
+        double *o = out + (size_t)r * 8;
+        o[0] = C;
+        o[1] = bin_err(be, gC);
+        o[2] = X;
+        o[3] = bin_err(bm, gX);
+        o[4] = me[0];
+        o[5] = bin_err(be, g0);
+        o[6] = mm[0];
+        o[7] = bin_err(bm, g0);
+    }
+    return CSMC_OK;
+}
+
+int32_t csmc_energy_grad_spins(csmc_ctx *c, int32_t id, double *dEdS) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!dEdS) return fail(CSMC_EINVAL, "null output");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    HIPCHK(launch_grad_spins(b->prec, pools(c, b), b->spins.p, b->nsys, b->maxN, b->stage.p, c->stream));
+    HIPCHK(hipMemcpyAsync(dEdS, b->stage.p, (size_t)b->total * 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_energy_grad_couplings(csmc_ctx *c, int32_t id, double *dEdJ) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!dEdJ) return fail(CSMC_EINVAL, "null output");
+    int gl = c->lats[b->lat[0]]->grad_len;
+    for (int s = 1; s < b->nsys; ++s)
+        if (c->lats[b->lat[s]]->grad_len != gl)
+            return fail(CSMC_EINVAL, "coupling gradients need systems with one term layout");
+    if (gl == 0) return CSMC_OK;
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    DevBuf<double> d;
+    HIPCHK(d.alloc((size_t)gl * b->nsys));
+    HIPCHK(launch_grad_couplings(b->prec, pools(c, b), b->spins.p, b->nsys, gl, d.p, c->stream));
+    HIPCHK(hipMemcpyAsync(dEdJ, d.p, (size_t)gl * b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_gradient_descent(csmc_ctx *c, int32_t id, double eta, int32_t n_steps) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    for (int k = 0; k < n_steps; ++k) {
+        HIPCHK(launch_grad_spins(b->prec, pools(c, b), b->spins.p, b->nsys, b->maxN, b->stage.p, c->stream));
+        HIPCHK(launch_gd(b->prec, pools(c, b), b->spins.p, b->nsys, b->maxN, b->stage.p, eta, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+}  // extern "C"

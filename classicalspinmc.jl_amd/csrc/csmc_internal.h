@@ -1,0 +1,85 @@
+// csmc_internal.h -- structures shared by the host runtime and the gfx950
+// kernels of libcsmc.
+//
+// Device data layout (one HBM pool per kind, all lattices concatenated):
+//   i32 pool   per lattice: nbr2[K2][N], nbr3[K3][2][N], nbr4[K4][3][N]
+//              (device site indices, -1 = absent), ref[N] (reference index of
+//              each device site), sub[N] (basis of each device site, 0-based),
+//              colrun[ncol+1] (run index boundaries per colour), incidence CSR
+//              for the spin gradient.
+//   run pool   int4 {start, end, sub, colour}: maximal ranges of device sites
+//              that share colour and basis.  Device sites are ordered by
+//              (colour, basis, reference index), so every colour class is one
+//              contiguous range and each run has one coefficient set.
+//   coef pool  per lattice (float and double copies at the same offset):
+//              onsite[nb][9], field[nb][3], c2[nb][K2][9], c3[nb][K3][27],
+//              c4[nb][K4][81]: the couplings of each (basis, slot) with the
+//              role permutation of lattice.jl:184-193, 215-227, 251-272
+//              already applied.
+//   spins      spin4<R> {x,y,z,pad} per device site, systems back to back.
+#pragma once
+#include <stdint.h>
+
+#include "../../include/csmc.h"
+
+namespace csmc {
+
+struct LatDev {
+    int32_t N, nb, K2, K3, K4, ncol, nruns, has_self;
+    int64_t nbr_off;     // i32 pool
+    int64_t ref_off;     // i32 pool
+    int64_t sub_off;     // i32 pool
+    int64_t colrun_off;  // i32 pool, run indices are absolute (into run pool)
+    int64_t run_off;     // run pool (absolute index of first run)
+    int64_t coef_off;    // coef pools
+    int64_t inc_off;     // i32 pool: incidence offsets [N+1] then entries
+    int64_t inc_n;       // number of incidence entries
+    int64_t grad_off;    // i32 pool: slot->gradient index tables
+    int32_t grad_len;    // coupling-gradient length per system
+    int32_t pad_;
+    double S;
+};
+
+struct SysDev {
+    int32_t lat;      // lattice id
+    int32_t stream;   // Philox stream word of this system
+    int64_t spin_off; // first spin4 of this system
+};
+
+// incidence entry: owner device site and a packed (kind, slot, role)
+//   code = kind(2 bits: 0 bil, 1 cub, 2 qua) | role << 2 (2 bits) | slot << 4
+struct Inc {
+    int32_t owner;
+    int32_t code;
+};
+
+// Philox counter tags (fourth counter word)
+constexpr uint32_t TAG_METRO = 0x4D455452u;  // "METR"
+constexpr uint32_t TAG_EXCH = 0x45584348u;   // "EXCH"
+constexpr uint32_t TAG_INIT = 0x494E4954u;   // "INIT"
+
+// Parallel-tempering device state (one per PT batch).
+struct PTDev {
+    int32_t R;            // replicas (= temperature slots)
+    int32_t alg;          // Metropolis kind
+    int64_t t_th, t_meas, probe, swap, or_rate, dosweep;
+    int32_t measure;
+    int32_t pad_;
+    uint64_t seed;
+    // device arrays
+    const double *temps;   // [R] temperature of slot
+    int32_t *slot_of_sys;  // [R]
+    int32_t *sys_of_slot;  // [R]
+    double *E;             // [R] per system energy (FP64, last Metropolis sweep)
+    double *sigma;         // [R] per slot
+    double *acc_local;     // [R] per slot
+    double *acc_exch;      // [R] per slot
+    double *binners;       // [R][2][BIN_DOUBLES] per slot: E then |M|
+    uint32_t *done;        // [1] WG completion counter for the exchange
+};
+
+constexpr int BIN_LVLS = 32;
+// per level: count, sum[2], sum2[4], pend[2], has  -> 10 doubles
+constexpr int BIN_DOUBLES = BIN_LVLS * 10;
+
+}  // namespace csmc

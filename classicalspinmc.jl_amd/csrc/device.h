@@ -1,0 +1,347 @@
+// device.h -- gfx950 device building blocks of libcsmc: spin storage,
+// Philox4x32-10, the local field / site energy of ClassicalSpinMC.jl's
+// hamiltonian.jl and the single-site updates of its sweeps.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "csmc_internal.h"
+
+namespace csmc {
+
+template <class R>
+struct alignas(4 * sizeof(R)) spin4 {
+    R x, y, z, w;
+};
+
+// View of one lattice's device tables (see csmc_internal.h for the layout).
+template <class R>
+struct LatView {
+    int N, nb, K2, K3, K4, ncol, has_self;
+    const int32_t *nbr2, *nbr3, *nbr4, *ref, *sub, *colrun;
+    const int4 *runs;
+    const R *onsite, *field, *c2, *c3, *c4;
+    R S;
+};
+
+template <class R>
+__device__ __forceinline__ LatView<R> lat_view(const LatDev &d, const int32_t *i32,
+                                               const int4 *runs, const R *coef) {
+    LatView<R> v;
+    v.N = d.N;
+    v.nb = d.nb;
+    v.K2 = d.K2;
+    v.K3 = d.K3;
+    v.K4 = d.K4;
+    v.ncol = d.ncol;
+    v.has_self = d.has_self;
+    v.nbr2 = i32 + d.nbr_off;
+    v.nbr3 = v.nbr2 + (int64_t)d.K2 * d.N;
+    v.nbr4 = v.nbr3 + (int64_t)d.K3 * 2 * d.N;
+    v.ref = i32 + d.ref_off;
+    v.sub = i32 + d.sub_off;
+    v.colrun = i32 + d.colrun_off;
+    v.runs = runs;
+    const R *c = coef + d.coef_off;
+    v.onsite = c;
+    v.field = c + d.nb * 9;
+    v.c2 = v.field + d.nb * 3;
+    v.c3 = v.c2 + (int64_t)d.nb * d.K2 * 9;
+    v.c4 = v.c3 + (int64_t)d.nb * d.K3 * 27;
+    v.S = (R)d.S;
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    }
+    return c;
+}
+
+// uniforms in (0,1)
+__device__ __forceinline__ double u01d(uint32_t x) {
+    return ((double)x + 0.5) * 2.3283064365386963e-10;
+}
+__device__ __forceinline__ float u01f(uint32_t x) {
+    return ((float)(x >> 8) + 0.5f) * 5.9604644775390625e-08f;
+}
+
+template <class R>
+struct Math;
+
+template <>
+struct Math<double> {
+    static __device__ __forceinline__ double u01(uint32_t x) { return u01d(x); }
+    // random_spin_orientation (lattice.jl:306-311)
+    static __device__ __forceinline__ void rand_spin(double S, double u1, double u2, double *o) {
+        double phi = 2.0 * M_PI * u1;
+        double z = 2.0 * u2 - 1.0;
+        double r = sqrt(1.0 - z * z);
+        double sn, cs;
+        sincos(phi, &sn, &cs);
+        o[0] = S * (r * cs);
+        o[1] = S * (r * sn);
+        o[2] = S * z;
+    }
+    static __device__ __forceinline__ double boltz(double dE, double T) { return exp(-dE / T); }
+    static __device__ __forceinline__ double rsqrt_(double x) { return 1.0 / sqrt(x); }
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+};
+
+template <>
+struct Math<float> {
+    static __device__ __forceinline__ float u01(uint32_t x) { return u01f(x); }
+    // same formula; v_sin/v_cos take the angle in turns, so phi = 2 pi u1
+    // needs no multiply and no range reduction
+    static __device__ __forceinline__ void rand_spin(float S, float u1, float u2, float *o) {
+        float z = 2.0f * u2 - 1.0f;
+        float r = __builtin_sqrtf(fmaxf(1.0f - z * z, 0.0f));
+        o[0] = S * (r * __builtin_amdgcn_cosf(u1));
+        o[1] = S * (r * __builtin_amdgcn_sinf(u1));
+        o[2] = S * z;
+    }
+    static __device__ __forceinline__ float boltz(float dE, float T) { return __expf(-dE / T); }
+    static __device__ __forceinline__ float rsqrt_(float x) { return __builtin_amdgcn_rsqf(x); }
+    static __device__ __forceinline__ float div(float a, float b) {
+        float r = __builtin_amdgcn_rcpf(b);
+        r = r * (2.0f - b * r);  // one Newton step
+        return a * r;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Interaction field F_i = sum_n J_n s_j + sum C s_j s_k + sum R s_j s_k s_l
+// (the non-onsite, non-Zeeman part of get_local_field, hamiltonian.jl:24-65).
+// `ld(j)` returns the spin of device site j; basis b is the site's basis.
+// A coefficient read is uniform across the wave whenever b is.
+// ---------------------------------------------------------------------------
+template <class R, class A, bool GEN = true, class LD>
+__device__ __forceinline__ void interaction_field(const LatView<R> &L, int b, int i, LD &&ld,
+                                                  A F[3]) {
+    F[0] = F[1] = F[2] = A(0);
+    for (int k = 0; k < L.K2; ++k) {
+        int j = L.nbr2[(int64_t)k * L.N + i];
+        if (j >= 0) {
+            spin4<R> sj = ld(j);
+            const R *M = L.c2 + (b * L.K2 + k) * 9;
+            F[0] += (A)M[0] * sj.x + (A)M[1] * sj.y + (A)M[2] * sj.z;
+            F[1] += (A)M[3] * sj.x + (A)M[4] * sj.y + (A)M[5] * sj.z;
+            F[2] += (A)M[6] * sj.x + (A)M[7] * sj.y + (A)M[8] * sj.z;
+        }
+    }
+    if (!GEN) return;
+    for (int k = 0; k < L.K3; ++k) {
+        int j = L.nbr3[((int64_t)k * 2) * L.N + i];
+        int kk = L.nbr3[((int64_t)k * 2 + 1) * L.N + i];
+        if (j >= 0) {
+            spin4<R> a = ld(j), c = ld(kk);
+            A sj[3] = {a.x, a.y, a.z}, sk[3] = {c.x, c.y, c.z};
+            const R *C = L.c3 + (b * L.K3 + k) * 27;
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                A acc = 0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    A t = 0;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) t += (A)C[p + 3 * q + 9 * r] * sk[r];
+                    acc += t * sj[q];
+                }
+                F[p] += acc;
+            }
+        }
+    }
+    for (int k = 0; k < L.K4; ++k) {
+        int j = L.nbr4[((int64_t)k * 3) * L.N + i];
+        int kk = L.nbr4[((int64_t)k * 3 + 1) * L.N + i];
+        int l = L.nbr4[((int64_t)k * 3 + 2) * L.N + i];
+        if (j >= 0) {
+            spin4<R> a = ld(j), c = ld(kk), e = ld(l);
+            A sj[3] = {a.x, a.y, a.z}, sk[3] = {c.x, c.y, c.z}, sl[3] = {e.x, e.y, e.z};
+            A w[9];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int t = 0; t < 3; ++t) w[r + 3 * t] = sk[r] * sl[t];
+            const R *Q = L.c4 + (b * L.K4 + k) * 81;
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                A acc = 0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    A t = 0;
+#pragma unroll
+                    for (int rt = 0; rt < 9; ++rt) t += (A)Q[p + 3 * q + 9 * rt] * w[rt];
+                    acc += t * sj[q];
+                }
+                F[p] += acc;
+            }
+        }
+    }
+}
+
+// get_local_field (hamiltonian.jl:3-67): H = 2 O s + F - h
+template <class R, class A>
+__device__ __forceinline__ void local_field(const LatView<R> &L, int b, const spin4<R> &s,
+                                            const A F[3], A H[3]) {
+    const R *O = L.onsite + b * 9;
+    const R *h = L.field + b * 3;
+    A sx = s.x, sy = s.y, sz = s.z;
+    H[0] = A(2) * ((A)O[0] * sx + (A)O[1] * sy + (A)O[2] * sz) + F[0] - (A)h[0];
+    H[1] = A(2) * ((A)O[3] * sx + (A)O[4] * sy + (A)O[5] * sz) + F[1] - (A)h[1];
+    H[2] = A(2) * ((A)O[6] * sx + (A)O[7] * sy + (A)O[8] * sz) + F[2] - (A)h[2];
+}
+
+// s.O.s
+template <class R, class A>
+__device__ __forceinline__ A onsite_energy(const LatView<R> &L, int b, A sx, A sy, A sz) {
+    const R *O = L.onsite + b * 9;
+    return sx * ((A)O[0] * sx + (A)O[1] * sy + (A)O[2] * sz) +
+           sy * ((A)O[3] * sx + (A)O[4] * sy + (A)O[5] * sz) +
+           sz * ((A)O[6] * sx + (A)O[7] * sy + (A)O[8] * sz);
+}
+
+// ---------------------------------------------------------------------------
+// single-site updates
+// ---------------------------------------------------------------------------
+
+// overrelaxation! body (monte_carlo.jl:128-137)
+template <class R, bool GEN, class SP>
+__device__ __forceinline__ void or_update(const LatView<R> &L, SP *S, int b, int i) {
+    spin4<R> s = S[i];
+    R F[3], H[3];
+    interaction_field<R, R, GEN>(L, b, i, [&](int j) { return S[j]; }, F);
+    local_field<R, R>(L, b, s, F, H);
+    if (H[0] == R(0) && H[1] == R(0) && H[2] == R(0)) return;
+    R hh = H[0] * H[0] + H[1] * H[1] + H[2] * H[2];
+    R proj = Math<R>::div(R(2) * (s.x * H[0] + s.y * H[1] + s.z * H[2]), hh);
+    spin4<R> n;
+    n.x = -s.x + proj * H[0];
+    n.y = -s.y + proj * H[1];
+    n.z = -s.z + proj * H[2];
+    n.w = R(0);
+    S[i] = n;
+}
+
+// deterministic_updates! body (monte_carlo.jl:203-209)
+template <class R, bool GEN, class SP>
+__device__ __forceinline__ void det_update(const LatView<R> &L, SP *S, int b, int i) {
+    spin4<R> s = S[i];
+    R F[3], H[3];
+    interaction_field<R, R, GEN>(L, b, i, [&](int j) { return S[j]; }, F);
+    local_field<R, R>(L, b, s, F, H);
+    if (H[0] == R(0) && H[1] == R(0) && H[2] == R(0)) return;
+    R nrm = sqrt(H[0] * H[0] + H[1] * H[1] + H[2] * H[2]);
+    spin4<R> n;
+    n.x = -H[0] / nrm * L.S;
+    n.y = -H[1] / nrm * L.S;
+    n.z = -H[2] / nrm * L.S;
+    n.w = R(0);
+    S[i] = n;
+}
+
+// One Metropolis proposal (metropolis.jl:70-78 with calculate_energy_diff!,
+// metropolis.jl:94-110).  kind: CSMC_METROPOLIS draws a fresh direction,
+// the cone kinds draw s + sigma*r renormalised (gaussian_move,
+// metropolis.jl:84-87).  Counter = (reference site, stream, ctr, METR).
+// SELF: some term of this lattice names the site itself, so the new energy
+// is evaluated with the candidate substituted in those slots as the
+// reference does (it writes the spin before calling energy()).
+template <class R, bool SELF, bool GEN, class SP>
+__device__ __forceinline__ int metro_update(const LatView<R> &L, SP *S, int b, int i, R T,
+                                            int kind, R sigma, uint32_t k0, uint32_t k1,
+                                            uint32_t stream, uint32_t ctr) {
+    spin4<R> s = S[i];
+    uint4 r = philox4x32_10(make_uint4((uint32_t)L.ref[i], stream, ctr, TAG_METRO), k0, k1);
+    R u1 = Math<R>::u01(r.x), u2 = Math<R>::u01(r.y), u3 = Math<R>::u01(r.z);
+    R rs[3];
+    Math<R>::rand_spin(L.S, u1, u2, rs);
+    spin4<R> n;
+    if (kind == 1) {
+        n.x = rs[0];
+        n.y = rs[1];
+        n.z = rs[2];
+    } else {
+        R a = s.x + sigma * rs[0], c = s.y + sigma * rs[1], d = s.z + sigma * rs[2];
+        R inv = Math<R>::rsqrt_(a * a + c * c + d * d) * L.S;
+        n.x = a * inv;
+        n.y = c * inv;
+        n.z = d * inv;
+    }
+    n.w = R(0);
+    const R *h = L.field + b * 3;
+    R dE;
+    if (!SELF) {
+        R F[3];
+        interaction_field<R, R, GEN>(L, b, i, [&](int j) { return S[j]; }, F);
+        R eo_new = onsite_energy<R, R>(L, b, n.x, n.y, n.z);
+        R eo_old = onsite_energy<R, R>(L, b, s.x, s.y, s.z);
+        dE = (eo_new - eo_old) + (n.x - s.x) * (F[0] - (R)h[0]) + (n.y - s.y) * (F[1] - (R)h[1]) +
+             (n.z - s.z) * (F[2] - (R)h[2]);
+    } else {
+        R Fo[3], Fn[3];
+        interaction_field<R, R, GEN>(L, b, i, [&](int j) { return S[j]; }, Fo);
+        interaction_field<R, R, GEN>(L, b, i, [&](int j) { return j == i ? n : (spin4<R>)S[j]; }, Fn);
+        R eo = onsite_energy<R, R>(L, b, s.x, s.y, s.z) + s.x * Fo[0] + s.y * Fo[1] + s.z * Fo[2] -
+               (s.x * h[0] + s.y * h[1] + s.z * h[2]);
+        R en = onsite_energy<R, R>(L, b, n.x, n.y, n.z) + n.x * Fn[0] + n.y * Fn[1] + n.z * Fn[2] -
+               (n.x * h[0] + n.y * h[1] + n.z * h[2]);
+        dE = en - eo;
+    }
+    int acc = (dE < R(0)) ? 1 : (u3 < Math<R>::boltz(dE, T));
+    if (acc) {
+        S[i] = n;
+    } else if (sizeof(R) == 4) {
+        // FP32 state: pull |s| back to S on every visit so reflections
+        // cannot accumulate rounding drift
+        R inv = Math<R>::rsqrt_(s.x * s.x + s.y * s.y + s.z * s.z) * L.S;
+        s.x *= inv;
+        s.y *= inv;
+        s.z *= inv;
+        S[i] = s;
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------------------
+// wave / workgroup reductions (wave64 DPP-free shuffles)
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// sums n values per thread over the workgroup; result valid in thread 0.
+// scratch: at least 16*n doubles of LDS.
+template <int NV>
+__device__ __forceinline__ void block_sum(double *v, double *scratch) {
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) scratch[wave * NV + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < nw; ++w)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) v[k] += scratch[w * NV + k];
+    }
+    __syncthreads();
+}
+
+}  // namespace csmc

@@ -1,0 +1,801 @@
+// kernels.hip -- gfx950 kernels of libcsmc.
+//
+// The sweep hot path is k_wg: one workgroup (16 wave64s) owns one system
+// (a replica or an independent lattice) for a whole segment of sweeps.  The
+// colour classes of the system are swept in turn with a workgroup barrier
+// between classes, so no grid-wide synchronisation is ever needed; when the
+// system fits, its spins live in LDS for the whole segment.  Parallel
+// tempering segments end with the replica exchange, done by the last
+// workgroup to finish (a completion counter, no waiting).
+//
+// k_pass is the plain form: one launch per colour pass, one thread per site.
+#include "device.h"
+#include "kernels.h"
+
+namespace csmc {
+
+template <class R>
+__device__ __forceinline__ const R *coef_of(const DevPools &P);
+template <>
+__device__ __forceinline__ const float *coef_of<float>(const DevPools &P) {
+    return P.coefF;
+}
+template <>
+__device__ __forceinline__ const double *coef_of<double>(const DevPools &P) {
+    return P.coefD;
+}
+
+template <class R>
+__device__ __forceinline__ spin4<double> to_d(const spin4<R> &s) {
+    spin4<double> d;
+    d.x = s.x;
+    d.y = s.y;
+    d.z = s.z;
+    d.w = 0.0;
+    return d;
+}
+
+// interaction field split by term kind (bilinear, cubic, quartic), FP64,
+// for energies and gradients
+template <bool GEN, class LD>
+__device__ __forceinline__ void field_parts(const LatView<double> &L, int b, int i, LD &&ld,
+                                            double F2[3], double F3[3], double F4[3]) {
+    interaction_field<double, double, false>(L, b, i, ld, F2);
+    F3[0] = F3[1] = F3[2] = F4[0] = F4[1] = F4[2] = 0.0;
+    if (!GEN) return;
+    LatView<double> Lc = L;
+    Lc.K2 = 0;
+    Lc.K4 = 0;
+    interaction_field<double, double>(Lc, b, i, ld, F3);
+    LatView<double> Lq = L;
+    Lq.K2 = 0;
+    Lq.K3 = 0;
+    interaction_field<double, double>(Lq, b, i, ld, F4);
+}
+
+// ---------------------------------------------------------------------------
+// workgroup building blocks
+// ---------------------------------------------------------------------------
+template <class R, class SP, class OP>
+__device__ __forceinline__ void wg_colour_sweep(const LatView<R> &L, OP &&op) {
+    for (int c = 0; c < L.ncol; ++c) {
+        for (int r = L.colrun[c]; r < L.colrun[c + 1]; ++r) {
+            int4 run = L.runs[r];
+            int b = __builtin_amdgcn_readfirstlane(run.z);
+            for (int i = run.x + (int)threadIdx.x; i < run.y; i += (int)blockDim.x) op(b, i);
+        }
+        __syncthreads();
+    }
+}
+
+// FP64 energy parts (bilinear, cubic, quartic, zeeman, onsite) of a system,
+// summed over the workgroup; valid in every thread on return
+template <class R, bool GEN, class SP>
+__device__ void wg_energy(const LatView<double> &Ld, SP *X, double *red, double out[5]) {
+    double p[5] = {0, 0, 0, 0, 0};
+    for (int r = Ld.colrun[0]; r < Ld.colrun[Ld.ncol]; ++r) {
+        int4 run = Ld.runs[r];
+        int b = __builtin_amdgcn_readfirstlane(run.z);
+        const double *h = Ld.field + b * 3;
+        for (int i = run.x + (int)threadIdx.x; i < run.y; i += (int)blockDim.x) {
+            spin4<double> s = to_d<R>(X[i]);
+            double F2[3], F3[3], F4[3];
+            field_parts<GEN>(Ld, b, i, [&](int j) { return to_d<R>(X[j]); }, F2, F3, F4);
+            p[0] += s.x * F2[0] + s.y * F2[1] + s.z * F2[2];
+            p[1] += s.x * F3[0] + s.y * F3[1] + s.z * F3[2];
+            p[2] += s.x * F4[0] + s.y * F4[1] + s.z * F4[2];
+            p[3] -= s.x * h[0] + s.y * h[1] + s.z * h[2];
+            p[4] += onsite_energy<double, double>(Ld, b, s.x, s.y, s.z);
+        }
+    }
+    block_sum<5>(p, red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; ++k) red[k] = p[k];
+    __syncthreads();
+    for (int k = 0; k < 5; ++k) out[k] = red[k];
+    __syncthreads();
+}
+
+__device__ __forceinline__ double combine_energy(const double p[5]) {
+    // hamiltonian.jl:131
+    return p[0] / 2 + p[1] / 3 + p[2] / 4 + p[3] + p[4];
+}
+
+template <class R, class SP>
+__device__ void wg_magnet(int N, SP *X, double *red, double out[3]) {
+    double m[3] = {0, 0, 0};
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        spin4<R> s = X[i];
+        m[0] += s.x;
+        m[1] += s.y;
+        m[2] += s.z;
+    }
+    block_sum<3>(m, red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) red[k] = m[k];
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) out[k] = red[k];
+    __syncthreads();
+}
+
+template <class R, bool SELF, bool GEN, class SP>
+__device__ double wg_metro(const LatView<R> &L, SP *X, R T, int kind, R sigma, uint32_t k0,
+                           uint32_t k1, uint32_t stream, uint32_t ctr, double *red) {
+    int acc = 0;
+    wg_colour_sweep<R, SP>(L, [&](int b, int i) {
+        acc += metro_update<R, SELF, GEN>(L, X, b, i, T, kind, sigma, k0, k1, stream, ctr);
+    });
+    double v[1] = {(double)acc};
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) red[0] = v[0];
+    __syncthreads();
+    double tot = red[0];
+    __syncthreads();
+    return tot;
+}
+
+// metropolis_adaptive! width update (metropolis.jl:129-131)
+__device__ __forceinline__ double adapt_sigma(double sigma, double acc, double n) {
+    double a = acc / n;
+    double f = 0.5 / fmax(1.0 - a, 0.05);
+    return fmin(fmax(sigma * f, 0.0), 100.0);
+}
+
+// one log-binning push (E or |M| with its square) into a slot's binner
+__device__ void bin_push(double *st, double v) {
+    double c0 = v, c1 = v * v;
+    for (int l = 0; l < BIN_LVLS; ++l) {
+        double *q = st + l * 10;
+        q[0] += 1.0;
+        q[1] += c0;
+        q[2] += c1;
+        q[3] += c0 * c0;
+        q[4] += c0 * c1;
+        q[5] += c1 * c0;
+        q[6] += c1 * c1;
+        if (q[9] == 0.0) {
+            q[7] = c0;
+            q[8] = c1;
+            q[9] = 1.0;
+            return;
+        }
+        q[9] = 0.0;
+        c0 = 0.5 * (q[7] + c0);
+        c1 = 0.5 * (q[8] + c1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_wg: persistent workgroup-per-system sweeps
+// ---------------------------------------------------------------------------
+template <class R, bool LDS, bool SELF, bool GEN>
+__global__ __launch_bounds__((GEN || sizeof(R) == 8) ? 512 : 1024) void k_wg(WGArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *red = reinterpret_cast<double *>(smem);  // 1 KB reduction scratch
+    spin4<R> *lds = reinterpret_cast<spin4<R> *>(smem + 1024);
+
+    const int sys = blockIdx.x;
+    const SysDev sd = a.P.sys[sys];
+    const LatDev ld = a.P.lats[sd.lat];
+    const int4 *runs = a.P.runs + ld.run_off;
+    const LatView<R> L = lat_view<R>(ld, a.P.i32, runs, coef_of<R>(a.P));
+    const LatView<double> Ld = lat_view<double>(ld, a.P.i32, runs, a.P.coefD);
+    spin4<R> *G = reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off;
+    spin4<R> *X = LDS ? lds : G;
+    if (LDS) {
+        for (int i = threadIdx.x; i < L.N; i += blockDim.x) lds[i] = G[i];
+        __syncthreads();
+    }
+    const uint32_t stream = (uint32_t)sd.stream;
+    uint32_t mctr = a.mcount[sys];
+
+    auto or_sweep = [&]() {
+        wg_colour_sweep<R, spin4<R>>(L, [&](int b, int i) { or_update<R, GEN>(L, X, b, i); });
+    };
+    auto det_sweep = [&]() {
+        wg_colour_sweep<R, spin4<R>>(L, [&](int b, int i) { det_update<R, GEN>(L, X, b, i); });
+    };
+    auto metro_sweep = [&](R T, int kind, double &sigma) {
+        double acc = wg_metro<R, SELF, GEN>(L, X, T, kind, (R)sigma, a.k0, a.k1, stream, mctr, red);
+        ++mctr;
+        if (kind == CSMC_CONE_ADAPTIVE) sigma = adapt_sigma(sigma, acc, (double)L.N);
+        return acc;
+    };
+
+    if (a.mode == WG_SWEEPS) {
+        const R T = (R)a.T_sys[sys];
+        double sigma = a.sigma_sys ? a.sigma_sys[sys] : 60.0;
+        double acc = 0.0;
+        for (int s = 0; s < a.n_sweeps; ++s) {
+            if (a.kind == CSMC_OVERRELAX) or_sweep();
+            else if (a.kind == CSMC_DETERMINISTIC) det_sweep();
+            else acc += metro_sweep(T, a.kind, sigma);
+        }
+        if (threadIdx.x == 0) {
+            if (a.acc_sys) a.acc_sys[sys] = acc;
+            if (a.sigma_sys) a.sigma_sys[sys] = sigma;
+        }
+    } else if (a.mode == WG_ANNEAL) {
+        // simulated_annealing! (monte_carlo.jl:166-187), t = 1 .. t_th-1
+        for (int k = 0; k < a.n_temps; ++k) {
+            const R T = (R)a.temps[k];
+            double sigma = a.sigma0;
+            double acc = 0.0;
+            for (int64_t t = 1; t < a.t_th; ++t) {
+                if (a.or_rate != 0) {
+                    or_sweep();
+                    if (t % a.or_rate == 0) acc += metro_sweep(T, a.kind, sigma);
+                } else {
+                    acc += metro_sweep(T, a.kind, sigma);
+                }
+            }
+            if (threadIdx.x == 0) a.acc_T[(int64_t)sys * a.n_temps + k] = acc;
+        }
+    } else {
+        // parallel_tempering! sweep loop (monte_carlo.jl:295-388) from the
+        // post-exchange part of seg_begin (begin_mid) to the pre-exchange
+        // part of seg_end (end_mid)
+        const PTDev &pt = a.pt;
+        const int slot = pt.slot_of_sys[sys];
+        const R T = (R)pt.temps[slot];
+        double sigma = pt.sigma[slot];
+        double E = pt.E[sys];
+        double acc_local = 0.0;
+        double *binE = pt.binners + (int64_t)slot * 2 * BIN_DOUBLES;
+        double *binM = binE + BIN_DOUBLES;
+        for (int64_t sw = a.seg_begin; sw <= a.seg_end; ++sw) {
+            const bool head = !(sw == a.seg_begin && a.begin_mid);
+            const bool tail = !(sw == a.seg_end && a.end_mid);
+            if (head) {
+                if (pt.or_rate != 0) or_sweep();
+                if (sw % pt.dosweep == 0) {
+                    acc_local += metro_sweep(T, pt.alg, sigma);
+                    double p[5];
+                    wg_energy<R, GEN>(Ld, X, red, p);
+                    E = combine_energy(p);
+                }
+            }
+            if (tail && pt.measure && sw >= pt.t_th && sw % pt.probe == 0) {
+                double m[3];
+                wg_magnet<R>(L.N, X, red, m);
+                if (threadIdx.x == 0) {
+                    bin_push(binE, E);
+                    bin_push(binM, sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]));
+                }
+            }
+        }
+        if (threadIdx.x == 0) {
+            pt.E[sys] = E;
+            pt.sigma[slot] = sigma;
+            pt.acc_local[slot] += acc_local;
+        }
+    }
+
+    if (threadIdx.x == 0) a.mcount[sys] = mctr;
+    if (LDS) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < L.N; i += blockDim.x) G[i] = lds[i];
+    }
+
+    if (a.mode == WG_PT && a.exchange) {
+        // replica exchange (monte_carlo.jl:308-349) by the last workgroup
+        __shared__ int last;
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned prev = atomicAdd(a.pt.done, 1u);
+            last = (prev == gridDim.x - 1);
+        }
+        __syncthreads();
+        if (last && threadIdx.x == 0) {
+            __threadfence();
+            const PTDev &pt = a.pt;
+            const uint32_t xc = (uint32_t)(a.seg_end / pt.swap);
+            for (int r = (xc % 2 == 0) ? 0 : 1; r + 1 < pt.R; r += 2) {
+                const int p = r + 1;
+                const int sr = pt.sys_of_slot[r], sp = pt.sys_of_slot[p];
+                uint4 rr = philox4x32_10(make_uint4((uint32_t)r, 0xFFFFFFFFu, xc, TAG_EXCH), a.k0,
+                                         a.k1);
+                double u = u01d(rr.x);
+                double dbeta = 1.0 / pt.temps[p] - 1.0 / pt.temps[r];
+                double pacc = fmin(1.0, exp(dbeta * (pt.E[sp] - pt.E[sr])));
+                if (u < pacc) {
+                    pt.sys_of_slot[r] = sp;
+                    pt.sys_of_slot[p] = sr;
+                    pt.slot_of_sys[sp] = r;
+                    pt.slot_of_sys[sr] = p;
+                    pt.acc_exch[r] += 1.0;
+                    pt.acc_exch[p] += 1.0;
+                }
+            }
+            *pt.done = 0u;
+            __threadfence();
+        }
+    }
+}
+
+int max_lds_bytes() {
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        return 65536;
+    return v;
+}
+
+template <class R, bool LDS, bool SELF, bool GEN>
+static hipError_t launch_wg_t(int nsys, size_t lds_bytes, const WGArgs &a, hipStream_t st) {
+    size_t bytes = 1024 + (LDS ? lds_bytes : 0);
+    if (bytes > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_wg<R, LDS, SELF, GEN>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_wg<R, LDS, SELF, GEN>), dim3(nsys), dim3((GEN || sizeof(R) == 8) ? 512 : 1024), bytes, st, a);
+    return hipGetLastError();
+}
+
+template <class R, bool LDS>
+static hipError_t launch_wg_r(bool self, bool gen, int nsys, size_t lds_bytes, const WGArgs &a,
+                              hipStream_t st) {
+    if (gen) return self ? launch_wg_t<R, LDS, true, true>(nsys, lds_bytes, a, st)
+                         : launch_wg_t<R, LDS, false, true>(nsys, lds_bytes, a, st);
+    return self ? launch_wg_t<R, LDS, true, false>(nsys, lds_bytes, a, st)
+                : launch_wg_t<R, LDS, false, false>(nsys, lds_bytes, a, st);
+}
+
+hipError_t launch_wg(int prec, bool lds, bool self, bool gen, int nsys, size_t lds_bytes,
+                     const WGArgs &a, hipStream_t st) {
+    if (prec == 32)
+        return lds ? launch_wg_r<float, true>(self, gen, nsys, lds_bytes, a, st)
+                   : launch_wg_r<float, false>(self, gen, nsys, lds_bytes, a, st);
+    return lds ? launch_wg_r<double, true>(self, gen, nsys, lds_bytes, a, st)
+               : launch_wg_r<double, false>(self, gen, nsys, lds_bytes, a, st);
+}
+
+// ---------------------------------------------------------------------------
+// k_pass: one colour pass, one thread per site (plain / stream form)
+// ---------------------------------------------------------------------------
+template <class R, bool SELF, bool GEN>
+__global__ __launch_bounds__(256) void k_pass(PassArgs a) {
+    const int sys = a.sys0 + blockIdx.y;
+    const SysDev sd = a.P.sys[sys];
+    const LatDev ld = a.P.lats[sd.lat];
+    if (a.colour >= ld.ncol) return;
+    const int4 *runs = a.P.runs + ld.run_off;
+    const LatView<R> L = lat_view<R>(ld, a.P.i32, runs, coef_of<R>(a.P));
+    const int start = runs[L.colrun[a.colour]].x;
+    const int end = runs[L.colrun[a.colour + 1] - 1].y;
+    const int i = start + blockIdx.x * blockDim.x + threadIdx.x;
+    int acc = 0;
+    if (i < end) {
+        spin4<R> *S = reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off;
+        const int b = L.sub[i];
+        if (a.kind == CSMC_OVERRELAX) {
+            or_update<R, GEN>(L, S, b, i);
+        } else if (a.kind == CSMC_DETERMINISTIC) {
+            det_update<R, GEN>(L, S, b, i);
+        } else {
+            R sigma = a.sigma_sys ? (R)a.sigma_sys[sys] : R(60);
+            acc = metro_update<R, SELF, GEN>(L, S, b, i, (R)a.T_sys[sys], a.kind, sigma, a.k0, a.k1,
+                                        (uint32_t)sd.stream, a.ctr ? a.ctr[sys] : a.ctr0);
+        }
+    }
+    if (a.kind != CSMC_OVERRELAX && a.kind != CSMC_DETERMINISTIC) {
+        unsigned long long m = __ballot(acc);
+        if ((threadIdx.x & 63) == 0 && m) atomicAdd(&a.acc[sys], (uint32_t)__popcll(m));
+    }
+}
+
+template <class R>
+static void launch_pass_r(bool self, bool gen, dim3 grid, const PassArgs &a, hipStream_t st) {
+    if (gen) {
+        if (self) hipLaunchKernelGGL((k_pass<R, true, true>), grid, dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_pass<R, false, true>), grid, dim3(256), 0, st, a);
+    } else {
+        if (self) hipLaunchKernelGGL((k_pass<R, true, false>), grid, dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_pass<R, false, false>), grid, dim3(256), 0, st, a);
+    }
+}
+
+hipError_t launch_pass(int prec, bool self, bool gen, int nsys, int max_colour_sites,
+                       const PassArgs &a, hipStream_t st) {
+    dim3 grid((max_colour_sites + 255) / 256, nsys);
+    if (prec == 32) launch_pass_r<float>(self, gen, grid, a, st);
+    else launch_pass_r<double>(self, gen, grid, a, st);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// evaluation kernels (reference order outputs)
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(256) void k_site_eval(DevPools P, const void *spins, int sys,
+                                                   double *H, double *e) {
+    const SysDev sd = P.sys[sys];
+    const LatDev ld = P.lats[sd.lat];
+    const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
+    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.N) return;
+    int b = L.sub[i];
+    spin4<double> s = to_d<R>(S[i]);
+    double F[3], Hl[3];
+    interaction_field<double, double>(L, b, i, [&](int j) { return to_d<R>(S[j]); }, F);
+    local_field<double, double>(L, b, s, F, Hl);
+    int r = L.ref[i];
+    if (H) {
+        H[3 * (int64_t)r] = Hl[0];
+        H[3 * (int64_t)r + 1] = Hl[1];
+        H[3 * (int64_t)r + 2] = Hl[2];
+    }
+    if (e) {
+        const double *h = L.field + b * 3;
+        e[r] = onsite_energy<double, double>(L, b, s.x, s.y, s.z) + s.x * F[0] + s.y * F[1] +
+               s.z * F[2] - (s.x * h[0] + s.y * h[1] + s.z * h[2]);
+    }
+}
+
+hipError_t launch_site_eval(int prec, const DevPools &P, const void *spins, int sys, int N,
+                            double *H, double *e, hipStream_t st) {
+    dim3 grid((N + 255) / 256);
+    if (prec == 32) hipLaunchKernelGGL((k_site_eval<float>), grid, dim3(256), 0, st, P, spins, sys, H, e);
+    else hipLaunchKernelGGL((k_site_eval<double>), grid, dim3(256), 0, st, P, spins, sys, H, e);
+    return hipGetLastError();
+}
+
+template <class R>
+__global__ __launch_bounds__(1024) void k_energy(DevPools P, const void *spins, double *E,
+                                                 double *parts) {
+    __shared__ double red[16 * 8];
+    const int sys = blockIdx.x;
+    const SysDev sd = P.sys[sys];
+    const LatDev ld = P.lats[sd.lat];
+    const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
+    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
+    double p[5];
+    wg_energy<R, true>(L, S, red, p);
+    if (threadIdx.x == 0) {
+        E[sys] = combine_energy(p);
+        if (parts)
+            for (int k = 0; k < 5; ++k) parts[sys * 5 + k] = p[k];
+    }
+}
+
+hipError_t launch_energy(int prec, const DevPools &P, const void *spins, int nsys, double *E,
+                         double *parts, hipStream_t st) {
+    if (prec == 32) hipLaunchKernelGGL((k_energy<float>), dim3(nsys), dim3(1024), 0, st, P, spins, E, parts);
+    else hipLaunchKernelGGL((k_energy<double>), dim3(nsys), dim3(1024), 0, st, P, spins, E, parts);
+    return hipGetLastError();
+}
+
+template <class R>
+__global__ __launch_bounds__(1024) void k_magnet(DevPools P, const void *spins, double *M) {
+    __shared__ double red[16 * 8];
+    const int sys = blockIdx.x;
+    const SysDev sd = P.sys[sys];
+    const LatDev ld = P.lats[sd.lat];
+    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
+    double m[3];
+    wg_magnet<R>(ld.N, S, red, m);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) M[sys * 3 + k] = m[k];
+}
+
+hipError_t launch_magnet(int prec, const DevPools &P, const void *spins, int nsys, double *M,
+                         hipStream_t st) {
+    if (prec == 32) hipLaunchKernelGGL((k_magnet<float>), dim3(nsys), dim3(1024), 0, st, P, spins, M);
+    else hipLaunchKernelGGL((k_magnet<double>), dim3(nsys), dim3(1024), 0, st, P, spins, M);
+    return hipGetLastError();
+}
+
+// random_spin_orientation (lattice.jl:306-311) at every site; counter =
+// (reference site, stream, 0, INIT)
+template <class R>
+__global__ __launch_bounds__(256) void k_randomize(DevPools P, void *spins, uint32_t k0,
+                                                   uint32_t k1) {
+    const int sys = blockIdx.y;
+    const SysDev sd = P.sys[sys];
+    const LatDev ld = P.lats[sd.lat];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ld.N) return;
+    const int32_t *ref = P.i32 + ld.ref_off;
+    uint4 r = philox4x32_10(make_uint4((uint32_t)ref[i], (uint32_t)sd.stream, 0u, TAG_INIT), k0, k1);
+    R o[3];
+    Math<R>::rand_spin((R)ld.S, Math<R>::u01(r.x), Math<R>::u01(r.y), o);
+    spin4<R> s;
+    s.x = o[0];
+    s.y = o[1];
+    s.z = o[2];
+    s.w = R(0);
+    reinterpret_cast<spin4<R> *>(spins)[sd.spin_off + i] = s;
+}
+
+hipError_t launch_randomize(int prec, const DevPools &P, void *spins, int nsys, int maxN,
+                            uint32_t k0, uint32_t k1, hipStream_t st) {
+    dim3 grid((maxN + 255) / 256, nsys);
+    if (prec == 32) hipLaunchKernelGGL((k_randomize<float>), grid, dim3(256), 0, st, P, spins, k0, k1);
+    else hipLaunchKernelGGL((k_randomize<double>), grid, dim3(256), 0, st, P, spins, k0, k1);
+    return hipGetLastError();
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_scatter(DevPools P, void *spins, int sys,
+                                                 const double *src) {
+    const SysDev sd = P.sys[sys];
+    const LatDev ld = P.lats[sd.lat];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ld.N) return;
+    int r = P.i32[ld.ref_off + i];
+    spin4<R> s;
+    s.x = (R)src[3 * (int64_t)r];
+    s.y = (R)src[3 * (int64_t)r + 1];
+    s.z = (R)src[3 * (int64_t)r + 2];
+    s.w = R(0);
+    reinterpret_cast<spin4<R> *>(spins)[sd.spin_off + i] = s;
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_gather(DevPools P, const void *spins, int sys,
+                                                double *dst) {
+    const SysDev sd = P.sys[sys];
+    const LatDev ld = P.lats[sd.lat];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ld.N) return;
+    int r = P.i32[ld.ref_off + i];
+    spin4<R> s = reinterpret_cast<const spin4<R> *>(spins)[sd.spin_off + i];
+    dst[3 * (int64_t)r] = s.x;
+    dst[3 * (int64_t)r + 1] = s.y;
+    dst[3 * (int64_t)r + 2] = s.z;
+}
+
+hipError_t launch_scatter_spins(int prec, const DevPools &P, void *spins, int sys, int N,
+                                const double *src, hipStream_t st) {
+    dim3 grid((N + 255) / 256);
+    if (prec == 32) hipLaunchKernelGGL((k_scatter<float>), grid, dim3(256), 0, st, P, spins, sys, src);
+    else hipLaunchKernelGGL((k_scatter<double>), grid, dim3(256), 0, st, P, spins, sys, src);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_spins(int prec, const DevPools &P, const void *spins, int sys, int N,
+                               double *dst, hipStream_t st) {
+    dim3 grid((N + 255) / 256);
+    if (prec == 32) hipLaunchKernelGGL((k_gather<float>), grid, dim3(256), 0, st, P, spins, sys, dst);
+    else hipLaunchKernelGGL((k_gather<double>), grid, dim3(256), 0, st, P, spins, sys, dst);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// adjoint of total_energy (hamiltonian.jl:70-132)
+// ---------------------------------------------------------------------------
+
+// dE/ds_x by gathering over the incidence list of x: every (owner site,
+// term slot, role) in which x appears contributes the derivative of that
+// entry's energy, weighted 1/2, 1/3, 1/4 as in hamiltonian.jl:131; onsite
+// contributes (O + O^T) s and Zeeman -h.
+template <class R>
+__global__ __launch_bounds__(256) void k_grad_spins(DevPools P, const void *spins, double *dEdS) {
+    const int sys = blockIdx.y;
+    const SysDev sd = P.sys[sys];
+    const LatDev ld = P.lats[sd.lat];
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= ld.N) return;
+    const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
+    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
+    auto sp = [&](int j) { return to_d<R>(S[j]); };
+    const int32_t *ioff = P.i32 + ld.inc_off;
+    const Inc *inc = reinterpret_cast<const Inc *>(ioff + ld.N + 1);
+    const int bx = L.sub[x];
+    spin4<double> s = sp(x);
+    const double *O = L.onsite + bx * 9;
+    const double *h = L.field + bx * 3;
+    double g[3];
+    for (int a = 0; a < 3; ++a)
+        g[a] = (O[a * 3 + 0] + O[0 * 3 + a]) * s.x + (O[a * 3 + 1] + O[1 * 3 + a]) * s.y +
+               (O[a * 3 + 2] + O[2 * 3 + a]) * s.z - h[a];
+    for (int e = ioff[x]; e < ioff[x + 1]; ++e) {
+        const Inc in = inc[e];
+        const int i = in.owner, kind = in.code & 3, role = (in.code >> 2) & 3, k = in.code >> 4;
+        const int b = L.sub[i];
+        if (kind == 0) {
+            const double *M = L.c2 + (b * L.K2 + k) * 9;
+            spin4<double> si = sp(i), sj = sp(L.nbr2[(int64_t)k * L.N + i]);
+            if (role == 0) {
+                for (int a = 0; a < 3; ++a) g[a] += 0.5 * (M[a * 3] * sj.x + M[a * 3 + 1] * sj.y + M[a * 3 + 2] * sj.z);
+            } else {
+                for (int a = 0; a < 3; ++a) g[a] += 0.5 * (M[a] * si.x + M[3 + a] * si.y + M[6 + a] * si.z);
+            }
+        } else if (kind == 1) {
+            const double *C = L.c3 + (b * L.K3 + k) * 27;
+            spin4<double> v[3] = {sp(i), sp(L.nbr3[((int64_t)k * 2) * L.N + i]),
+                                  sp(L.nbr3[((int64_t)k * 2 + 1) * L.N + i])};
+            double w[3][3];
+            for (int q = 0; q < 3; ++q) {
+                w[q][0] = v[q].x;
+                w[q][1] = v[q].y;
+                w[q][2] = v[q].z;
+            }
+            for (int a = 0; a < 3; ++a)
+                for (int b2 = 0; b2 < 3; ++b2)
+                    for (int c = 0; c < 3; ++c) {
+                        double cf = C[a + 3 * b2 + 9 * c] / 3.0;
+                        if (role == 0) g[a] += cf * w[1][b2] * w[2][c];
+                        else if (role == 1) g[b2] += cf * w[0][a] * w[2][c];
+                        else g[c] += cf * w[0][a] * w[1][b2];
+                    }
+        } else {
+            const double *Q = L.c4 + (b * L.K4 + k) * 81;
+            spin4<double> v[4] = {sp(i), sp(L.nbr4[((int64_t)k * 3) * L.N + i]),
+                                  sp(L.nbr4[((int64_t)k * 3 + 1) * L.N + i]),
+                                  sp(L.nbr4[((int64_t)k * 3 + 2) * L.N + i])};
+            double w[4][3];
+            for (int q = 0; q < 4; ++q) {
+                w[q][0] = v[q].x;
+                w[q][1] = v[q].y;
+                w[q][2] = v[q].z;
+            }
+            for (int a = 0; a < 3; ++a)
+                for (int b2 = 0; b2 < 3; ++b2)
+                    for (int c = 0; c < 3; ++c)
+                        for (int d = 0; d < 3; ++d) {
+                            double cf = Q[a + 3 * b2 + 9 * c + 27 * d] / 4.0;
+                            if (role == 0) g[a] += cf * w[1][b2] * w[2][c] * w[3][d];
+                            else if (role == 1) g[b2] += cf * w[0][a] * w[2][c] * w[3][d];
+                            else if (role == 2) g[c] += cf * w[0][a] * w[1][b2] * w[3][d];
+                            else g[d] += cf * w[0][a] * w[1][b2] * w[2][c];
+                        }
+        }
+    }
+    int r = L.ref[x];
+    double *o = dEdS + 3 * (sd.spin_off + r);
+    o[0] = g[0];
+    o[1] = g[1];
+    o[2] = g[2];
+}
+
+hipError_t launch_grad_spins(int prec, const DevPools &P, const void *spins, int nsys, int maxN,
+                             double *dEdS, hipStream_t st) {
+    dim3 grid((maxN + 255) / 256, nsys);
+    if (prec == 32) hipLaunchKernelGGL((k_grad_spins<float>), grid, dim3(256), 0, st, P, spins, dEdS);
+    else hipLaunchKernelGGL((k_grad_spins<double>), grid, dim3(256), 0, st, P, spins, dEdS);
+    return hipGetLastError();
+}
+
+// dE/dJ per unit-cell term: one workgroup per system accumulates in LDS
+// with FP64 atomics.  Gradient tables at grad_off:
+//   t2[nb*K2], r2[nb*K2], t3[nb*K3], r3[nb*K3], t4[nb*K4], r4[nb*K4],
+//   fsrc[nb], osrc[nb], then the term offsets {field, onsite, bil, cub, qua}.
+template <class R>
+__global__ __launch_bounds__(1024) void k_grad_couplings(DevPools P, const void *spins,
+                                                         int grad_len, double *dEdJ) {
+    extern __shared__ __align__(16) unsigned char gsm[];
+    double *acc = reinterpret_cast<double *>(gsm);
+    const int sys = blockIdx.x;
+    const SysDev sd = P.sys[sys];
+    const LatDev ld = P.lats[sd.lat];
+    const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
+    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
+    auto sp = [&](int j) { return to_d<R>(S[j]); };
+    for (int k = threadIdx.x; k < grad_len; k += blockDim.x) acc[k] = 0.0;
+    __syncthreads();
+    const int nb = L.nb;
+    const int32_t *g = P.i32 + ld.grad_off;
+    const int32_t *t2 = g, *r2 = t2 + nb * L.K2, *t3 = r2 + nb * L.K2, *r3 = t3 + nb * L.K3;
+    const int32_t *t4 = r3 + nb * L.K3, *r4 = t4 + nb * L.K4, *fsrc = r4 + nb * L.K4;
+    const int32_t *osrc = fsrc + nb, *toff = osrc + nb;
+    const int off_f = toff[0], off_o = toff[1], off_2 = toff[2], off_3 = toff[3], off_4 = toff[4];
+    for (int i = threadIdx.x; i < L.N; i += blockDim.x) {
+        const int b = L.sub[i];
+        spin4<double> s = sp(i);
+        double si[3] = {s.x, s.y, s.z};
+        if (fsrc[b] >= 0)
+            for (int a = 0; a < 3; ++a) atomicAdd(&acc[off_f + fsrc[b] * 3 + a], -si[a]);
+        if (osrc[b] >= 0)
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c) atomicAdd(&acc[off_o + osrc[b] * 9 + a * 3 + c], si[a] * si[c]);
+        for (int k = 0; k < L.K2; ++k) {
+            int j = L.nbr2[(int64_t)k * L.N + i];
+            int t = t2[b * L.K2 + k];
+            if (j < 0 || t < 0) continue;
+            spin4<double> q = sp(j);
+            double sj[3] = {q.x, q.y, q.z};
+            bool rev = r2[b * L.K2 + k] != 0;
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c) {
+                    int idx = rev ? (c * 3 + a) : (a * 3 + c);
+                    atomicAdd(&acc[off_2 + t * 9 + idx], 0.5 * si[a] * sj[c]);
+                }
+        }
+        for (int k = 0; k < L.K3; ++k) {
+            int j = L.nbr3[((int64_t)k * 2) * L.N + i], kk = L.nbr3[((int64_t)k * 2 + 1) * L.N + i];
+            int t = t3[b * L.K3 + k];
+            if (j < 0 || t < 0) continue;
+            int role = r3[b * L.K3 + k];
+            spin4<double> q1 = sp(j), q2 = sp(kk);
+            double w[3][3] = {{si[0], si[1], si[2]}, {q1.x, q1.y, q1.z}, {q2.x, q2.y, q2.z}};
+            for (int a = 0; a < 3; ++a)
+                for (int b2 = 0; b2 < 3; ++b2)
+                    for (int c = 0; c < 3; ++c) {
+                        int id[3] = {a, b2, c};
+                        int tmp = id[0];
+                        id[0] = id[role];
+                        id[role] = tmp;
+                        atomicAdd(&acc[off_3 + t * 27 + id[0] + 3 * id[1] + 9 * id[2]],
+                                  w[0][a] * w[1][b2] * w[2][c] / 3.0);
+                    }
+        }
+        for (int k = 0; k < L.K4; ++k) {
+            int j = L.nbr4[((int64_t)k * 3) * L.N + i];
+            int kk = L.nbr4[((int64_t)k * 3 + 1) * L.N + i];
+            int l = L.nbr4[((int64_t)k * 3 + 2) * L.N + i];
+            int t = t4[b * L.K4 + k];
+            if (j < 0 || t < 0) continue;
+            int role = r4[b * L.K4 + k];
+            spin4<double> q1 = sp(j), q2 = sp(kk), q3 = sp(l);
+            double w[4][3] = {{si[0], si[1], si[2]}, {q1.x, q1.y, q1.z}, {q2.x, q2.y, q2.z},
+                              {q3.x, q3.y, q3.z}};
+            for (int a = 0; a < 3; ++a)
+                for (int b2 = 0; b2 < 3; ++b2)
+                    for (int c = 0; c < 3; ++c)
+                        for (int d = 0; d < 3; ++d) {
+                            int id[4] = {a, b2, c, d};
+                            int tmp = id[0];
+                            id[0] = id[role];
+                            id[role] = tmp;
+                            atomicAdd(&acc[off_4 + t * 81 + id[0] + 3 * id[1] + 9 * id[2] + 27 * id[3]],
+                                      w[0][a] * w[1][b2] * w[2][c] * w[3][d] / 4.0);
+                        }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < grad_len; k += blockDim.x) dEdJ[(int64_t)sys * grad_len + k] = acc[k];
+}
+
+hipError_t launch_grad_couplings(int prec, const DevPools &P, const void *spins, int nsys,
+                                 int grad_len, double *dEdJ, hipStream_t st) {
+    size_t bytes = (size_t)(grad_len > 0 ? grad_len : 1) * sizeof(double);
+    if (bytes > 65536) {
+        hipError_t e = hipFuncSetAttribute(prec == 32 ? (const void *)k_grad_couplings<float>
+                                                      : (const void *)k_grad_couplings<double>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+    }
+    if (prec == 32) hipLaunchKernelGGL((k_grad_couplings<float>), dim3(nsys), dim3(1024), bytes, st, P, spins, grad_len, dEdJ);
+    else hipLaunchKernelGGL((k_grad_couplings<double>), dim3(nsys), dim3(1024), bytes, st, P, spins, grad_len, dEdJ);
+    return hipGetLastError();
+}
+
+// projected gradient step on the sphere
+template <class R>
+__global__ __launch_bounds__(256) void k_gd(DevPools P, void *spins, const double *dEdS,
+                                            double eta) {
+    const int sys = blockIdx.y;
+    const SysDev sd = P.sys[sys];
+    const LatDev ld = P.lats[sd.lat];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ld.N) return;
+    int r = P.i32[ld.ref_off + i];
+    spin4<R> *S = reinterpret_cast<spin4<R> *>(spins) + sd.spin_off;
+    spin4<R> s = S[i];
+    const double *g = dEdS + 3 * (sd.spin_off + r);
+    double sx = s.x, sy = s.y, sz = s.z;
+    double n = sqrt(sx * sx + sy * sy + sz * sz);
+    double ux = sx / n, uy = sy / n, uz = sz / n;
+    double gu = g[0] * ux + g[1] * uy + g[2] * uz;
+    double px = g[0] - gu * ux, py = g[1] - gu * uy, pz = g[2] - gu * uz;
+    double nx = sx - eta * px, ny = sy - eta * py, nz = sz - eta * pz;
+    double nn = sqrt(nx * nx + ny * ny + nz * nz);
+    s.x = (R)(nx / nn * ld.S);
+    s.y = (R)(ny / nn * ld.S);
+    s.z = (R)(nz / nn * ld.S);
+    S[i] = s;
+}
+
+hipError_t launch_gd(int prec, const DevPools &P, void *spins, int nsys, int maxN,
+                     const double *dEdS, double eta, hipStream_t st) {
+    dim3 grid((maxN + 255) / 256, nsys);
+    if (prec == 32) hipLaunchKernelGGL((k_gd<float>), grid, dim3(256), 0, st, P, spins, dEdS, eta);
+    else hipLaunchKernelGGL((k_gd<double>), grid, dim3(256), 0, st, P, spins, dEdS, eta);
+    return hipGetLastError();
+}
+
+}  // namespace csmc

@@ -1,0 +1,228 @@
+/*
+ * csmc.h -- C ABI of libcsmc, the MI355X (gfx950) sweep engine for
+ * ClassicalSpinMC.jl-style classical spin Monte Carlo.
+ *
+ * Conventions (chosen so a Julia ccall can pass the reference's own arrays):
+ *   - every entry point returns int32 status, CSMC_OK == 0; the message of the
+ *     last failure on the calling thread is csmc_last_error();
+ *   - host pointers are borrowed for the duration of the call only;
+ *   - basis indices are 1-based (unit_cell.jl:23-75), site indices exported in
+ *     reference tables are 1-based with 0 = absent (lattice.jl:16-21);
+ *   - spins are double[N][3] == Julia's 3xN column-major Array{Float64,2}
+ *     (lattice.jl:6), in the reference's site order (lattice.jl:29-33);
+ *   - InteractionMatrix is row-major m11..m33 (interaction_matrix.jl:2-12);
+ *     cubic / quartic tensors are Julia column-major Array{Float64,3/4}:
+ *     C[a,b,c] at a + 3b + 9c, R[a,b,c,d] at a + 3b + 9c + 27d (0-based).
+ *   - calls on one context are serialised on that context's HIP stream and
+ *     are not re-entrant; use one context per host thread.
+ */
+#ifndef CSMC_H
+#define CSMC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CSMC_OK 0
+#define CSMC_EINVAL 1   /* invalid argument                                    */
+#define CSMC_EBOUNDS 2  /* the reference raises BoundsError (lattice.jl:132,138) */
+#define CSMC_EUNDEF 3   /* the reference reads an undef entry (lattice.jl:122,126) */
+#define CSMC_EBC 4      /* error("Invalid boundary condition option") (lattice.jl:107) */
+#define CSMC_EHIP 5     /* HIP runtime failure                                  */
+#define CSMC_ENOMEM 6
+#define CSMC_ESTATE 7   /* object in the wrong state / unknown handle           */
+
+#define CSMC_FP32 32
+#define CSMC_FP64 64
+
+/* sweep kinds; the Metropolis kinds are the alg plugin of
+ * metropolis.jl:181-199 */
+#define CSMC_OVERRELAX 0       /* overrelaxation!        monte_carlo.jl:126-139 */
+#define CSMC_METROPOLIS 1      /* metropolis!            metropolis.jl:65-82    */
+#define CSMC_CONE 2            /* metropolis_fixed_cone! metropolis.jl:136-153  */
+#define CSMC_CONE_ADAPTIVE 3   /* metropolis_adaptive!   metropolis.jl:113-133  */
+#define CSMC_DETERMINISTIC 4   /* deterministic_updates! monte_carlo.jl:201-213 */
+
+typedef struct csmc_ctx csmc_ctx;
+
+/* UnitCell{D} (unit_cell.jl:3-18) flattened.  Term arrays are in call order
+ * of addZeemanCoupling!/addOnSite!/addBilinear!/addCubic!/addQuartic!. */
+typedef struct csmc_unit_cell {
+    int32_t dim;                    /* D in 1..3                               */
+    int32_t n_basis;
+    const double *lattice_vectors;  /* [D][D]: vector d at d*D                 */
+    const double *basis;            /* [n_basis][D]                            */
+    int32_t n_field;
+    const int32_t *field_b;         /* [n_field]                               */
+    const double *field_h;          /* [n_field][3]                            */
+    int32_t n_onsite;
+    const int32_t *onsite_b;        /* [n_onsite]                              */
+    const double *onsite_m;         /* [n_onsite][9] row-major                 */
+    int32_t n_bilinear;
+    const int32_t *bilinear_b;      /* [n][2]  (b1, b2)                        */
+    const double *bilinear_m;       /* [n][9]  row-major                       */
+    const int32_t *bilinear_off;    /* [n][D]                                  */
+    int32_t n_cubic;
+    const int32_t *cubic_b;         /* [n][3]                                  */
+    const double *cubic_t;          /* [n][27] column-major                    */
+    const int32_t *cubic_off;       /* [n][2][D]  (o2, o3)                     */
+    int32_t n_quartic;
+    const int32_t *quartic_b;       /* [n][4]                                  */
+    const double *quartic_t;        /* [n][81] column-major                    */
+    const int32_t *quartic_off;     /* [n][3][D]  (o2, o3, o4)                 */
+} csmc_unit_cell;
+
+typedef struct csmc_lattice_info {
+    int64_t n_sites;       /* Lattice.size (lattice.jl:10)                     */
+    int32_t dim, n_basis;
+    int32_t n2, n3, n4;    /* per-site tuple widths N2, N3, N4 (lattice.jl:4)   */
+    int32_t k2, k3, k4;    /* device slots per site after compaction           */
+    int32_t n_colours;     /* independent sets of the interaction hypergraph   */
+    int32_t has_self;      /* some term of a site references the site itself   */
+    double S;
+} csmc_lattice_info;
+
+/* Parallel-tempering schedule: SimulationParameters (metropolis.jl:4-13). */
+typedef struct csmc_pt_params {
+    int64_t t_thermalization, t_measurement, probe_rate, swap_rate;
+    int64_t overrelaxation_rate, report_interval, checkpoint_rate;
+    int32_t alg;            /* CSMC_METROPOLIS / CSMC_CONE / CSMC_CONE_ADAPTIVE */
+    int32_t measure;        /* 1: accumulate E and |M| binning per slot         */
+} csmc_pt_params;
+
+/* ---- library / context -------------------------------------------------- */
+int32_t csmc_version(void);
+const char *csmc_last_error(void);
+/* no reference counterpart: device discovery for the HIP backend */
+int32_t csmc_device_count(int32_t *n);
+int32_t csmc_ctx_create(int32_t device, csmc_ctx **out);
+int32_t csmc_ctx_destroy(csmc_ctx *ctx);
+int32_t csmc_ctx_sync(csmc_ctx *ctx);
+/* no reference counterpart: kernel timing with HIP events recorded on the
+ * context's stream around every sweep-kernel launch (enable != 0 starts a
+ * fresh window); read returns the summed kernel time and the launch count. */
+int32_t csmc_timing(csmc_ctx *ctx, int32_t enable);
+int32_t csmc_timing_read(csmc_ctx *ctx, double *total_ms, int64_t *launches);
+
+/* ---- Lattice ------------------------------------------------------------ */
+/* Host-only form of the Lattice constructor (lattice.jl:65-291): builds the
+ * tables and the colouring without a device.  info receives the sizes; any
+ * output pointer may be NULL (layouts as csmc_lattice_tables; colour[N]). */
+int32_t csmc_build_tables(const csmc_unit_cell *uc, const int32_t *shape, int32_t periodic,
+                          double S, csmc_lattice_info *info, double *pos, double *field,
+                          double *onsite, int64_t *bsite, double *bmat, int64_t *csite,
+                          double *ctens, int64_t *qsite, double *qtens, int32_t *colour);
+/* Lattice(shape, uc, S; bc) (lattice.jl:65-291): builds the per-site tables,
+ * colours the interaction hypergraph and uploads device tables.
+ * periodic: 1 = "periodic", 0 = "open", other = CSMC_EBC. */
+int32_t csmc_lattice_create(csmc_ctx *ctx, const csmc_unit_cell *uc, const int32_t *shape,
+                            int32_t periodic, double S, int32_t *lattice_id);
+int32_t csmc_lattice_info_get(csmc_ctx *ctx, int32_t lattice_id, csmc_lattice_info *info);
+/* The reference's Lattice fields (lattice.jl:12-22) in reference layout:
+ * pos[N][D] (site_positions transposed), field[N][3], onsite[N][9],
+ * bsite[N][n2], bmat[N][n2][9], csite[N][n3][2], ctens[N][n3][27],
+ * qsite[N][n4][3], qtens[N][n4][81].  Any pointer may be NULL. */
+int32_t csmc_lattice_tables(csmc_ctx *ctx, int32_t lattice_id, double *pos, double *field,
+                            double *onsite, int64_t *bsite, double *bmat, int64_t *csite,
+                            double *ctens, int64_t *qsite, double *qtens);
+/* no reference counterpart: colour of each site (reference order), 0-based */
+int32_t csmc_lattice_colouring(csmc_ctx *ctx, int32_t lattice_id, int32_t *colour);
+/* no reference counterpart: replace the couplings of a lattice whose unit
+ * cell has the same terms (same bases, offsets, order) -- used to fit
+ * couplings without rebuilding neighbour tables */
+int32_t csmc_lattice_update_couplings(csmc_ctx *ctx, int32_t lattice_id,
+                                      const csmc_unit_cell *uc);
+
+/* ---- batches of systems (replicas or independent lattices) --------------- */
+/* One MonteCarlo per system (metropolis.jl:15-29); the batch is what replaces
+ * one MPI rank per replica (monte_carlo.jl:246-262). */
+int32_t csmc_batch_create(csmc_ctx *ctx, int32_t n_systems, const int32_t *lattice_ids,
+                          int32_t precision, int32_t *batch_id);
+int32_t csmc_batch_destroy(csmc_ctx *ctx, int32_t batch_id);
+/* Lattice.spins get / set (lattice.jl:293-301) for one system, double[N][3] */
+int32_t csmc_set_spins(csmc_ctx *ctx, int32_t batch_id, int32_t system, const double *spins);
+int32_t csmc_get_spins(csmc_ctx *ctx, int32_t batch_id, int32_t system, double *spins);
+/* random_spin_orientation (lattice.jl:306-311) for every site of every system
+ * from the Philox stream (seed); initialCondition :random (lattice.jl:76-79) */
+int32_t csmc_randomize_spins(csmc_ctx *ctx, int32_t batch_id, uint64_t seed);
+
+/* ---- Hamiltonian (hamiltonian.jl) ----------------------------------------- */
+/* get_local_field (hamiltonian.jl:3-67) at every site of one system: H[N][3] */
+int32_t csmc_local_field(csmc_ctx *ctx, int32_t batch_id, int32_t system, double *H);
+/* energy(lattice, point) (hamiltonian.jl:139-196) at every site: e[N] */
+int32_t csmc_site_energy(csmc_ctx *ctx, int32_t batch_id, int32_t system, double *e);
+/* total_energy (hamiltonian.jl:70-132), FP64 reduction, per system: E[n_sys];
+ * parts[n_sys][5] (optional) = bilinear, cubic, quartic, zeeman, onsite sums
+ * before the 1/2, 1/3, 1/4 weights of hamiltonian.jl:131 */
+int32_t csmc_total_energy(csmc_ctx *ctx, int32_t batch_id, double *E, double *parts);
+/* get_magnetization (observables.jl:12-18) as the vector sum, FP64: M[n_sys][3] */
+int32_t csmc_magnetization(csmc_ctx *ctx, int32_t batch_id, double *M);
+
+/* ---- sweeps ------------------------------------------------------------- */
+/* n_sweeps coloured sweeps of `kind` on every system, each system at its
+ * temperature T[sys] (sigma[sys] cone width for the cone kinds; adaptive
+ * kinds update sigma in place, metropolis.jl:129-131).  accepted[sys]
+ * (optional) receives the accepted-proposal count summed over the sweeps.
+ * Replaces one call of the alg plugin per sweep (metropolis.jl:181-199),
+ * overrelaxation! (monte_carlo.jl:126-139) and the deterministic loop
+ * (monte_carlo.jl:201-213).  Philox key = seed; the per-system Metropolis
+ * sweep counter advances by one per Metropolis sweep. */
+int32_t csmc_sweep(csmc_ctx *ctx, int32_t batch_id, int32_t kind, const double *T,
+                   double *sigma, int32_t n_sweeps, uint64_t seed, double *accepted);
+/* The plain unbatched form of csmc_sweep: one system at a time, one kernel
+ * launch per colour pass, no persistence and no graph.  Same results. */
+int32_t csmc_sweep_plain(csmc_ctx *ctx, int32_t batch_id, int32_t kind, const double *T,
+                         double *sigma, int32_t n_sweeps, uint64_t seed, double *accepted);
+/* Set / read the per-system Metropolis sweep counters (Philox counter word). */
+int32_t csmc_set_counters(csmc_ctx *ctx, int32_t batch_id, const uint32_t *counters);
+int32_t csmc_get_counters(csmc_ctx *ctx, int32_t batch_id, uint32_t *counters);
+
+/* ---- simulated_annealing! (monte_carlo.jl:157-190) ------------------------ */
+/* For each temperature temps[k] (k < n_temps, the caller evaluates the
+ * schedule: T0, schedule(1), ... while T > mc.T), every system runs
+ * t_thermalization-1 sweep steps: overrelaxation every step (unless
+ * or_rate == 0) and `alg` every or_rate-th step (every step if 0); sigma is
+ * reset to sigma0 per temperature.  accepted[sys][k] receives the accepted
+ * count per temperature.  All systems anneal in one persistent launch. */
+int32_t csmc_anneal(csmc_ctx *ctx, int32_t batch_id, const double *temps, int32_t n_temps,
+                    int64_t t_thermalization, int64_t overrelaxation_rate, int32_t alg,
+                    double sigma0, uint64_t seed, double *accepted);
+
+/* ---- parallel_tempering! (monte_carlo.jl:235-398) ------------------------- */
+/* The batch's systems are the replicas; temps[n_sys] ascending-by-slot
+ * temperature ladder (slot r plays MPI rank r).  Exchange = temperature-label
+ * swap on device, decided in FP64. */
+int32_t csmc_pt_init(csmc_ctx *ctx, int32_t batch_id, const double *temps,
+                     const csmc_pt_params *params, double sigma0, uint64_t seed);
+/* Advance the schedule from the current sweep up to (not including) sweep
+ * `until` (or to the end if until < 0).  The call stops early at sweeps where
+ * the reference performs host work (checkpoint_rate, report_interval);
+ * *stopped receives the sweep it stopped at and *event the reason
+ * (0 = until reached, 1 = checkpoint due, 2 = report due, 3 = finished). */
+int32_t csmc_pt_advance(csmc_ctx *ctx, int32_t batch_id, int64_t until, int64_t *stopped,
+                        int32_t *event);
+/* Per temperature slot: slot->system map, FP64 energy of the configuration
+ * at that slot, local accepted count, exchange accepted count, sigma. */
+int32_t csmc_pt_state(csmc_ctx *ctx, int32_t batch_id, int32_t *system_of_slot, double *E,
+                      double *accepted_local, double *accepted_exchange, double *sigma);
+/* Final observables per slot (hdf5.jl:204-239 / observables.jl:36-63):
+ * out[n_sys][8] = C, dC, chi, dchi, <E>, dE, <|M|>, d|M| (absolute errors). */
+int32_t csmc_pt_observables(csmc_ctx *ctx, int32_t batch_id, double *out);
+
+/* ---- differentiation (no reference counterpart: adjoint of
+ * total_energy, hamiltonian.jl:70-132) ------------------------------------ */
+/* dE/dS_i for every site of every system, double[n_sys][N][3]. */
+int32_t csmc_energy_grad_spins(csmc_ctx *ctx, int32_t batch_id, double *dEdS);
+/* dE/d(coupling) per unit-cell term, per system:
+ * out[n_sys][n_field*3 + n_onsite*9 + n2*9 + n3*27 + n4*81] in unit-cell term
+ * order, each tensor in the csmc_unit_cell layout. */
+int32_t csmc_energy_grad_couplings(csmc_ctx *ctx, int32_t batch_id, double *dEdJ);
+/* One projected gradient-descent step on the sphere for every system:
+ * s <- S * normalise(s - eta * (g - (g.s^) s^)), g = dE/ds. */
+int32_t csmc_gradient_descent(csmc_ctx *ctx, int32_t batch_id, double eta, int32_t n_steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
