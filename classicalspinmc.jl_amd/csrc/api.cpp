@@ -75,6 +75,8 @@ struct Batch {
     int prec = 64;
     int nsys = 0;
     int maxN = 0;
+    int maxK2 = 0;
+    int k2c = 0;  // bilinear slot count shared by all systems (0 if mixed)
     int64_t total = 0;
     bool self = false;
     bool gen = false;  // some lattice has cubic or quartic terms
@@ -109,7 +111,8 @@ struct csmc_ctx {
     std::map<int, std::unique_ptr<Batch>> batches;
     int next_batch = 1;
     int max_lds = 65536;
-    bool no_lds = false;
+    int lds_cap = 2;  // highest workgroup storage mode allowed
+    bool specialise = true;  // use slot-count specialised kernels when they exist
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     double timed_ms = 0.0;
@@ -151,6 +154,22 @@ DevPools pools(csmc_ctx *c, Batch *b) {
     return P;
 }
 
+int pad4(int k) { return (k + 3) & ~3; }
+
+// device coefficient block of a lattice: the host block with the bilinear
+// slots padded to a multiple of 4 (zero couplings in the padding)
+std::vector<double> device_coef(const HostLattice &L) {
+    const int K2P = pad4(L.K2);
+    std::vector<double> v(L.coef.begin(), L.coef.begin() + L.coef_c2);
+    for (int b = 0; b < L.nb; ++b) {
+        const double *src = &L.coef[L.coef_c2 + (size_t)b * L.K2 * 9];
+        v.insert(v.end(), src, src + (size_t)L.K2 * 9);
+        v.insert(v.end(), (size_t)(K2P - L.K2) * 9, 0.0);
+    }
+    v.insert(v.end(), L.coef.begin() + L.coef_c3, L.coef.end());
+    return v;
+}
+
 // append one lattice's device tables to the host-side pools
 void append_lattice(csmc_ctx *c, const HostLattice &L) {
     LatDev d{};
@@ -158,24 +177,33 @@ void append_lattice(csmc_ctx *c, const HostLattice &L) {
     d.N = N;
     d.nb = L.nb;
     d.K2 = L.K2;
+    d.K2P = pad4(L.K2);
     d.K3 = L.K3;
     d.K4 = L.K4;
     d.ncol = L.ncol;
     d.nruns = (int)L.runs.size();
     d.has_self = L.has_self;
     d.S = L.S;
+    d.flags = 0;
+    for (long k = 0; k < (long)L.nb * 9; ++k)
+        if (L.coef[L.coef_onsite + k] != 0.0) d.flags |= LAT_ONSITE;
+    for (long k = 0; k < (long)L.nb * 3; ++k)
+        if (L.coef[L.coef_field + k] != 0.0) d.flags |= LAT_FIELD;
     auto &I = c->h_i32;
     auto dev = [&](int ref) { return ref < 0 ? -1 : L.dev_of_ref[ref]; };
+    auto devp = [&](int ref) { return ref < 0 ? N : L.dev_of_ref[ref]; };  // absent -> pad site
     // neighbour tables in device order, slot-major
     d.nbr_off = (int64_t)I.size();
     for (int k = 0; k < L.K2; ++k)
-        for (int i = 0; i < N; ++i) I.push_back(dev(L.nbr2[(size_t)L.ref_of_dev[i] * L.K2 + k]));
+        for (int i = 0; i < N; ++i) I.push_back(devp(L.nbr2[(size_t)L.ref_of_dev[i] * L.K2 + k]));
+    for (int k = L.K2; k < d.K2P; ++k)
+        for (int i = 0; i < N; ++i) I.push_back(N);
     for (int k = 0; k < L.K3; ++k)
         for (int q = 0; q < 2; ++q)
-            for (int i = 0; i < N; ++i) I.push_back(dev(L.nbr3[((size_t)L.ref_of_dev[i] * L.K3 + k) * 2 + q]));
+            for (int i = 0; i < N; ++i) I.push_back(devp(L.nbr3[((size_t)L.ref_of_dev[i] * L.K3 + k) * 2 + q]));
     for (int k = 0; k < L.K4; ++k)
         for (int q = 0; q < 3; ++q)
-            for (int i = 0; i < N; ++i) I.push_back(dev(L.nbr4[((size_t)L.ref_of_dev[i] * L.K4 + k) * 3 + q]));
+            for (int i = 0; i < N; ++i) I.push_back(devp(L.nbr4[((size_t)L.ref_of_dev[i] * L.K4 + k) * 3 + q]));
     d.ref_off = (int64_t)I.size();
     for (int i = 0; i < N; ++i) I.push_back(L.ref_of_dev[i]);
     d.sub_off = (int64_t)I.size();
@@ -240,7 +268,8 @@ void append_lattice(csmc_ctx *c, const HostLattice &L) {
     I.push_back(3 * nf + 9 * no + 9 * L.n2 + 27 * L.n3);
     d.grad_len = L.grad_len;
     d.coef_off = (int64_t)c->h_coef.size();
-    c->h_coef.insert(c->h_coef.end(), L.coef.begin(), L.coef.end());
+    std::vector<double> dc = device_coef(L);
+    c->h_coef.insert(c->h_coef.end(), dc.begin(), dc.end());
     c->h_lat.push_back(d);
     c->dirty = true;
 }
@@ -256,10 +285,25 @@ Batch *get_batch(csmc_ctx *c, int id) {
     Batch *b = get_batch(c, id);                                                \
     if (!b) return fail(CSMC_ESTATE, "unknown batch id " + std::to_string(id))
 
-bool use_lds(csmc_ctx *c, Batch *b) {
-    if (c->no_lds) return false;
-    size_t need = 1024 + (size_t)b->maxN * b->spin_bytes();
-    return need <= (size_t)c->max_lds;
+// storage mode of the workgroup kernel: the most on-chip layout whose LDS
+// footprint fits (spins as three component arrays, then uint16 neighbour
+// tables), capped by CSMC_LDS (0, 1, 2) for measurements of each mode
+int choose_lm(csmc_ctx *c, Batch *b, size_t *bytes) {
+    auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
+    size_t comp = b->prec == 32 ? 4 : 8;
+    size_t spins = al(3 * comp * (size_t)(b->maxN + 1));  // + the zero pad site
+    size_t idx = al(2 * (size_t)b->maxK2 * b->maxN);
+    size_t avail = (size_t)c->max_lds > 1024 ? (size_t)c->max_lds - 1024 : 0;
+    if (c->lds_cap >= 2 && b->maxN < 65535 && spins + idx <= avail) {
+        *bytes = spins + idx;
+        return 2;
+    }
+    if (c->lds_cap >= 1 && spins <= avail) {
+        *bytes = spins;
+        return 1;
+    }
+    *bytes = 0;
+    return 0;
 }
 
 WGArgs wg_base(csmc_ctx *c, Batch *b, uint64_t seed) {
@@ -291,15 +335,16 @@ int drain_events(csmc_ctx *c, bool wait) {
 }
 
 int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
-    bool lds = use_lds(c, b);
+    size_t lds_bytes = 0;
+    int lm = choose_lm(c, b, &lds_bytes);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, c->stream));
     }
-    HIPCHK(launch_wg(b->prec, lds, b->self, b->gen, b->nsys, lds ? (size_t)b->maxN * b->spin_bytes() : 0, a,
-                     c->stream));
+    HIPCHK(launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0, b->nsys, lds_bytes,
+                     a, c->stream));
     if (c->timing) {
         HIPCHK(hipEventRecord(e1, c->stream));
         c->events.push_back({e0, e1});
@@ -339,15 +384,17 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
         return fail(CSMC_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
     c->max_lds = max_lds_bytes();
-    const char *nl = getenv("CSMC_NO_LDS");
-    c->no_lds = nl && nl[0] == '1';
+    const char *lc = getenv("CSMC_LDS");
+    if (lc && lc[0] >= '0' && lc[0] <= '2') c->lds_cap = lc[0] - '0';
+    const char *sp = getenv("CSMC_SPECIALISE");
+    if (sp && sp[0] == '0') c->specialise = false;
     const char *vb = getenv("CSMC_VERBOSE");
     if (vb && vb[0] == '1') {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess)
             fprintf(stderr, "csmc: device %d %s (%s), %d CUs, max LDS per block %d B%s\n", device,
                     prop.name, prop.gcnArchName, prop.multiProcessorCount, c->max_lds,
-                    c->no_lds ? " (LDS residency off)" : "");
+                    c->lds_cap < 2 ? " (LDS modes capped)" : "");
     }
     *out = c;
     return CSMC_OK;
@@ -462,7 +509,14 @@ int32_t csmc_lattice_update_couplings(csmc_ctx *c, int32_t id, const csmc_unit_c
     std::string err;
     int rc = update_couplings(uc, L, err);
     if (rc) return fail(rc, err);
-    std::copy(L.coef.begin(), L.coef.end(), c->h_coef.begin() + c->h_lat[id].coef_off);
+    std::vector<double> dc = device_coef(L);
+    std::copy(dc.begin(), dc.end(), c->h_coef.begin() + c->h_lat[id].coef_off);
+    int flags = 0;
+    for (long k = 0; k < (long)L.nb * 9; ++k)
+        if (L.coef[L.coef_onsite + k] != 0.0) flags |= LAT_ONSITE;
+    for (long k = 0; k < (long)L.nb * 3; ++k)
+        if (L.coef[L.coef_field + k] != 0.0) flags |= LAT_FIELD;
+    c->h_lat[id].flags = flags;
     HIPCHK(hipStreamSynchronize(c->stream));
     c->dirty = true;
     return CSMC_OK;
@@ -476,7 +530,7 @@ int32_t csmc_batch_create(csmc_ctx *c, int32_t n, const int32_t *lat_ids, int32_
     auto b = std::make_unique<Batch>();
     b->prec = prec;
     b->nsys = n;
-    int64_t off = 0;
+    int64_t off = 0, sites = 0;
     for (int s = 0; s < n; ++s) {
         int id = lat_ids[s];
         if (id < 0 || id >= (int)c->lats.size())
@@ -486,14 +540,18 @@ int32_t csmc_batch_create(csmc_ctx *c, int32_t n, const int32_t *lat_ids, int32_
         sd.lat = id;
         sd.stream = s;
         sd.spin_off = off;
+        sd.site_off = sites;
         b->sys.push_back(sd);
         b->lat.push_back(id);
-        off += L.N;
+        off += L.N + 1;  // one zero pad site per system
+        sites += L.N;
         b->maxN = std::max<int>(b->maxN, (int)L.N);
+        b->maxK2 = std::max<int>(b->maxK2, L.K2);
+        b->k2c = (s == 0 || b->k2c == L.K2) ? L.K2 : 0;
         b->self |= (L.has_self != 0);
         b->gen |= (L.K3 > 0 || L.K4 > 0);
     }
-    b->total = off;
+    b->total = sites;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(b->sys_d.alloc(n));
     HIPCHK(hipMemcpy(b->sys_d.p, b->sys.data(), n * sizeof(SysDev), hipMemcpyHostToDevice));

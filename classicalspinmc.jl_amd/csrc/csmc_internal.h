@@ -2,8 +2,11 @@
 // kernels of libcsmc.
 //
 // Device data layout (one HBM pool per kind, all lattices concatenated):
-//   i32 pool   per lattice: nbr2[K2][N], nbr3[K3][2][N], nbr4[K4][3][N]
-//              (device site indices, -1 = absent), ref[N] (reference index of
+//   i32 pool   per lattice: nbr2[K2P][N], nbr3[K3][2][N], nbr4[K4][3][N]
+//              (device site indices; an absent neighbour -- open boundary,
+//              or one of the K2P - K2 padding slots -- is the pad site N,
+//              whose spin is kept zero, so sweeps need no branch for it),
+//              ref[N] (reference index of
 //              each device site), sub[N] (basis of each device site, 0-based),
 //              colrun[ncol+1] (run index boundaries per colour), incidence CSR
 //              for the spin gradient.
@@ -12,11 +15,12 @@
 //              (colour, basis, reference index), so every colour class is one
 //              contiguous range and each run has one coefficient set.
 //   coef pool  per lattice (float and double copies at the same offset):
-//              onsite[nb][9], field[nb][3], c2[nb][K2][9], c3[nb][K3][27],
+//              onsite[nb][9], field[nb][3], c2[nb][K2P][9], c3[nb][K3][27],
 //              c4[nb][K4][81]: the couplings of each (basis, slot) with the
 //              role permutation of lattice.jl:184-193, 215-227, 251-272
-//              already applied.
-//   spins      spin4<R> {x,y,z,pad} per device site, systems back to back.
+//              already applied; padding slots are zero.
+//   spins      spin4<R> {x,y,z,pad} per device site plus one zero pad site
+//              (N + 1 records per system), systems back to back.
 #pragma once
 #include <stdint.h>
 
@@ -26,6 +30,8 @@ namespace csmc {
 
 struct LatDev {
     int32_t N, nb, K2, K3, K4, ncol, nruns, has_self;
+    int32_t K2P;         // bilinear slots padded to a multiple of 4
+    int32_t flags;       // LAT_ONSITE | LAT_FIELD: some coupling is non-zero
     int64_t nbr_off;     // i32 pool
     int64_t ref_off;     // i32 pool
     int64_t sub_off;     // i32 pool
@@ -43,8 +49,12 @@ struct LatDev {
 struct SysDev {
     int32_t lat;      // lattice id
     int32_t stream;   // Philox stream word of this system
-    int64_t spin_off; // first spin4 of this system
+    int64_t spin_off; // first spin4 of this system (N + 1 records per system)
+    int64_t site_off; // first site of this system in host-facing outputs
 };
+
+constexpr int32_t LAT_ONSITE = 1;
+constexpr int32_t LAT_FIELD = 2;
 
 // incidence entry: owner device site and a packed (kind, slot, role)
 //   code = kind(2 bits: 0 bil, 1 cub, 2 qua) | role << 2 (2 bits) | slot << 4

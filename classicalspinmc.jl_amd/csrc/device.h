@@ -1,4 +1,4 @@
-// device.h -- gfx950 device building blocks of libcsmc: spin storage,
+// device.h -- gfx950 device building blocks of libcsmc: spin storage views,
 // Philox4x32-10, the local field / site energy of ClassicalSpinMC.jl's
 // hamiltonian.jl and the single-site updates of its sweeps.
 #pragma once
@@ -15,9 +15,18 @@ struct alignas(4 * sizeof(R)) spin4 {
 };
 
 // View of one lattice's device tables (see csmc_internal.h for the layout).
+// Read-only table data through the constant address space: with a
+// wave-uniform address the compiler emits scalar loads (and hoists them out
+// of site loops), which a plain global pointer does not allow because the
+// kernel also writes global memory.
+template <class T>
+__device__ __forceinline__ T ldc(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)(p);
+}
+
 template <class R>
 struct LatView {
-    int N, nb, K2, K3, K4, ncol, has_self;
+    int N, nb, K2, K2P, K3, K4, ncol, has_self, flags;
     const int32_t *nbr2, *nbr3, *nbr4, *ref, *sub, *colrun;
     const int4 *runs;
     const R *onsite, *field, *c2, *c3, *c4;
@@ -31,12 +40,14 @@ __device__ __forceinline__ LatView<R> lat_view(const LatDev &d, const int32_t *i
     v.N = d.N;
     v.nb = d.nb;
     v.K2 = d.K2;
+    v.K2P = d.K2P;
     v.K3 = d.K3;
     v.K4 = d.K4;
     v.ncol = d.ncol;
     v.has_self = d.has_self;
+    v.flags = d.flags;
     v.nbr2 = i32 + d.nbr_off;
-    v.nbr3 = v.nbr2 + (int64_t)d.K2 * d.N;
+    v.nbr3 = v.nbr2 + (int64_t)d.K2P * d.N;
     v.nbr4 = v.nbr3 + (int64_t)d.K3 * 2 * d.N;
     v.ref = i32 + d.ref_off;
     v.sub = i32 + d.sub_off;
@@ -46,11 +57,60 @@ __device__ __forceinline__ LatView<R> lat_view(const LatDev &d, const int32_t *i
     v.onsite = c;
     v.field = c + d.nb * 9;
     v.c2 = v.field + d.nb * 3;
-    v.c3 = v.c2 + (int64_t)d.nb * d.K2 * 9;
+    v.c3 = v.c2 + (int64_t)d.nb * d.K2P * 9;
     v.c4 = v.c3 + (int64_t)d.nb * d.K3 * 27;
     v.S = (R)d.S;
     return v;
 }
+
+// ---------------------------------------------------------------------------
+// spin storage views: spin4 records in global memory, or three component
+// arrays (12 B per FP32 site) in LDS
+// ---------------------------------------------------------------------------
+template <class R>
+struct SpinsAoS {
+    spin4<R> *p;
+    __device__ __forceinline__ spin4<R> ld(int i) const { return p[i]; }
+    __device__ __forceinline__ void st(int i, const spin4<R> &v) const { p[i] = v; }
+};
+
+template <class R>
+struct SpinsSoA {
+    R *x, *y, *z;
+    __device__ __forceinline__ spin4<R> ld(int i) const {
+        spin4<R> s;
+        s.x = x[i];
+        s.y = y[i];
+        s.z = z[i];
+        s.w = R(0);
+        return s;
+    }
+    __device__ __forceinline__ void st(int i, const spin4<R> &v) const {
+        x[i] = v.x;
+        y[i] = v.y;
+        z[i] = v.z;
+    }
+};
+
+// bilinear neighbour tables, slot-major, an absent neighbour is the zero pad
+// site N: int32 in global memory with all K2P slots, or uint16 in LDS with
+// the K2 real slots only (the padding slots then resolve to N without a load)
+// at(k, i) is the load for a slot known to be real (k < K2).
+struct NbrG {
+    const int32_t *p;
+    int N;
+    __device__ __forceinline__ int operator()(int k, int i) const { return p[(int64_t)k * N + i]; }
+    __device__ __forceinline__ int at(int k, int i) const { return p[(int64_t)k * N + i]; }
+};
+
+struct NbrL {
+    const uint16_t *p;
+    int N, K2;
+    __device__ __forceinline__ int operator()(int k, int i) const {
+        return k < K2 ? (int)p[k * N + i] : N;
+    }
+    __device__ __forceinline__ int at(int k, int i) const { return (int)p[k * N + i]; }
+};
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
@@ -94,7 +154,10 @@ struct Math<double> {
         o[1] = S * (r * sn);
         o[2] = S * z;
     }
-    static __device__ __forceinline__ double boltz(double dE, double T) { return exp(-dE / T); }
+    // Boltzmann factor exp(-dE/T), evaluated as the reference writes it
+    static __device__ __forceinline__ double boltz(double dE, double T, double) {
+        return exp(-dE / T);
+    }
     static __device__ __forceinline__ double rsqrt_(double x) { return 1.0 / sqrt(x); }
     static __device__ __forceinline__ double div(double a, double b) { return a / b; }
 };
@@ -111,7 +174,9 @@ struct Math<float> {
         o[1] = S * (r * __builtin_amdgcn_sinf(u1));
         o[2] = S * z;
     }
-    static __device__ __forceinline__ float boltz(float dE, float T) { return __expf(-dE / T); }
+    static __device__ __forceinline__ float boltz(float dE, float, float invT) {
+        return __expf(-dE * invT);
+    }
     static __device__ __forceinline__ float rsqrt_(float x) { return __builtin_amdgcn_rsqf(x); }
     static __device__ __forceinline__ float div(float a, float b) {
         float r = __builtin_amdgcn_rcpf(b);
@@ -123,105 +188,150 @@ struct Math<float> {
 // ---------------------------------------------------------------------------
 // Interaction field F_i = sum_n J_n s_j + sum C s_j s_k + sum R s_j s_k s_l
 // (the non-onsite, non-Zeeman part of get_local_field, hamiltonian.jl:24-65).
-// `ld(j)` returns the spin of device site j; basis b is the site's basis.
-// A coefficient read is uniform across the wave whenever b is.
+// `nb(k, i)` gives the bilinear neighbours, `ld(j)` the spin of device site j;
+// basis b is the site's basis (wave-uniform in the sweep kernels).  Bilinear
+// slots are taken four at a time with every index load issued before the
+// first use, so one memory round trip serves four neighbours; absent and
+// padding slots hit the zero pad site, so there is no branch.  Couplings are
+// read through the constant address space (scalar loads when b is uniform).
 // ---------------------------------------------------------------------------
-template <class R, class A, bool GEN = true, class LD>
-__device__ __forceinline__ void interaction_field(const LatView<R> &L, int b, int i, LD &&ld,
-                                                  A F[3]) {
+// K2C > 0: the lattice has exactly K2C bilinear slots (compile-time), so
+// the slot loop is fully unrolled with no padding work.
+template <class R, class A, bool GEN = true, int K2C = 0, class NB, class LD>
+__device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB &nb, int b, int i,
+                                                  LD &&ld, A F[3]) {
     F[0] = F[1] = F[2] = A(0);
-    for (int k = 0; k < L.K2; ++k) {
-        int j = L.nbr2[(int64_t)k * L.N + i];
-        if (j >= 0) {
-            spin4<R> sj = ld(j);
-            const R *M = L.c2 + (b * L.K2 + k) * 9;
-            F[0] += (A)M[0] * sj.x + (A)M[1] * sj.y + (A)M[2] * sj.z;
-            F[1] += (A)M[3] * sj.x + (A)M[4] * sj.y + (A)M[5] * sj.z;
-            F[2] += (A)M[6] * sj.x + (A)M[7] * sj.y + (A)M[8] * sj.z;
+    if constexpr (K2C > 0) {
+        int j[K2C];
+#pragma unroll
+        for (int k = 0; k < K2C; ++k) j[k] = nb.at(k, i);
+        spin4<R> sj[K2C];
+#pragma unroll
+        for (int k = 0; k < K2C; ++k) sj[k] = ld(j[k]);
+#pragma unroll
+        for (int k = 0; k < K2C; ++k) {
+            const R *M = L.c2 + (b * L.K2P + k) * 9;
+            A f0 = (A)ldc(M + 0) * sj[k].x + (A)ldc(M + 1) * sj[k].y + (A)ldc(M + 2) * sj[k].z;
+            A f1 = (A)ldc(M + 3) * sj[k].x + (A)ldc(M + 4) * sj[k].y + (A)ldc(M + 5) * sj[k].z;
+            A f2 = (A)ldc(M + 6) * sj[k].x + (A)ldc(M + 7) * sj[k].y + (A)ldc(M + 8) * sj[k].z;
+            if (k == 0) {
+                F[0] = f0;
+                F[1] = f1;
+                F[2] = f2;
+            } else {
+                F[0] += f0;
+                F[1] += f1;
+                F[2] += f2;
+            }
+        }
+    }
+    for (int k0 = 0; K2C == 0 && k0 < L.K2P; k0 += 4) {
+        int j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = nb(k0 + u, i);
+        spin4<R> sj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sj[u] = ld(j[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const R *M = L.c2 + (b * L.K2P + k0 + u) * 9;
+            F[0] += (A)ldc(M + 0) * sj[u].x + (A)ldc(M + 1) * sj[u].y + (A)ldc(M + 2) * sj[u].z;
+            F[1] += (A)ldc(M + 3) * sj[u].x + (A)ldc(M + 4) * sj[u].y + (A)ldc(M + 5) * sj[u].z;
+            F[2] += (A)ldc(M + 6) * sj[u].x + (A)ldc(M + 7) * sj[u].y + (A)ldc(M + 8) * sj[u].z;
         }
     }
     if (!GEN) return;
     for (int k = 0; k < L.K3; ++k) {
         int j = L.nbr3[((int64_t)k * 2) * L.N + i];
         int kk = L.nbr3[((int64_t)k * 2 + 1) * L.N + i];
-        if (j >= 0) {
-            spin4<R> a = ld(j), c = ld(kk);
-            A sj[3] = {a.x, a.y, a.z}, sk[3] = {c.x, c.y, c.z};
-            const R *C = L.c3 + (b * L.K3 + k) * 27;
+        spin4<R> a = ld(j), c = ld(kk);
+        A sj[3] = {a.x, a.y, a.z}, sk[3] = {c.x, c.y, c.z};
+        const R *C = L.c3 + (b * L.K3 + k) * 27;
 #pragma unroll
-            for (int p = 0; p < 3; ++p) {
-                A acc = 0;
+        for (int p = 0; p < 3; ++p) {
+            A acc = 0;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    A t = 0;
+            for (int q = 0; q < 3; ++q) {
+                A t = 0;
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) t += (A)C[p + 3 * q + 9 * r] * sk[r];
-                    acc += t * sj[q];
-                }
-                F[p] += acc;
+                for (int r = 0; r < 3; ++r) t += (A)ldc(C + p + 3 * q + 9 * r) * sk[r];
+                acc += t * sj[q];
             }
+            F[p] += acc;
         }
     }
     for (int k = 0; k < L.K4; ++k) {
         int j = L.nbr4[((int64_t)k * 3) * L.N + i];
         int kk = L.nbr4[((int64_t)k * 3 + 1) * L.N + i];
         int l = L.nbr4[((int64_t)k * 3 + 2) * L.N + i];
-        if (j >= 0) {
-            spin4<R> a = ld(j), c = ld(kk), e = ld(l);
-            A sj[3] = {a.x, a.y, a.z}, sk[3] = {c.x, c.y, c.z}, sl[3] = {e.x, e.y, e.z};
-            A w[9];
+        spin4<R> a = ld(j), c = ld(kk), e = ld(l);
+        A sj[3] = {a.x, a.y, a.z}, sk[3] = {c.x, c.y, c.z}, sl[3] = {e.x, e.y, e.z};
+        A w[9];
 #pragma unroll
-            for (int r = 0; r < 3; ++r)
+        for (int r = 0; r < 3; ++r)
 #pragma unroll
-                for (int t = 0; t < 3; ++t) w[r + 3 * t] = sk[r] * sl[t];
-            const R *Q = L.c4 + (b * L.K4 + k) * 81;
+            for (int t = 0; t < 3; ++t) w[r + 3 * t] = sk[r] * sl[t];
+        const R *Q = L.c4 + (b * L.K4 + k) * 81;
 #pragma unroll
-            for (int p = 0; p < 3; ++p) {
-                A acc = 0;
+        for (int p = 0; p < 3; ++p) {
+            A acc = 0;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    A t = 0;
+            for (int q = 0; q < 3; ++q) {
+                A t = 0;
 #pragma unroll
-                    for (int rt = 0; rt < 9; ++rt) t += (A)Q[p + 3 * q + 9 * rt] * w[rt];
-                    acc += t * sj[q];
-                }
-                F[p] += acc;
+                for (int rt = 0; rt < 9; ++rt) t += (A)ldc(Q + p + 3 * q + 9 * rt) * w[rt];
+                acc += t * sj[q];
             }
+            F[p] += acc;
         }
     }
 }
 
 // get_local_field (hamiltonian.jl:3-67): H = 2 O s + F - h
+// (the onsite and Zeeman parts are skipped by a uniform branch when the
+// lattice has none: 2 O s and h are then exactly zero)
 template <class R, class A>
 __device__ __forceinline__ void local_field(const LatView<R> &L, int b, const spin4<R> &s,
                                             const A F[3], A H[3]) {
     const R *O = L.onsite + b * 9;
     const R *h = L.field + b * 3;
     A sx = s.x, sy = s.y, sz = s.z;
-    H[0] = A(2) * ((A)O[0] * sx + (A)O[1] * sy + (A)O[2] * sz) + F[0] - (A)h[0];
-    H[1] = A(2) * ((A)O[3] * sx + (A)O[4] * sy + (A)O[5] * sz) + F[1] - (A)h[1];
-    H[2] = A(2) * ((A)O[6] * sx + (A)O[7] * sy + (A)O[8] * sz) + F[2] - (A)h[2];
+    H[0] = F[0];
+    H[1] = F[1];
+    H[2] = F[2];
+    if (L.flags & LAT_ONSITE) {
+        H[0] = A(2) * ((A)ldc(O + 0) * sx + (A)ldc(O + 1) * sy + (A)ldc(O + 2) * sz) + H[0];
+        H[1] = A(2) * ((A)ldc(O + 3) * sx + (A)ldc(O + 4) * sy + (A)ldc(O + 5) * sz) + H[1];
+        H[2] = A(2) * ((A)ldc(O + 6) * sx + (A)ldc(O + 7) * sy + (A)ldc(O + 8) * sz) + H[2];
+    }
+    if (L.flags & LAT_FIELD) {
+        H[0] -= (A)ldc(h + 0);
+        H[1] -= (A)ldc(h + 1);
+        H[2] -= (A)ldc(h + 2);
+    }
 }
 
 // s.O.s
 template <class R, class A>
 __device__ __forceinline__ A onsite_energy(const LatView<R> &L, int b, A sx, A sy, A sz) {
+    if (!(L.flags & LAT_ONSITE)) return A(0);
     const R *O = L.onsite + b * 9;
-    return sx * ((A)O[0] * sx + (A)O[1] * sy + (A)O[2] * sz) +
-           sy * ((A)O[3] * sx + (A)O[4] * sy + (A)O[5] * sz) +
-           sz * ((A)O[6] * sx + (A)O[7] * sy + (A)O[8] * sz);
+    return sx * ((A)ldc(O + 0) * sx + (A)ldc(O + 1) * sy + (A)ldc(O + 2) * sz) +
+           sy * ((A)ldc(O + 3) * sx + (A)ldc(O + 4) * sy + (A)ldc(O + 5) * sz) +
+           sz * ((A)ldc(O + 6) * sx + (A)ldc(O + 7) * sy + (A)ldc(O + 8) * sz);
 }
 
 // ---------------------------------------------------------------------------
-// single-site updates
+// single-site updates; X is a spin view, nb the bilinear neighbour view
 // ---------------------------------------------------------------------------
 
 // overrelaxation! body (monte_carlo.jl:128-137)
-template <class R, bool GEN, class SP>
-__device__ __forceinline__ void or_update(const LatView<R> &L, SP *S, int b, int i) {
-    spin4<R> s = S[i];
+template <class R, bool GEN, int K2C, class NB, class SV>
+__device__ __forceinline__ void or_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
+                                          int i) {
+    spin4<R> s = X.ld(i);
     R F[3], H[3];
-    interaction_field<R, R, GEN>(L, b, i, [&](int j) { return S[j]; }, F);
+    interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
     local_field<R, R>(L, b, s, F, H);
     if (H[0] == R(0) && H[1] == R(0) && H[2] == R(0)) return;
     R hh = H[0] * H[0] + H[1] * H[1] + H[2] * H[2];
@@ -231,15 +341,16 @@ __device__ __forceinline__ void or_update(const LatView<R> &L, SP *S, int b, int
     n.y = -s.y + proj * H[1];
     n.z = -s.z + proj * H[2];
     n.w = R(0);
-    S[i] = n;
+    X.st(i, n);
 }
 
 // deterministic_updates! body (monte_carlo.jl:203-209)
-template <class R, bool GEN, class SP>
-__device__ __forceinline__ void det_update(const LatView<R> &L, SP *S, int b, int i) {
-    spin4<R> s = S[i];
+template <class R, bool GEN, int K2C, class NB, class SV>
+__device__ __forceinline__ void det_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
+                                           int i) {
+    spin4<R> s = X.ld(i);
     R F[3], H[3];
-    interaction_field<R, R, GEN>(L, b, i, [&](int j) { return S[j]; }, F);
+    interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
     local_field<R, R>(L, b, s, F, H);
     if (H[0] == R(0) && H[1] == R(0) && H[2] == R(0)) return;
     R nrm = sqrt(H[0] * H[0] + H[1] * H[1] + H[2] * H[2]);
@@ -248,7 +359,7 @@ __device__ __forceinline__ void det_update(const LatView<R> &L, SP *S, int b, in
     n.y = -H[1] / nrm * L.S;
     n.z = -H[2] / nrm * L.S;
     n.w = R(0);
-    S[i] = n;
+    X.st(i, n);
 }
 
 // One Metropolis proposal (metropolis.jl:70-78 with calculate_energy_diff!,
@@ -258,11 +369,11 @@ __device__ __forceinline__ void det_update(const LatView<R> &L, SP *S, int b, in
 // SELF: some term of this lattice names the site itself, so the new energy
 // is evaluated with the candidate substituted in those slots as the
 // reference does (it writes the spin before calling energy()).
-template <class R, bool SELF, bool GEN, class SP>
-__device__ __forceinline__ int metro_update(const LatView<R> &L, SP *S, int b, int i, R T,
-                                            int kind, R sigma, uint32_t k0, uint32_t k1,
-                                            uint32_t stream, uint32_t ctr) {
-    spin4<R> s = S[i];
+template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
+__device__ __forceinline__ int metro_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
+                                            int i, R T, R invT, int kind, R sigma, uint32_t k0,
+                                            uint32_t k1, uint32_t stream, uint32_t ctr) {
+    spin4<R> s = X.ld(i);
     uint4 r = philox4x32_10(make_uint4((uint32_t)L.ref[i], stream, ctr, TAG_METRO), k0, k1);
     R u1 = Math<R>::u01(r.x), u2 = Math<R>::u01(r.y), u3 = Math<R>::u01(r.z);
     R rs[3];
@@ -280,28 +391,29 @@ __device__ __forceinline__ int metro_update(const LatView<R> &L, SP *S, int b, i
         n.z = d * inv;
     }
     n.w = R(0);
-    const R *h = L.field + b * 3;
+    const R *hp = L.field + b * 3;
+    const R h[3] = {ldc(hp), ldc(hp + 1), ldc(hp + 2)};
     R dE;
     if (!SELF) {
         R F[3];
-        interaction_field<R, R, GEN>(L, b, i, [&](int j) { return S[j]; }, F);
+        interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
         R eo_new = onsite_energy<R, R>(L, b, n.x, n.y, n.z);
         R eo_old = onsite_energy<R, R>(L, b, s.x, s.y, s.z);
         dE = (eo_new - eo_old) + (n.x - s.x) * (F[0] - (R)h[0]) + (n.y - s.y) * (F[1] - (R)h[1]) +
              (n.z - s.z) * (F[2] - (R)h[2]);
     } else {
         R Fo[3], Fn[3];
-        interaction_field<R, R, GEN>(L, b, i, [&](int j) { return S[j]; }, Fo);
-        interaction_field<R, R, GEN>(L, b, i, [&](int j) { return j == i ? n : (spin4<R>)S[j]; }, Fn);
+        interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, Fo);
+        interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return j == i ? n : X.ld(j); }, Fn);
         R eo = onsite_energy<R, R>(L, b, s.x, s.y, s.z) + s.x * Fo[0] + s.y * Fo[1] + s.z * Fo[2] -
                (s.x * h[0] + s.y * h[1] + s.z * h[2]);
         R en = onsite_energy<R, R>(L, b, n.x, n.y, n.z) + n.x * Fn[0] + n.y * Fn[1] + n.z * Fn[2] -
                (n.x * h[0] + n.y * h[1] + n.z * h[2]);
         dE = en - eo;
     }
-    int acc = (dE < R(0)) ? 1 : (u3 < Math<R>::boltz(dE, T));
+    int acc = (dE < R(0)) ? 1 : (u3 < Math<R>::boltz(dE, T, invT));
     if (acc) {
-        S[i] = n;
+        X.st(i, n);
     } else if (sizeof(R) == 4) {
         // FP32 state: pull |s| back to S on every visit so reflections
         // cannot accumulate rounding drift
@@ -309,13 +421,13 @@ __device__ __forceinline__ int metro_update(const LatView<R> &L, SP *S, int b, i
         s.x *= inv;
         s.y *= inv;
         s.z *= inv;
-        S[i] = s;
+        X.st(i, s);
     }
     return acc;
 }
 
 // ---------------------------------------------------------------------------
-// wave / workgroup reductions (wave64 DPP-free shuffles)
+// wave / workgroup reductions (wave64 shuffles)
 // ---------------------------------------------------------------------------
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -324,8 +436,8 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
-// sums n values per thread over the workgroup; result valid in thread 0.
-// scratch: at least 16*n doubles of LDS.
+// sums NV values per thread over the workgroup; result valid in thread 0.
+// scratch: at least 16*NV doubles of LDS.
 template <int NV>
 __device__ __forceinline__ void block_sum(double *v, double *scratch) {
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;

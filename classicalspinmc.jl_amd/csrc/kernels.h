@@ -60,8 +60,11 @@ struct PassArgs {
 };
 
 int max_lds_bytes();
-// gen: some lattice of the batch has cubic or quartic terms
-hipError_t launch_wg(int prec, bool lds, bool self, bool gen, int nsys, size_t lds_bytes,
+// lm: storage mode (0 global, 1 LDS spins, 2 LDS spins + uint16 neighbour
+// tables); self / gen: some lattice of the batch has a self-referencing
+// term / cubic or quartic terms; k2c: the bilinear slot count shared by
+// every lattice of the batch (0 if they differ)
+hipError_t launch_wg(int prec, int lm, bool self, bool gen, int k2c, int nsys, size_t lds_bytes,
                      const WGArgs &a, hipStream_t st);
 hipError_t launch_pass(int prec, bool self, bool gen, int nsys, int max_colour_sites,
                        const PassArgs &a, hipStream_t st);

@@ -1,11 +1,14 @@
 // kernels.hip -- gfx950 kernels of libcsmc.
 //
-// The sweep hot path is k_wg: one workgroup (16 wave64s) owns one system
-// (a replica or an independent lattice) for a whole segment of sweeps.  The
-// colour classes of the system are swept in turn with a workgroup barrier
-// between classes, so no grid-wide synchronisation is ever needed; when the
-// system fits, its spins live in LDS for the whole segment.  Parallel
-// tempering segments end with the replica exchange, done by the last
+// The sweep hot path is k_wg: one workgroup owns one system (a replica or an
+// independent lattice) for a whole segment of sweeps.  The colour classes of
+// the system are swept in turn with a workgroup barrier between classes, so
+// no grid-wide synchronisation is ever needed.  Storage modes (LM):
+//   0  spins and neighbour tables in global memory (L2 / MALL resident)
+//   1  spins in LDS as three FP component arrays, neighbour tables global
+//   2  spins and uint16 neighbour tables in LDS: a sweep touches no memory
+//      outside the CU
+// Parallel-tempering segments end with the replica exchange, done by the last
 // workgroup to finish (a completion counter, no waiting).
 //
 // k_pass is the plain form: one launch per colour pass, one thread per site.
@@ -37,26 +40,26 @@ __device__ __forceinline__ spin4<double> to_d(const spin4<R> &s) {
 
 // interaction field split by term kind (bilinear, cubic, quartic), FP64,
 // for energies and gradients
-template <bool GEN, class LD>
-__device__ __forceinline__ void field_parts(const LatView<double> &L, int b, int i, LD &&ld,
-                                            double F2[3], double F3[3], double F4[3]) {
-    interaction_field<double, double, false>(L, b, i, ld, F2);
+template <bool GEN, class NB, class LD>
+__device__ __forceinline__ void field_parts(const LatView<double> &L, const NB &nb, int b, int i,
+                                            LD &&ld, double F2[3], double F3[3], double F4[3]) {
+    interaction_field<double, double, false>(L, nb, b, i, ld, F2);
     F3[0] = F3[1] = F3[2] = F4[0] = F4[1] = F4[2] = 0.0;
     if (!GEN) return;
     LatView<double> Lc = L;
-    Lc.K2 = 0;
+    Lc.K2 = Lc.K2P = 0;
     Lc.K4 = 0;
-    interaction_field<double, double>(Lc, b, i, ld, F3);
+    interaction_field<double, double>(Lc, nb, b, i, ld, F3);
     LatView<double> Lq = L;
-    Lq.K2 = 0;
+    Lq.K2 = Lq.K2P = 0;
     Lq.K3 = 0;
-    interaction_field<double, double>(Lq, b, i, ld, F4);
+    interaction_field<double, double>(Lq, nb, b, i, ld, F4);
 }
 
 // ---------------------------------------------------------------------------
 // workgroup building blocks
 // ---------------------------------------------------------------------------
-template <class R, class SP, class OP>
+template <class R, class OP>
 __device__ __forceinline__ void wg_colour_sweep(const LatView<R> &L, OP &&op) {
     for (int c = 0; c < L.ncol; ++c) {
         for (int r = L.colrun[c]; r < L.colrun[c + 1]; ++r) {
@@ -70,17 +73,18 @@ __device__ __forceinline__ void wg_colour_sweep(const LatView<R> &L, OP &&op) {
 
 // FP64 energy parts (bilinear, cubic, quartic, zeeman, onsite) of a system,
 // summed over the workgroup; valid in every thread on return
-template <class R, bool GEN, class SP>
-__device__ void wg_energy(const LatView<double> &Ld, SP *X, double *red, double out[5]) {
+template <class R, bool GEN, class NB, class SV>
+__device__ void wg_energy(const LatView<double> &Ld, const NB &nb, const SV &X, double *red,
+                          double out[5]) {
     double p[5] = {0, 0, 0, 0, 0};
     for (int r = Ld.colrun[0]; r < Ld.colrun[Ld.ncol]; ++r) {
         int4 run = Ld.runs[r];
         int b = __builtin_amdgcn_readfirstlane(run.z);
         const double *h = Ld.field + b * 3;
         for (int i = run.x + (int)threadIdx.x; i < run.y; i += (int)blockDim.x) {
-            spin4<double> s = to_d<R>(X[i]);
+            spin4<double> s = to_d<R>(X.ld(i));
             double F2[3], F3[3], F4[3];
-            field_parts<GEN>(Ld, b, i, [&](int j) { return to_d<R>(X[j]); }, F2, F3, F4);
+            field_parts<GEN>(Ld, nb, b, i, [&](int j) { return to_d<R>(X.ld(j)); }, F2, F3, F4);
             p[0] += s.x * F2[0] + s.y * F2[1] + s.z * F2[2];
             p[1] += s.x * F3[0] + s.y * F3[1] + s.z * F3[2];
             p[2] += s.x * F4[0] + s.y * F4[1] + s.z * F4[2];
@@ -101,11 +105,11 @@ __device__ __forceinline__ double combine_energy(const double p[5]) {
     return p[0] / 2 + p[1] / 3 + p[2] / 4 + p[3] + p[4];
 }
 
-template <class R, class SP>
-__device__ void wg_magnet(int N, SP *X, double *red, double out[3]) {
+template <class R, class SV>
+__device__ void wg_magnet(int N, const SV &X, double *red, double out[3]) {
     double m[3] = {0, 0, 0};
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        spin4<R> s = X[i];
+        spin4<R> s = X.ld(i);
         m[0] += s.x;
         m[1] += s.y;
         m[2] += s.z;
@@ -118,12 +122,14 @@ __device__ void wg_magnet(int N, SP *X, double *red, double out[3]) {
     __syncthreads();
 }
 
-template <class R, bool SELF, bool GEN, class SP>
-__device__ double wg_metro(const LatView<R> &L, SP *X, R T, int kind, R sigma, uint32_t k0,
-                           uint32_t k1, uint32_t stream, uint32_t ctr, double *red) {
+template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
+__device__ double wg_metro(const LatView<R> &L, const NB &nb, const SV &X, R T, int kind, R sigma,
+                           uint32_t k0, uint32_t k1, uint32_t stream, uint32_t ctr, double *red) {
     int acc = 0;
-    wg_colour_sweep<R, SP>(L, [&](int b, int i) {
-        acc += metro_update<R, SELF, GEN>(L, X, b, i, T, kind, sigma, k0, k1, stream, ctr);
+    const R invT = R(1) / T;
+    wg_colour_sweep<R>(L, [&](int b, int i) {
+        acc += metro_update<R, SELF, GEN, K2C>(L, nb, X, b, i, T, invT, kind, sigma, k0, k1, stream,
+                                               ctr);
     });
     double v[1] = {(double)acc};
     block_sum<1>(v, red);
@@ -168,41 +174,27 @@ __device__ void bin_push(double *st, double v) {
 // ---------------------------------------------------------------------------
 // k_wg: persistent workgroup-per-system sweeps
 // ---------------------------------------------------------------------------
-template <class R, bool LDS, bool SELF, bool GEN>
-__global__ __launch_bounds__((GEN || sizeof(R) == 8) ? 512 : 1024) void k_wg(WGArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    double *red = reinterpret_cast<double *>(smem);  // 1 KB reduction scratch
-    spin4<R> *lds = reinterpret_cast<spin4<R> *>(smem + 1024);
-
-    const int sys = blockIdx.x;
-    const SysDev sd = a.P.sys[sys];
-    const LatDev ld = a.P.lats[sd.lat];
-    const int4 *runs = a.P.runs + ld.run_off;
-    const LatView<R> L = lat_view<R>(ld, a.P.i32, runs, coef_of<R>(a.P));
-    const LatView<double> Ld = lat_view<double>(ld, a.P.i32, runs, a.P.coefD);
-    spin4<R> *G = reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off;
-    spin4<R> *X = LDS ? lds : G;
-    if (LDS) {
-        for (int i = threadIdx.x; i < L.N; i += blockDim.x) lds[i] = G[i];
-        __syncthreads();
-    }
-    const uint32_t stream = (uint32_t)sd.stream;
+template <class R, bool SELF, bool GEN, int MODE, int K2C, class SV, class NB>
+__device__ void wg_run(const WGArgs &a, int sys, const LatView<R> &L, const LatView<double> &Ld,
+                       const SV &X, const NB &nb, double *red) {
+    const uint32_t stream = (uint32_t)a.P.sys[sys].stream;
     uint32_t mctr = a.mcount[sys];
 
     auto or_sweep = [&]() {
-        wg_colour_sweep<R, spin4<R>>(L, [&](int b, int i) { or_update<R, GEN>(L, X, b, i); });
+        wg_colour_sweep<R>(L, [&](int b, int i) { or_update<R, GEN, K2C>(L, nb, X, b, i); });
     };
     auto det_sweep = [&]() {
-        wg_colour_sweep<R, spin4<R>>(L, [&](int b, int i) { det_update<R, GEN>(L, X, b, i); });
+        wg_colour_sweep<R>(L, [&](int b, int i) { det_update<R, GEN, K2C>(L, nb, X, b, i); });
     };
     auto metro_sweep = [&](R T, int kind, double &sigma) {
-        double acc = wg_metro<R, SELF, GEN>(L, X, T, kind, (R)sigma, a.k0, a.k1, stream, mctr, red);
+        double acc =
+            wg_metro<R, SELF, GEN, K2C>(L, nb, X, T, kind, (R)sigma, a.k0, a.k1, stream, mctr, red);
         ++mctr;
         if (kind == CSMC_CONE_ADAPTIVE) sigma = adapt_sigma(sigma, acc, (double)L.N);
         return acc;
     };
 
-    if (a.mode == WG_SWEEPS) {
+    if constexpr (MODE == WG_SWEEPS) {
         const R T = (R)a.T_sys[sys];
         double sigma = a.sigma_sys ? a.sigma_sys[sys] : 60.0;
         double acc = 0.0;
@@ -215,7 +207,7 @@ __global__ __launch_bounds__((GEN || sizeof(R) == 8) ? 512 : 1024) void k_wg(WGA
             if (a.acc_sys) a.acc_sys[sys] = acc;
             if (a.sigma_sys) a.sigma_sys[sys] = sigma;
         }
-    } else if (a.mode == WG_ANNEAL) {
+    } else if constexpr (MODE == WG_ANNEAL) {
         // simulated_annealing! (monte_carlo.jl:166-187), t = 1 .. t_th-1
         for (int k = 0; k < a.n_temps; ++k) {
             const R T = (R)a.temps[k];
@@ -251,7 +243,7 @@ __global__ __launch_bounds__((GEN || sizeof(R) == 8) ? 512 : 1024) void k_wg(WGA
                 if (sw % pt.dosweep == 0) {
                     acc_local += metro_sweep(T, pt.alg, sigma);
                     double p[5];
-                    wg_energy<R, GEN>(Ld, X, red, p);
+                    wg_energy<R, GEN>(Ld, nb, X, red, p);
                     E = combine_energy(p);
                 }
             }
@@ -270,14 +262,51 @@ __global__ __launch_bounds__((GEN || sizeof(R) == 8) ? 512 : 1024) void k_wg(WGA
             pt.acc_local[slot] += acc_local;
         }
     }
-
     if (threadIdx.x == 0) a.mcount[sys] = mctr;
-    if (LDS) {
+}
+
+__device__ __forceinline__ size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
+
+// GEN (cubic / quartic terms) and SELF (a term names its own site) are
+// launched together: the general variant is exact for every lattice, the
+// fast one for bilinear-only lattices without self references.  K2C > 0
+// specialises the bilinear slot count (all lattices of the batch have it).
+template <class R, int LM, bool SELF, bool GEN, int MODE, int K2C>
+__global__ __launch_bounds__((GEN || sizeof(R) == 8) ? 512 : 1024) void k_wg(WGArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *red = reinterpret_cast<double *>(smem);  // 1 KB reduction scratch
+
+    const int sys = blockIdx.x;
+    const SysDev sd = a.P.sys[sys];
+    const LatDev ld = a.P.lats[sd.lat];
+    const int4 *runs = a.P.runs + ld.run_off;
+    const LatView<R> L = lat_view<R>(ld, a.P.i32, runs, coef_of<R>(a.P));
+    const LatView<double> Ld = lat_view<double>(ld, a.P.i32, runs, a.P.coefD);
+    spin4<R> *G = reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off;
+    const int N = L.N;
+
+    if constexpr (LM == 0) {
+        wg_run<R, SELF, GEN, MODE, K2C>(a, sys, L, Ld, SpinsAoS<R>{G}, NbrG{L.nbr2, N}, red);
+    } else {
+        // N + 1 entries per component: the last one is the zero pad site
+        R *sx = reinterpret_cast<R *>(smem + 1024);
+        SpinsSoA<R> X{sx, sx + (N + 1), sx + 2 * (N + 1)};
+        for (int i = threadIdx.x; i <= N; i += blockDim.x) X.st(i, G[i]);
+        if constexpr (LM == 1) {
+            __syncthreads();
+            wg_run<R, SELF, GEN, MODE, K2C>(a, sys, L, Ld, X, NbrG{L.nbr2, N}, red);
+        } else {
+            uint16_t *idx =
+                reinterpret_cast<uint16_t *>(smem + 1024 + align16(3 * sizeof(R) * (N + 1)));
+            for (int t = threadIdx.x; t < L.K2 * N; t += blockDim.x) idx[t] = (uint16_t)L.nbr2[t];
+            __syncthreads();
+            wg_run<R, SELF, GEN, MODE, K2C>(a, sys, L, Ld, X, NbrL{idx, N, L.K2}, red);
+        }
         __syncthreads();
-        for (int i = threadIdx.x; i < L.N; i += blockDim.x) G[i] = lds[i];
+        for (int i = threadIdx.x; i < N; i += blockDim.x) G[i] = X.ld(i);
     }
 
-    if (a.mode == WG_PT && a.exchange) {
+    if (MODE == WG_PT && a.exchange) {
         // replica exchange (monte_carlo.jl:308-349) by the last workgroup
         __shared__ int last;
         __threadfence();
@@ -322,34 +351,60 @@ int max_lds_bytes() {
     return v;
 }
 
-template <class R, bool LDS, bool SELF, bool GEN>
+template <class R, int LM, bool GENERAL, int MODE, int K2C>
 static hipError_t launch_wg_t(int nsys, size_t lds_bytes, const WGArgs &a, hipStream_t st) {
-    size_t bytes = 1024 + (LDS ? lds_bytes : 0);
+    size_t bytes = 1024 + (LM ? lds_bytes : 0);
+    const void *fn = (const void *)k_wg<R, LM, GENERAL, GENERAL, MODE, K2C>;
     if (bytes > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_wg<R, LDS, SELF, GEN>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_wg<R, LDS, SELF, GEN>), dim3(nsys), dim3((GEN || sizeof(R) == 8) ? 512 : 1024), bytes, st, a);
+    const int threads = (GENERAL || sizeof(R) == 8) ? 512 : 1024;
+    hipLaunchKernelGGL((k_wg<R, LM, GENERAL, GENERAL, MODE, K2C>), dim3(nsys), dim3(threads), bytes,
+                       st, a);
     return hipGetLastError();
 }
 
-template <class R, bool LDS>
-static hipError_t launch_wg_r(bool self, bool gen, int nsys, size_t lds_bytes, const WGArgs &a,
+template <class R, int LM, bool GENERAL, int MODE>
+static hipError_t launch_wg_k(int k2c, int nsys, size_t lds_bytes, const WGArgs &a,
                               hipStream_t st) {
-    if (gen) return self ? launch_wg_t<R, LDS, true, true>(nsys, lds_bytes, a, st)
-                         : launch_wg_t<R, LDS, false, true>(nsys, lds_bytes, a, st);
-    return self ? launch_wg_t<R, LDS, true, false>(nsys, lds_bytes, a, st)
-                : launch_wg_t<R, LDS, false, false>(nsys, lds_bytes, a, st);
+    // slot-count specialisations exist for the FP32 fast path in LDS modes
+    if constexpr (sizeof(R) == 4 && !GENERAL && LM > 0) {
+        if (k2c == 3) return launch_wg_t<R, LM, GENERAL, MODE, 3>(nsys, lds_bytes, a, st);
+        if (k2c == 4) return launch_wg_t<R, LM, GENERAL, MODE, 4>(nsys, lds_bytes, a, st);
+        if (k2c == 6) return launch_wg_t<R, LM, GENERAL, MODE, 6>(nsys, lds_bytes, a, st);
+    }
+    return launch_wg_t<R, LM, GENERAL, MODE, 0>(nsys, lds_bytes, a, st);
 }
 
-hipError_t launch_wg(int prec, bool lds, bool self, bool gen, int nsys, size_t lds_bytes,
+template <class R, int LM, bool GENERAL>
+static hipError_t launch_wg_g(int mode, int k2c, int nsys, size_t lds_bytes, const WGArgs &a,
+                              hipStream_t st) {
+    if (mode == WG_PT) return launch_wg_k<R, LM, GENERAL, WG_PT>(k2c, nsys, lds_bytes, a, st);
+    if (mode == WG_ANNEAL) return launch_wg_k<R, LM, GENERAL, WG_ANNEAL>(k2c, nsys, lds_bytes, a, st);
+    return launch_wg_k<R, LM, GENERAL, WG_SWEEPS>(k2c, nsys, lds_bytes, a, st);
+}
+
+template <class R, int LM>
+static hipError_t launch_wg_m(bool general, int k2c, int nsys, size_t lds_bytes, const WGArgs &a,
+                              hipStream_t st) {
+    return general ? launch_wg_g<R, LM, true>(a.mode, k2c, nsys, lds_bytes, a, st)
+                   : launch_wg_g<R, LM, false>(a.mode, k2c, nsys, lds_bytes, a, st);
+}
+
+template <class R>
+static hipError_t launch_wg_r(int lm, bool general, int k2c, int nsys, size_t lds_bytes,
+                              const WGArgs &a, hipStream_t st) {
+    if (lm == 2) return launch_wg_m<R, 2>(general, k2c, nsys, lds_bytes, a, st);
+    if (lm == 1) return launch_wg_m<R, 1>(general, k2c, nsys, lds_bytes, a, st);
+    return launch_wg_m<R, 0>(general, k2c, nsys, lds_bytes, a, st);
+}
+
+hipError_t launch_wg(int prec, int lm, bool self, bool gen, int k2c, int nsys, size_t lds_bytes,
                      const WGArgs &a, hipStream_t st) {
-    if (prec == 32)
-        return lds ? launch_wg_r<float, true>(self, gen, nsys, lds_bytes, a, st)
-                   : launch_wg_r<float, false>(self, gen, nsys, lds_bytes, a, st);
-    return lds ? launch_wg_r<double, true>(self, gen, nsys, lds_bytes, a, st)
-               : launch_wg_r<double, false>(self, gen, nsys, lds_bytes, a, st);
+    const bool general = self || gen;
+    if (prec == 32) return launch_wg_r<float>(lm, general, k2c, nsys, lds_bytes, a, st);
+    return launch_wg_r<double>(lm, general, k2c, nsys, lds_bytes, a, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -368,16 +423,18 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a) {
     const int i = start + blockIdx.x * blockDim.x + threadIdx.x;
     int acc = 0;
     if (i < end) {
-        spin4<R> *S = reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off;
+        const SpinsAoS<R> X{reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off};
+        const NbrG nb{L.nbr2, L.N};
         const int b = L.sub[i];
         if (a.kind == CSMC_OVERRELAX) {
-            or_update<R, GEN>(L, S, b, i);
+            or_update<R, GEN, 0>(L, nb, X, b, i);
         } else if (a.kind == CSMC_DETERMINISTIC) {
-            det_update<R, GEN>(L, S, b, i);
+            det_update<R, GEN, 0>(L, nb, X, b, i);
         } else {
             R sigma = a.sigma_sys ? (R)a.sigma_sys[sys] : R(60);
-            acc = metro_update<R, SELF, GEN>(L, S, b, i, (R)a.T_sys[sys], a.kind, sigma, a.k0, a.k1,
-                                        (uint32_t)sd.stream, a.ctr ? a.ctr[sys] : a.ctr0);
+            R T = (R)a.T_sys[sys];
+            acc = metro_update<R, SELF, GEN, 0>(L, nb, X, b, i, T, R(1) / T, a.kind, sigma, a.k0,
+                                                a.k1, (uint32_t)sd.stream, a.ctr ? a.ctr[sys] : a.ctr0);
         }
     }
     if (a.kind != CSMC_OVERRELAX && a.kind != CSMC_DETERMINISTIC) {
@@ -420,7 +477,7 @@ __global__ __launch_bounds__(256) void k_site_eval(DevPools P, const void *spins
     int b = L.sub[i];
     spin4<double> s = to_d<R>(S[i]);
     double F[3], Hl[3];
-    interaction_field<double, double>(L, b, i, [&](int j) { return to_d<R>(S[j]); }, F);
+    interaction_field<double, double>(L, NbrG{L.nbr2, L.N}, b, i, [&](int j) { return to_d<R>(S[j]); }, F);
     local_field<double, double>(L, b, s, F, Hl);
     int r = L.ref[i];
     if (H) {
@@ -451,9 +508,9 @@ __global__ __launch_bounds__(1024) void k_energy(DevPools P, const void *spins, 
     const SysDev sd = P.sys[sys];
     const LatDev ld = P.lats[sd.lat];
     const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
-    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
+    const SpinsAoS<R> S{const_cast<spin4<R> *>(reinterpret_cast<const spin4<R> *>(spins)) + sd.spin_off};
     double p[5];
-    wg_energy<R, true>(L, S, red, p);
+    wg_energy<R, true>(L, NbrG{L.nbr2, L.N}, S, red, p);
     if (threadIdx.x == 0) {
         E[sys] = combine_energy(p);
         if (parts)
@@ -474,7 +531,7 @@ __global__ __launch_bounds__(1024) void k_magnet(DevPools P, const void *spins, 
     const int sys = blockIdx.x;
     const SysDev sd = P.sys[sys];
     const LatDev ld = P.lats[sd.lat];
-    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
+    const SpinsAoS<R> S{const_cast<spin4<R> *>(reinterpret_cast<const spin4<R> *>(spins)) + sd.spin_off};
     double m[3];
     wg_magnet<R>(ld.N, S, red, m);
     if (threadIdx.x == 0)
@@ -597,7 +654,7 @@ __global__ __launch_bounds__(256) void k_grad_spins(DevPools P, const void *spin
         const int i = in.owner, kind = in.code & 3, role = (in.code >> 2) & 3, k = in.code >> 4;
         const int b = L.sub[i];
         if (kind == 0) {
-            const double *M = L.c2 + (b * L.K2 + k) * 9;
+            const double *M = L.c2 + (b * L.K2P + k) * 9;
             spin4<double> si = sp(i), sj = sp(L.nbr2[(int64_t)k * L.N + i]);
             if (role == 0) {
                 for (int a = 0; a < 3; ++a) g[a] += 0.5 * (M[a * 3] * sj.x + M[a * 3 + 1] * sj.y + M[a * 3 + 2] * sj.z);
@@ -695,7 +752,7 @@ __global__ __launch_bounds__(1024) void k_grad_couplings(DevPools P, const void 
         for (int k = 0; k < L.K2; ++k) {
             int j = L.nbr2[(int64_t)k * L.N + i];
             int t = t2[b * L.K2 + k];
-            if (j < 0 || t < 0) continue;
+            if (j >= L.N || t < 0) continue;
             spin4<double> q = sp(j);
             double sj[3] = {q.x, q.y, q.z};
             bool rev = r2[b * L.K2 + k] != 0;
@@ -708,7 +765,7 @@ __global__ __launch_bounds__(1024) void k_grad_couplings(DevPools P, const void 
         for (int k = 0; k < L.K3; ++k) {
             int j = L.nbr3[((int64_t)k * 2) * L.N + i], kk = L.nbr3[((int64_t)k * 2 + 1) * L.N + i];
             int t = t3[b * L.K3 + k];
-            if (j < 0 || t < 0) continue;
+            if (j >= L.N || t < 0) continue;
             int role = r3[b * L.K3 + k];
             spin4<double> q1 = sp(j), q2 = sp(kk);
             double w[3][3] = {{si[0], si[1], si[2]}, {q1.x, q1.y, q1.z}, {q2.x, q2.y, q2.z}};
@@ -728,7 +785,7 @@ __global__ __launch_bounds__(1024) void k_grad_couplings(DevPools P, const void 
             int kk = L.nbr4[((int64_t)k * 3 + 1) * L.N + i];
             int l = L.nbr4[((int64_t)k * 3 + 2) * L.N + i];
             int t = t4[b * L.K4 + k];
-            if (j < 0 || t < 0) continue;
+            if (j >= L.N || t < 0) continue;
             int role = r4[b * L.K4 + k];
             spin4<double> q1 = sp(j), q2 = sp(kk), q3 = sp(l);
             double w[4][3] = {{si[0], si[1], si[2]}, {q1.x, q1.y, q1.z}, {q2.x, q2.y, q2.z},
