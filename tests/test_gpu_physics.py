@@ -92,11 +92,6 @@ def _oracle_chain(T, spins, temp, S, n_sweeps, or_rate, seed):
     return np.array(Es), np.array(Ms)
 
 
-def _binned(x, nb=40):
-    x = x[: len(x) // nb * nb].reshape(nb, -1).mean(axis=1)
-    return x.mean(), x.std(ddof=1) / math.sqrt(nb)
-
-
 @pytest.mark.parametrize("precision", [64, 32])
 def test_statistical_parity_with_reference_chain(precision):
     # square Heisenberg FM L=6 at T = 1.2 (paramagnet, short correlation
@@ -108,15 +103,15 @@ def test_statistical_parity_with_reference_chain(precision):
     lat = csmc.Lattice((L, L), uc, S, seed=4)
     N = lat.size
     T = orc.Tables(uc, (L, L))
-    n_sw = 100000
+    n_sw, n_ch = 50000, 8
     chains_E, chains_M = [], []
-    for c in range(4):
+    for c in range(n_ch):
         s = np.ascontiguousarray(lat._spins.copy())
         Eo, Mo = _oracle_chain(T, s, temp, S, n_sw, 2, 12345 + c)
         chains_E.append(Eo[1000:].mean())
         chains_M.append(Mo[1000:].mean())
-    me_o, de_o = np.mean(chains_E), np.std(chains_E, ddof=1) / 2.0
-    mm_o, dm_o = np.mean(chains_M), np.std(chains_M, ddof=1) / 2.0
+    me_o, de_o = np.mean(chains_E), np.std(chains_E, ddof=1) / math.sqrt(n_ch)
+    mm_o, dm_o = np.mean(chains_M), np.std(chains_M, ddof=1) / math.sqrt(n_ch)
     R = 8  # independent replicas at the same temperature (no exchange effect)
     mcs = [csmc.MonteCarlo(temp, lat, {"t_thermalization": 2000, "t_measurement": n_sw,
                                        "probe_rate": 2, "swap_rate": 2, "overrelaxation_rate": 2,
