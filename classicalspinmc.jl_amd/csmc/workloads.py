@@ -136,3 +136,32 @@ def triangular_j1j2(J1=1.0, J2=0.1):
 
 def geometric_temperatures(Tmin, Tmax, R):
     return Tmin * (Tmax / Tmin) ** (np.arange(R) / max(R - 1, 1))
+
+
+def uneven_batch(n=256, sizes=tuple(range(16, 129, 8)), seed=3):
+    """independent lattices of uneven size for one batched launch: the first
+    half square, the second half triangular, side L drawn uniformly from
+    `sizes`; nearest-neighbour a*I + D*[z]x (a ~ N(-1, 0.2), D ~ N(0, 0.1))
+    and a Zeeman field of random direction with |h| ~ U(0, 0.5), all per
+    lattice.  Returns [(unit cell, (L, L))]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        L = int(rng.choice(sizes))
+        a, D = rng.normal(-1.0, 0.2), rng.normal(0.0, 0.1)
+        u = rng.normal(size=3)
+        h = rng.uniform(0.0, 0.5) * u / np.linalg.norm(u)
+        uc = square_dm(a, D, h) if k < n // 2 else triangular_dm(a, D, h)
+        out.append((uc, (L, L)))
+    return out
+
+
+def anneal_temperatures(T0=1.0, rate=0.9, Tmin=0.05):
+    """simulated_annealing!'s temperature list for schedule T0 * rate^x
+    (monte_carlo.jl:166-187): T0, schedule(1), ... while T > Tmin"""
+    temps, T, x = [], T0, 1
+    while T > Tmin:
+        temps.append(T)
+        T = T0 * rate ** x
+        x += 1
+    return temps
