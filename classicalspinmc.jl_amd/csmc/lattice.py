@@ -69,8 +69,15 @@ class Lattice:
         rng = np.random.default_rng(seed)
         self._spins = np.zeros((self.size, 3))
         if initialCondition == "random" or initialCondition == ":random":
-            for i in range(self.size):
-                self._spins[i] = random_spin_orientation(S, rng)
+            # random_spin_orientation per site, vectorised: the same two
+            # uniforms per site, in site order, as the scalar loop draws
+            u = rng.random(2 * self.size).reshape(self.size, 2)
+            phi = 2.0 * math.pi * u[:, 0]
+            z = 2.0 * u[:, 1] - 1.0
+            r = np.sqrt(1.0 - z * z)
+            self._spins[:, 0] = S * (r * np.cos(phi))
+            self._spins[:, 1] = S * (r * np.sin(phi))
+            self._spins[:, 2] = S * z
         elif initialCondition == "fm" or initialCondition == ":fm":
             self._spins[:] = random_spin_orientation(S, rng)
         else:
@@ -156,6 +163,17 @@ class Lattice:
                                                       self._periodic, float(self.S), C.byref(lid)))
             self._dev[key] = lid.value
         return self._dev[key]
+
+    def update_couplings(self, uc):
+        """replace the couplings of this lattice by those of `uc`, a unit
+        cell with the same bases, offsets and term order (for fitting
+        couplings without rebuilding tables); device copies are updated in
+        place, so batches holding this lattice see the new values"""
+        s, keep = uc._pack()
+        for key, lid in self._dev.items():
+            _lib.check(_lib.lib().csmc_lattice_update_couplings(C.c_void_p(key), lid, C.byref(s)))
+        self.unit_cell = uc
+        self._tables = None
 
     def _eval_batch(self, ctx=None):
         ctx = ctx or _lib.default_context()

@@ -106,6 +106,39 @@ class UnitCell:
         return (3 * len(self.field) + 9 * len(self.onsite) + 9 * len(self.bilinear)
                 + 27 * len(self.cubic) + 81 * len(self.quartic))
 
+    def params(self):
+        """every coupling as one float64 vector, in the order and layout of
+        the coupling gradient (cubic / quartic tensors column-major)"""
+        parts = [np.asarray(h, float).reshape(3) for _, h in self.field]
+        parts += [np.asarray(M, float).reshape(9) for _, M in self.onsite]
+        parts += [np.asarray(t[2], float).reshape(9) for t in self.bilinear]
+        parts += [np.asarray(t[3], float).flatten(order="F") for t in self.cubic]
+        parts += [np.asarray(t[4], float).flatten(order="F") for t in self.quartic]
+        return np.concatenate(parts) if parts else np.zeros(0)
+
+    def with_params(self, theta):
+        """a copy of this unit cell with the couplings replaced by `theta`
+        (layout of params()); bases, offsets and term order are kept"""
+        theta = np.asarray(theta, dtype=np.float64).reshape(-1)
+        if theta.size != self.n_params():
+            raise ValueError(f"expected {self.n_params()} couplings, got {theta.size}")
+        u = UnitCell(*self.lattice_vectors)
+        u.basis = [np.array(b) for b in self.basis]
+        k = 0
+
+        def take(n, shape, order="C"):
+            nonlocal k
+            v = theta[k:k + n].reshape(shape, order=order).copy()
+            k += n
+            return v
+
+        u.field = [(b, take(3, (3,))) for b, _ in self.field]
+        u.onsite = [(b, take(9, (3, 3))) for b, _ in self.onsite]
+        u.bilinear = [(t[0], t[1], take(9, (3, 3)), t[3]) for t in self.bilinear]
+        u.cubic = [t[:3] + (take(27, (3, 3, 3), "F"),) + t[4:] for t in self.cubic]
+        u.quartic = [t[:4] + (take(81, (3, 3, 3, 3), "F"),) + t[5:] for t in self.quartic]
+        return u
+
 
 def addBasisSite(uc, site):
     """addBasisSite! (unit_cell.jl:23-25)."""

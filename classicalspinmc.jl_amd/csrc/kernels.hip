@@ -703,7 +703,7 @@ __global__ __launch_bounds__(256) void k_grad_spins(DevPools P, const void *spin
         }
     }
     int r = L.ref[x];
-    double *o = dEdS + 3 * (sd.spin_off + r);
+    double *o = dEdS + 3 * (sd.site_off + r);
     o[0] = g[0];
     o[1] = g[1];
     o[2] = g[2];
@@ -833,7 +833,7 @@ __global__ __launch_bounds__(256) void k_gd(DevPools P, void *spins, const doubl
     int r = P.i32[ld.ref_off + i];
     spin4<R> *S = reinterpret_cast<spin4<R> *>(spins) + sd.spin_off;
     spin4<R> s = S[i];
-    const double *g = dEdS + 3 * (sd.spin_off + r);
+    const double *g = dEdS + 3 * (sd.site_off + r);
     double sx = s.x, sy = s.y, sz = s.z;
     double n = sqrt(sx * sx + sy * sy + sz * sz);
     double ux = sx / n, uy = sy / n, uz = sz / n;

@@ -147,3 +147,85 @@ def test_gradient_descent_reaches_deterministic_energy():
     Edet = det.total_energy()[0] / lat.size
     assert E0 > -2.0
     assert abs(Egd + 2.1) < 1e-9 and abs(Edet + 2.1) < 1e-9, (Egd, Edet)
+
+
+def test_autograd_function_matches_adjoint_and_fd():
+    # csmc.autodiff.total_energy: the torch backward equals the adjoint
+    # kernel and central differences of the oracle energy, with weights
+    torch = pytest.importorskip("torch")
+    from csmc import autodiff
+    uc = _random_cell()
+    shape = (3, 4)
+    lat = csmc.Lattice(shape, uc, 1.0, seed=9)
+    T = orc.Tables(uc, shape)
+    rng = np.random.default_rng(4)
+    n = 3
+    confs = np.concatenate([random_spins(lat.size, 1.0, rng) for _ in range(n)])
+    b = csmc.SpinBatch([lat] * n, precision=64)
+    S = torch.tensor(confs, requires_grad=True)
+    E = autodiff.total_energy(b, S)
+    for r in range(n):
+        assert abs(E[r].item() - orc.total_energy(T, confs[r * lat.size:(r + 1) * lat.size])) < 1e-10
+    w = torch.tensor([0.5, -1.0, 2.0], dtype=torch.float64)
+    (E * w).sum().backward()
+    g = S.grad.numpy()
+    for r in range(n):
+        s = confs[r * lat.size:(r + 1) * lat.size]
+        i, a = 5, 1
+        sp, sm = s.copy(), s.copy()
+        sp[i, a] += EPS
+        sm[i, a] -= EPS
+        fd = (orc.total_energy(T, sp) - orc.total_energy(T, sm)) / (2 * EPS)
+        assert abs(g[r * lat.size + i, a] - w[r].item() * fd) < 1e-6 * max(1.0, abs(fd))
+
+
+def test_fit_couplings_by_gradient():
+    # dE/dJ drives a least-squares fit of J2 on the triangular J1-J2 model:
+    # energies of 24 configurations at J2 = 0.13 are the targets, the fit
+    # starts from J2 = 0 and must recover 0.13
+    torch = pytest.importorskip("torch")
+    from csmc import autodiff
+    uc = W.triangular_j1j2(1.0, 0.13)
+    lat = csmc.Lattice((12, 12), uc, 1.0, seed=5)
+    n = 24
+    rng = np.random.default_rng(6)
+    confs = np.concatenate([random_spins(lat.size, 1.0, rng) for _ in range(n)])
+    b = csmc.SpinBatch([lat] * n, precision=64)
+    S = torch.tensor(confs)
+    target = autodiff.total_energy(b, S).detach()
+    I3 = torch.eye(3, dtype=torch.float64).reshape(9)
+    J2 = torch.zeros(1, dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.LBFGS([J2], lr=1.0, max_iter=50, line_search_fn="strong_wolfe")
+
+    def closure():
+        opt.zero_grad()
+        theta = torch.cat([I3.repeat(6), (J2 * I3).repeat(6)])
+        loss = ((autodiff.total_energy(b, S, theta) - target) ** 2).mean()
+        loss.backward()
+        return loss
+
+    opt.step(closure)
+    assert abs(J2.item() - 0.13) < 1e-8, J2.item()
+
+
+def test_batched_gradients_equal_single_system():
+    # every system of a batch gets its own dE/dS (site offsets of the
+    # concatenated output) and its own gradient step
+    uc = _random_cell()
+    lats = [csmc.Lattice((3, 4), uc, 1.0, seed=30 + k) for k in range(3)]
+    b = csmc.SpinBatch(lats, precision=64)
+    for k, lat in enumerate(lats):
+        b.set_spins(k, lat._spins)
+    g = b.grad_spins()
+    for k, lat in enumerate(lats):
+        one = csmc.SpinBatch([lat], precision=64)
+        one.set_spins(0, lat._spins)
+        assert np.array_equal(g[k], one.grad_spins()[0]), k
+        one.gradient_descent(0.01, 3)
+        if k == len(lats) - 1:
+            b.gradient_descent(0.01, 3)
+            for j, lj in enumerate(lats):
+                ref = csmc.SpinBatch([lj], precision=64)
+                ref.set_spins(0, lj._spins)
+                ref.gradient_descent(0.01, 3)
+                assert np.array_equal(b.get_spins(j), ref.get_spins(0)), j

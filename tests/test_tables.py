@@ -132,3 +132,23 @@ def test_norm_spin_and_params_buffer():
     with pytest.warns(UserWarning, match="not a valid MC parameter"):
         p = csmc.MCParamsBuffer(d)
     assert d["overrelaxation_rate"] == 10 and p.t_thermalization == 5 and p.report_interval == 0
+
+
+def test_params_round_trip():
+    # UnitCell.params / with_params: the coupling vector in the gradient
+    # layout, and a unit cell rebuilt from it equals the original
+    from csmc import workloads as W
+    uc = W.pyrochlore_mixed()
+    uc.field.append((2, np.array([0.1, -0.2, 0.3])))
+    theta = uc.params()
+    assert theta.size == uc.n_params()
+    u2 = uc.with_params(theta)
+    for grp in ("field", "onsite", "bilinear", "cubic", "quartic"):
+        for a, b in zip(getattr(uc, grp), getattr(u2, grp)):
+            for x, y in zip(a, b):
+                assert np.array_equal(np.asarray(x), np.asarray(y)), grp
+    u3 = uc.with_params(theta * 2.0)
+    assert np.array_equal(u3.params(), 2.0 * theta)
+    # cubic tensors travel column-major (the csmc_unit_cell layout)
+    k = 3 * len(uc.field) + 9 * len(uc.onsite) + 9 * len(uc.bilinear)
+    assert np.array_equal(theta[k:k + 27], np.asarray(uc.cubic[0][3]).flatten(order="F"))
