@@ -6,6 +6,7 @@ missing the import raises, and every device entry point fails with the
 library's own error when no HIP device is present.
 """
 import ctypes as C
+import itertools
 import os
 
 import numpy as np
@@ -99,6 +100,7 @@ SIGNATURES = {
     "csmc_sweep_plain": [_P, _I32, _I32, _P, _P, _I32, _U64, _P],
     "csmc_set_counters": [_P, _I32, _P],
     "csmc_get_counters": [_P, _I32, _P],
+    "csmc_batch_storage_mode": [_P, _I32, _P],
     "csmc_anneal": [_P, _I32, _P, _I32, _I64, _I64, _I32, _D, _U64, _P],
     "csmc_pt_init": [_P, _I32, _P, _P, _D, _U64],
     "csmc_pt_advance": [_P, _I32, _I64, _P, _P],
@@ -162,10 +164,15 @@ def device_count():
 class Context:
     """One libcsmc context (HIP stream + table pools) on one device."""
 
+    _serial = itertools.count(1)
+
     def __init__(self, device=0):
         self.h = C.c_void_p()
         check(lib().csmc_ctx_create(device, C.byref(self.h)))
         self.device = device
+        # cache key for per-context device objects: a handle value can be
+        # reused by a later context once this one is destroyed
+        self.uid = next(Context._serial)
 
     def close(self):
         if self.h:

@@ -96,6 +96,14 @@ class SpinBatch:
                       _lib.ptr(acc)))
         return acc
 
+    def storage_mode(self):
+        """storage mode of the persistent sweep kernel for this batch: 3
+        register tables + LDS spins, 2 LDS spins and neighbour tables, 1 LDS
+        spins, 0 global memory"""
+        m = C.c_int32()
+        _lib.check(_lib.lib().csmc_batch_storage_mode(self._h, self.id, C.byref(m)))
+        return m.value
+
     def counters(self):
         c = np.zeros(self.n, dtype=np.uint32)
         _lib.check(_lib.lib().csmc_get_counters(self._h, self.id, _lib.ptr(c)))

@@ -61,8 +61,8 @@ class Lattice:
             self._periodic = 1
         self._info = _lib.LatticeInfo()
         self._tables = None
-        self._dev = {}  # context handle -> lattice id
-        self._eval = {}  # context handle -> FP64 one-system batch id
+        self._dev = {}  # context uid -> (context, lattice id)
+        self._eval = {}  # context uid -> FP64 one-system batch id
         pos = np.zeros((self.size, uc.D))
         self._build(pos=pos)
         self.site_positions = pos.T.copy()  # D x N as in lattice.jl:12
@@ -154,15 +154,15 @@ class Lattice:
     # -- device residency -------------------------------------------------------
     def device_id(self, ctx=None):
         ctx = ctx or _lib.default_context()
-        key = ctx.h.value
-        if key not in self._dev:
+        key = ctx.uid
+        if key not in self._dev or not self._dev[key][0].h:
             s, keep = self.unit_cell._pack()
             shape = np.ascontiguousarray(np.array(self.shape, dtype=np.int32))
             lid = C.c_int32()
             _lib.check(_lib.lib().csmc_lattice_create(ctx.h, C.byref(s), _lib.ptr(shape),
                                                       self._periodic, float(self.S), C.byref(lid)))
-            self._dev[key] = lid.value
-        return self._dev[key]
+            self._dev[key] = (ctx, lid.value)
+        return self._dev[key][1]
 
     def update_couplings(self, uc):
         """replace the couplings of this lattice by those of `uc`, a unit
@@ -170,14 +170,15 @@ class Lattice:
         couplings without rebuilding tables); device copies are updated in
         place, so batches holding this lattice see the new values"""
         s, keep = uc._pack()
-        for key, lid in self._dev.items():
-            _lib.check(_lib.lib().csmc_lattice_update_couplings(C.c_void_p(key), lid, C.byref(s)))
+        for ctx, lid in self._dev.values():
+            if ctx.h:
+                _lib.check(_lib.lib().csmc_lattice_update_couplings(ctx.h, lid, C.byref(s)))
         self.unit_cell = uc
         self._tables = None
 
     def _eval_batch(self, ctx=None):
         ctx = ctx or _lib.default_context()
-        key = ctx.h.value
+        key = ctx.uid
         if key not in self._eval:
             lid = np.array([self.device_id(ctx)], dtype=np.int32)
             bid = C.c_int32()

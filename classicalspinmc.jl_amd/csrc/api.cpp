@@ -68,8 +68,19 @@ struct PTHost {
     DevBuf<double> temps_d, E, sigma, acc_local, acc_exch, bins;
     DevBuf<int32_t> slot_of_sys, sys_of_slot;
     DevBuf<uint32_t> done;
+    // chained launches: double-buffered exchange state, barrier counter and
+    // wait-failure flag
+    DevBuf<double> E2, sigma2;
+    DevBuf<int32_t> slot_of_sys2, sys_of_slot2;
+    DevBuf<uint32_t> bar, err;
+    bool chain_ok = true;  // cleared when a cooperative launch is refused
+    // probe series not yet folded into the binners: probes [0, probe_pos)
+    DevBuf<double> series;
+    int64_t probe_pos = 0;
     uint64_t seed = 0;
 };
+
+constexpr int PROBE_CAP = 4096;  // probes per slot held before a fold
 
 struct Batch {
     int prec = 64;
@@ -77,6 +88,7 @@ struct Batch {
     int maxN = 0;
     int maxK2 = 0;
     int k2c = 0;  // bilinear slot count shared by all systems (0 if mixed)
+    int nslots = 0;  // most register-table slots of any system
     int64_t total = 0;
     bool self = false;
     bool gen = false;  // some lattice has cubic or quartic terms
@@ -111,8 +123,9 @@ struct csmc_ctx {
     std::map<int, std::unique_ptr<Batch>> batches;
     int next_batch = 1;
     int max_lds = 65536;
-    int lds_cap = 2;  // highest workgroup storage mode allowed
+    int lds_cap = 3;  // highest workgroup storage mode allowed
     bool specialise = true;  // use slot-count specialised kernels when they exist
+    bool pt_chain = true;    // chain PT exchange periods in cooperative launches
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     double timed_ms = 0.0;
@@ -170,6 +183,20 @@ std::vector<double> device_coef(const HostLattice &L) {
     return v;
 }
 
+// LAT_* flags of a lattice's couplings: which optional terms are non-zero,
+// and whether every bilinear matrix is diagonal
+int coupling_flags(const HostLattice &L) {
+    int flags = LAT_DIAG;
+    for (long k = 0; k < (long)L.nb * 9; ++k)
+        if (L.coef[L.coef_onsite + k] != 0.0) flags |= LAT_ONSITE;
+    for (long k = 0; k < (long)L.nb * 3; ++k)
+        if (L.coef[L.coef_field + k] != 0.0) flags |= LAT_FIELD;
+    for (long m = 0; m < (long)L.nb * L.K2; ++m)
+        for (int e = 0; e < 9; ++e)
+            if (e % 4 != 0 && L.coef[L.coef_c2 + m * 9 + e] != 0.0) flags &= ~LAT_DIAG;
+    return flags;
+}
+
 // append one lattice's device tables to the host-side pools
 void append_lattice(csmc_ctx *c, const HostLattice &L) {
     LatDev d{};
@@ -184,11 +211,7 @@ void append_lattice(csmc_ctx *c, const HostLattice &L) {
     d.nruns = (int)L.runs.size();
     d.has_self = L.has_self;
     d.S = L.S;
-    d.flags = 0;
-    for (long k = 0; k < (long)L.nb * 9; ++k)
-        if (L.coef[L.coef_onsite + k] != 0.0) d.flags |= LAT_ONSITE;
-    for (long k = 0; k < (long)L.nb * 3; ++k)
-        if (L.coef[L.coef_field + k] != 0.0) d.flags |= LAT_FIELD;
+    d.flags = coupling_flags(L);
     auto &I = c->h_i32;
     auto dev = [&](int ref) { return ref < 0 ? -1 : L.dev_of_ref[ref]; };
     auto devp = [&](int ref) { return ref < 0 ? N : L.dev_of_ref[ref]; };  // absent -> pad site
@@ -212,6 +235,19 @@ void append_lattice(csmc_ctx *c, const HostLattice &L) {
     for (int v : L.colrun) I.push_back(v);
     d.run_off = (int64_t)c->h_runs.size();
     for (auto &r : L.runs) c->h_runs.push_back(make_int4(r[0], r[1], r[2], r[3]));
+    // register-table schedule: each run cut into REG_THREADS-site slots,
+    // colour by colour (runs are already in colour order)
+    d.slot_off = (int64_t)c->h_runs.size();
+    d.colslot_off = (int64_t)I.size();
+    int nslots = 0;
+    for (int col = 0; col < L.ncol; ++col) {
+        I.push_back(nslots);
+        for (int r = L.colrun[col]; r < L.colrun[col + 1]; ++r)
+            for (int first = L.runs[r][0]; first < L.runs[r][1]; first += REG_THREADS, ++nslots)
+                c->h_runs.push_back(make_int4(first, L.runs[r][1], L.runs[r][2], col));
+    }
+    I.push_back(nslots);
+    d.nslots = nslots;
     // incidence lists: participant -> (owner, kind, role, slot)
     std::vector<std::vector<Inc>> inc(N);
     for (int i = 0; i < N; ++i) {
@@ -285,15 +321,34 @@ Batch *get_batch(csmc_ctx *c, int id) {
     Batch *b = get_batch(c, id);                                                \
     if (!b) return fail(CSMC_ESTATE, "unknown batch id " + std::to_string(id))
 
-// storage mode of the workgroup kernel: the most on-chip layout whose LDS
-// footprint fits (spins as three component arrays, then uint16 neighbour
-// tables), capped by CSMC_LDS (0, 1, 2) for measurements of each mode
+// register-table slot budgets compiled (kernels_reg.hip): per thread, SPT
+// sites per sweep with K2C neighbour offsets each held in registers (8 slots
+// only for up to 4 neighbours); 0 if the batch does not fit
+int reg_spt(int nslots, int k2c) {
+    if (nslots <= 4) return 4;
+    if (nslots <= 8 && k2c <= 4) return 8;
+    return 0;
+}
+
+// storage mode of the workgroup kernel, capped by CSMC_LDS (0..3) for
+// measurements of each mode:
+//   3  register tables + float4 spins in LDS (FP32, bilinear-only lattices
+//      with a shared slot count of 3, 4 or 6 and at most 8 slots a thread)
+//   2  spins as three component arrays + uint16 neighbour tables in LDS
+//   1  spins in LDS, tables in global memory
+//   0  everything in global memory
 int choose_lm(csmc_ctx *c, Batch *b, size_t *bytes) {
     auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
     size_t comp = b->prec == 32 ? 4 : 8;
     size_t spins = al(3 * comp * (size_t)(b->maxN + 1));  // + the zero pad site
     size_t idx = al(2 * (size_t)b->maxK2 * b->maxN);
-    size_t avail = (size_t)c->max_lds > 1024 ? (size_t)c->max_lds - 1024 : 0;
+    size_t avail = (size_t)c->max_lds > RED_BYTES ? (size_t)c->max_lds - RED_BYTES : 0;
+    const bool k2ok = b->k2c == 3 || b->k2c == 4 || b->k2c == 6;
+    if (c->lds_cap >= 3 && c->specialise && b->prec == 32 && !b->self && !b->gen && k2ok &&
+        reg_spt(b->nslots, b->k2c) > 0 && 16 * (size_t)(b->maxN + 1) <= avail) {
+        *bytes = 16 * (size_t)(b->maxN + 1);
+        return 3;
+    }
     if (c->lds_cap >= 2 && b->maxN < 65535 && spins + idx <= avail) {
         *bytes = spins + idx;
         return 2;
@@ -343,8 +398,14 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, c->stream));
     }
-    HIPCHK(launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0, b->nsys, lds_bytes,
-                     a, c->stream));
+    if (lm == 3) {
+        bool diag = true;
+        for (int id : b->lat) diag = diag && (c->h_lat[id].flags & LAT_DIAG);
+        HIPCHK(launch_wg_reg(b->k2c, reg_spt(b->nslots, b->k2c), diag, b->nsys, lds_bytes, a, c->stream));
+    } else {
+        HIPCHK(launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0, b->nsys,
+                         lds_bytes, a, c->stream));
+    }
     if (c->timing) {
         HIPCHK(hipEventRecord(e1, c->stream));
         c->events.push_back({e0, e1});
@@ -385,16 +446,18 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
     }
     c->max_lds = max_lds_bytes();
     const char *lc = getenv("CSMC_LDS");
-    if (lc && lc[0] >= '0' && lc[0] <= '2') c->lds_cap = lc[0] - '0';
+    if (lc && lc[0] >= '0' && lc[0] <= '3') c->lds_cap = lc[0] - '0';
     const char *sp = getenv("CSMC_SPECIALISE");
     if (sp && sp[0] == '0') c->specialise = false;
+    const char *ch = getenv("CSMC_PT_CHAIN");
+    if (ch && ch[0] == '0') c->pt_chain = false;
     const char *vb = getenv("CSMC_VERBOSE");
     if (vb && vb[0] == '1') {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess)
             fprintf(stderr, "csmc: device %d %s (%s), %d CUs, max LDS per block %d B%s\n", device,
                     prop.name, prop.gcnArchName, prop.multiProcessorCount, c->max_lds,
-                    c->lds_cap < 2 ? " (LDS modes capped)" : "");
+                    c->lds_cap < 3 ? " (storage modes capped)" : "");
     }
     *out = c;
     return CSMC_OK;
@@ -511,12 +574,7 @@ int32_t csmc_lattice_update_couplings(csmc_ctx *c, int32_t id, const csmc_unit_c
     if (rc) return fail(rc, err);
     std::vector<double> dc = device_coef(L);
     std::copy(dc.begin(), dc.end(), c->h_coef.begin() + c->h_lat[id].coef_off);
-    int flags = 0;
-    for (long k = 0; k < (long)L.nb * 9; ++k)
-        if (L.coef[L.coef_onsite + k] != 0.0) flags |= LAT_ONSITE;
-    for (long k = 0; k < (long)L.nb * 3; ++k)
-        if (L.coef[L.coef_field + k] != 0.0) flags |= LAT_FIELD;
-    c->h_lat[id].flags = flags;
+    c->h_lat[id].flags = coupling_flags(L);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->dirty = true;
     return CSMC_OK;
@@ -548,6 +606,7 @@ int32_t csmc_batch_create(csmc_ctx *c, int32_t n, const int32_t *lat_ids, int32_
         b->maxN = std::max<int>(b->maxN, (int)L.N);
         b->maxK2 = std::max<int>(b->maxK2, L.K2);
         b->k2c = (s == 0 || b->k2c == L.K2) ? L.K2 : 0;
+        b->nslots = std::max(b->nslots, c->h_lat[id].nslots);
         b->self |= (L.has_self != 0);
         b->gen |= (L.K3 > 0 || L.K4 > 0);
     }
@@ -782,6 +841,15 @@ int32_t csmc_get_counters(csmc_ctx *c, int32_t id, uint32_t *counters) {
     return CSMC_OK;
 }
 
+int32_t csmc_batch_storage_mode(csmc_ctx *c, int32_t id, int32_t *mode) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!mode) return fail(CSMC_EINVAL, "null output");
+    size_t bytes = 0;
+    *mode = choose_lm(c, b, &bytes);
+    return CSMC_OK;
+}
+
 int32_t csmc_anneal(csmc_ctx *c, int32_t id, const double *temps, int32_t n_temps,
                     int64_t t_th, int64_t or_rate, int32_t alg, double sigma0, uint64_t seed,
                     double *accepted) {
@@ -849,6 +917,16 @@ int32_t csmc_pt_init(csmc_ctx *c, int32_t id, const double *temps, const csmc_pt
     HIPCHK(pt.slot_of_sys.alloc(R));
     HIPCHK(pt.sys_of_slot.alloc(R));
     HIPCHK(pt.done.alloc(1));
+    HIPCHK(pt.E2.alloc(2 * (size_t)R));
+    HIPCHK(pt.sigma2.alloc(2 * (size_t)R));
+    HIPCHK(pt.slot_of_sys2.alloc(2 * (size_t)R));
+    HIPCHK(pt.sys_of_slot2.alloc(2 * (size_t)R));
+    HIPCHK(pt.bar.alloc(1));
+    HIPCHK(pt.err.alloc(1));
+    HIPCHK(hipMemsetAsync(pt.err.p, 0, 4, c->stream));
+    HIPCHK(pt.series.alloc((size_t)R * PROBE_CAP * 2));
+    pt.probe_pos = 0;
+    pt.chain_ok = true;
     std::vector<int32_t> ident(R);
     for (int r = 0; r < R; ++r) ident[r] = r;
     std::vector<double> sg(R, sigma0);
@@ -904,6 +982,92 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
     a.pt.acc_exch = pt.acc_exch.p;
     a.pt.binners = pt.bins.p;
     a.pt.done = pt.done.p;
+    a.pt.E2 = pt.E2.p;
+    a.pt.sigma2 = pt.sigma2.p;
+    a.pt.slot_of_sys2 = pt.slot_of_sys2.p;
+    a.pt.sys_of_slot2 = pt.sys_of_slot2.p;
+    a.pt.bar = pt.bar.p;
+    a.pt.err = pt.err.p;
+    a.spin_limit = 1u << 26;
+    a.pt.series = pt.series.p;
+    a.pt.series_cap = PROBE_CAP;
+    // probe sweeps of a segment (the tail of sweep se is not run when emid)
+    auto probes_in = [&](int64_t sb, int64_t se, bool emid) -> int64_t {
+        if (!a.pt.measure) return 0;
+        const int64_t lo = std::max<int64_t>(sb, p.t_thermalization), hi = emid ? se - 1 : se;
+        if (hi < lo) return 0;
+        const int64_t q = a.pt.probe;
+        auto fdiv = [q](int64_t x) { return x >= 0 ? x / q : -((-x + q - 1) / q); };
+        return fdiv(hi) - fdiv(lo - 1);  // multiples of the probe rate in [lo, hi]
+    };
+    // room for n more probes in the series, folding it first if needed
+    auto reserve = [&](int64_t n) -> int {
+        if (pt.probe_pos + n > PROBE_CAP) {
+            HIPCHK(launch_bin_fold(R, pt.series.p, PROBE_CAP, (int)pt.probe_pos, pt.bins.p, c->stream));
+            pt.probe_pos = 0;
+        }
+        return CSMC_OK;
+    };
+    // Consecutive segments that end with an exchange (and no host event) and
+    // are evenly spaced run as one chain: one cooperative launch, a grid
+    // barrier per exchange, the spins resident on chip throughout.
+    const int chain_max = 1024;
+    int nchain = 0;
+    int64_t chain_begin = 0, chain_e0 = 0, chain_P = 0, chain_probes = 0;
+    bool chain_bmid = false, chained = false;
+    auto launch_one = [&](int64_t sb, int64_t se, bool bmid, bool emid, bool exch) -> int {
+        const int64_t n = probes_in(sb, se, emid);
+        int rc1 = reserve(n);
+        if (rc1) return rc1;
+        WGArgs s1 = a;
+        s1.seg_begin = sb;
+        s1.seg_end = se;
+        s1.begin_mid = bmid;
+        s1.end_mid = emid;
+        s1.exchange = exch;
+        s1.probe_base = pt.probe_pos;
+        rc1 = run_wg(c, b, s1);
+        if (rc1) return rc1;
+        pt.probe_pos += n;
+        return CSMC_OK;
+    };
+    auto flush = [&]() -> int {
+        if (nchain == 0) return CSMC_OK;
+        int rc2 = CSMC_OK;
+        bool done_chain = false;
+        if (nchain > 1 && pt.chain_ok) {
+            WGArgs ca = a;
+            ca.mode = WG_PT_CHAIN;
+            ca.seg_begin = chain_begin;
+            ca.begin_mid = chain_bmid;
+            ca.chain_e0 = chain_e0;
+            ca.chain_P = chain_P;
+            ca.nseg = nchain;
+            rc2 = reserve(chain_probes);
+            if (rc2) return rc2;
+            ca.probe_base = pt.probe_pos;
+            HIPCHK(hipMemsetAsync(pt.bar.p, 0, 4, c->stream));
+            rc2 = run_wg(c, b, ca);
+            if (rc2 == CSMC_OK) {
+                done_chain = chained = true;
+                pt.probe_pos += chain_probes;
+            } else {
+                (void)hipGetLastError();  // the grid cannot be co-resident: fall back
+                pt.chain_ok = false;
+            }
+        }
+        if (!done_chain) {
+            for (int g = 0; g < nchain; ++g) {
+                const int64_t se = chain_e0 + g * chain_P;
+                const int64_t sb = g == 0 ? chain_begin : se - chain_P;
+                rc2 = launch_one(sb, se, g == 0 ? chain_bmid : true, true, true);
+                if (rc2) return rc2;
+            }
+        }
+        nchain = 0;
+        chain_probes = 0;
+        return CSMC_OK;
+    };
     int32_t ev = 0;
     while (true) {
         if (pt.sweep >= pt.total) {
@@ -927,12 +1091,34 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
             if (report_at(s + 1) || s + 1 >= pt.total || s + 1 >= until || s - pt.sweep + 1 >= max_seg) break;
             ++s;
         }
-        a.seg_begin = pt.sweep;
-        a.seg_end = s;
-        a.begin_mid = begin_mid;
-        a.end_mid = end_mid;
-        a.exchange = exch;
-        rc = run_wg(c, b, a);
+        if (c->pt_chain && R > 1 && exch && !checkpoint_at(s)) {
+            const int64_t np = probes_in(pt.sweep, s, true);
+            const bool extends = nchain > 0 && chain_probes + np <= PROBE_CAP &&
+                                 (nchain == 1 ? s > chain_e0 : s == chain_e0 + nchain * chain_P);
+            if (nchain > 0 && !extends) {
+                rc = flush();
+                if (rc) return rc;
+            }
+            if (nchain == 0) {
+                chain_begin = pt.sweep;
+                chain_bmid = begin_mid;
+                chain_e0 = s;
+            } else if (nchain == 1) {
+                chain_P = s - chain_e0;
+            }
+            ++nchain;
+            chain_probes += np;
+            pt.sweep = s;
+            pt.mid = true;
+            if (nchain >= chain_max) {
+                rc = flush();
+                if (rc) return rc;
+            }
+            continue;
+        }
+        rc = flush();
+        if (rc) return rc;
+        rc = launch_one(pt.sweep, s, begin_mid, end_mid, exch);
         if (rc) return rc;
         if (end_mid) {
             pt.sweep = s;
@@ -950,7 +1136,14 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
             }
         }
     }
+    rc = flush();
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (chained) {
+        uint32_t err = 0;
+        HIPCHK(hipMemcpy(&err, pt.err.p, 4, hipMemcpyDeviceToHost));
+        if (err) return fail(CSMC_EHIP, "parallel tempering: a grid barrier wait timed out");
+    }
     if (stopped) *stopped = pt.sweep;
     if (event) *event = (pt.sweep >= pt.total && !pt.mid) ? 3 : ev;
     return CSMC_OK;
@@ -1012,6 +1205,8 @@ int32_t csmc_pt_observables(csmc_ctx *c, int32_t id, double *out) {
     PTHost &pt = b->pt;
     if (!pt.init) return fail(CSMC_ESTATE, "csmc_pt_init was not called for this batch");
     int R = b->nsys;
+    HIPCHK(launch_bin_fold(R, pt.series.p, PROBE_CAP, (int)pt.probe_pos, pt.bins.p, c->stream));
+    pt.probe_pos = 0;
     std::vector<double> bins((size_t)R * 2 * BIN_DOUBLES);
     HIPCHK(hipMemcpyAsync(bins.data(), pt.bins.p, bins.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

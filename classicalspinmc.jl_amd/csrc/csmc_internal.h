@@ -14,6 +14,9 @@
 //              that share colour and basis.  Device sites are ordered by
 //              (colour, basis, reference index), so every colour class is one
 //              contiguous range and each run has one coefficient set.
+//              After the runs: the register-table slots, int4 {first site,
+//              run end, basis, colour}, each run cut into REG_THREADS-site
+//              pieces (thread t of a slot owns site first + t if < end).
 //   coef pool  per lattice (float and double copies at the same offset):
 //              onsite[nb][9], field[nb][3], c2[nb][K2P][9], c3[nb][K3][27],
 //              c4[nb][K4][81]: the couplings of each (basis, slot) with the
@@ -35,16 +38,28 @@ struct LatDev {
     int64_t nbr_off;     // i32 pool
     int64_t ref_off;     // i32 pool
     int64_t sub_off;     // i32 pool
-    int64_t colrun_off;  // i32 pool, run indices are absolute (into run pool)
+    int64_t colrun_off;  // i32 pool, run indices relative to run_off
     int64_t run_off;     // run pool (absolute index of first run)
     int64_t coef_off;    // coef pools
     int64_t inc_off;     // i32 pool: incidence offsets [N+1] then entries
     int64_t inc_n;       // number of incidence entries
     int64_t grad_off;    // i32 pool: slot->gradient index tables
     int32_t grad_len;    // coupling-gradient length per system
-    int32_t pad_;
+    int32_t nslots;      // register-table schedule: thread slots per sweep
+    int64_t slot_off;    // run pool: int4 {first site, run end, basis, colour}
+    int64_t colslot_off; // i32 pool: first slot of each colour [ncol + 1]
     double S;
 };
+
+// LDS scratch at the start of every persistent sweep kernel's dynamic LDS:
+// the workgroup reductions (16 waves x 8 doubles) then a few flags.
+constexpr int RED_BYTES = 2048;
+constexpr int RED_FLAG = 250;   // double index of the exchange flag
+constexpr int RED_XCHG = 248;   // double index of the chain-exchange scratch (2)
+
+// Workgroup size of the register-table sweep kernel; the slot schedule
+// (one slot = REG_THREADS consecutive device sites of one run) is built for it.
+constexpr int REG_THREADS = 1024;
 
 struct SysDev {
     int32_t lat;      // lattice id
@@ -55,6 +70,7 @@ struct SysDev {
 
 constexpr int32_t LAT_ONSITE = 1;
 constexpr int32_t LAT_FIELD = 2;
+constexpr int32_t LAT_DIAG = 4;  // every bilinear coupling matrix is diagonal
 
 // incidence entry: owner device site and a packed (kind, slot, role)
 //   code = kind(2 bits: 0 bil, 1 cub, 2 qua) | role << 2 (2 bits) | slot << 4
@@ -86,7 +102,21 @@ struct PTDev {
     double *acc_exch;      // [R] per slot
     double *binners;       // [R][2][BIN_DOUBLES] per slot: E then |M|
     uint32_t *done;        // [1] WG completion counter for the exchange
+    // chained segments (one cooperative launch, a grid barrier per exchange):
+    // state published at an exchange, double-buffered by exchange parity
+    double *E2;            // [2][R] per system
+    double *sigma2;        // [2][R] per slot
+    int32_t *slot_of_sys2; // [2][R]
+    int32_t *sys_of_slot2; // [2][R]
+    uint32_t *bar;         // [1] grid barrier arrivals (zeroed before a launch)
+    uint32_t *err;         // [1] non-zero: a barrier wait ran out (no result)
+    // probe series: (E, |M|) of each slot per probe sweep, folded into the
+    // binners off the sweep path; [R][series_cap][2]
+    double *series;
+    int32_t series_cap;
+    int32_t pad2_;
 };
+
 
 constexpr int BIN_LVLS = 32;
 // per level: count, sum[2], sum2[4], pend[2], has  -> 10 doubles

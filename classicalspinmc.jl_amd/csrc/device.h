@@ -122,9 +122,11 @@ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1
             k0 += 0x9E3779B9u;
             k1 += 0xBB67AE85u;
         }
-        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
-        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        // one 32x32->64 multiply per product (v_mad_u64_u32) gives both halves
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        c = make_uint4((uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1,
+                       (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0);
     }
     return c;
 }
@@ -166,10 +168,11 @@ template <>
 struct Math<float> {
     static __device__ __forceinline__ float u01(uint32_t x) { return u01f(x); }
     // same formula; v_sin/v_cos take the angle in turns, so phi = 2 pi u1
-    // needs no multiply and no range reduction
+    // needs no multiply and no range reduction; v_sqrt_f32 (1 ulp) serves
+    // the FP32 state
     static __device__ __forceinline__ void rand_spin(float S, float u1, float u2, float *o) {
         float z = 2.0f * u2 - 1.0f;
-        float r = __builtin_sqrtf(fmaxf(1.0f - z * z, 0.0f));
+        float r = __builtin_amdgcn_sqrtf(fmaxf(1.0f - z * z, 0.0f));
         o[0] = S * (r * __builtin_amdgcn_cosf(u1));
         o[1] = S * (r * __builtin_amdgcn_sinf(u1));
         o[2] = S * z;
@@ -195,6 +198,45 @@ struct Math<float> {
 // padding slots hit the zero pad site, so there is no branch.  Couplings are
 // read through the constant address space (scalar loads when b is uniform).
 // ---------------------------------------------------------------------------
+// Bilinear part of the field from the K2C neighbour spins of a site of basis
+// b, as one multiply-add chain per component.  DIAG: every coupling matrix of
+// the lattice is diagonal, so only m11, m22, m33 take part.
+template <class R, class A, int K2C, bool DIAG, class S>
+__device__ __forceinline__ void bilinear_field(const LatView<R> &L, int b, const S *sj, A F[3]) {
+#pragma unroll
+    for (int k = 0; k < K2C; ++k) {
+        const R *M = L.c2 + (b * L.K2P + k) * 9;
+        const A x = sj[k].x, y = sj[k].y, z = sj[k].z;
+        if constexpr (DIAG) {
+            if (k == 0) {
+                F[0] = (A)ldc(M + 0) * x;
+                F[1] = (A)ldc(M + 4) * y;
+                F[2] = (A)ldc(M + 8) * z;
+            } else {
+                F[0] += (A)ldc(M + 0) * x;
+                F[1] += (A)ldc(M + 4) * y;
+                F[2] += (A)ldc(M + 8) * z;
+            }
+        } else {
+            if (k == 0) {
+                F[0] = (A)ldc(M + 0) * x;
+                F[1] = (A)ldc(M + 3) * x;
+                F[2] = (A)ldc(M + 6) * x;
+            } else {
+                F[0] += (A)ldc(M + 0) * x;
+                F[1] += (A)ldc(M + 3) * x;
+                F[2] += (A)ldc(M + 6) * x;
+            }
+            F[0] += (A)ldc(M + 1) * y;
+            F[1] += (A)ldc(M + 4) * y;
+            F[2] += (A)ldc(M + 7) * y;
+            F[0] += (A)ldc(M + 2) * z;
+            F[1] += (A)ldc(M + 5) * z;
+            F[2] += (A)ldc(M + 8) * z;
+        }
+    }
+}
+
 // K2C > 0: the lattice has exactly K2C bilinear slots (compile-time), so
 // the slot loop is fully unrolled with no padding work.
 template <class R, class A, bool GEN = true, int K2C = 0, class NB, class LD>
@@ -208,22 +250,7 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
         spin4<R> sj[K2C];
 #pragma unroll
         for (int k = 0; k < K2C; ++k) sj[k] = ld(j[k]);
-#pragma unroll
-        for (int k = 0; k < K2C; ++k) {
-            const R *M = L.c2 + (b * L.K2P + k) * 9;
-            A f0 = (A)ldc(M + 0) * sj[k].x + (A)ldc(M + 1) * sj[k].y + (A)ldc(M + 2) * sj[k].z;
-            A f1 = (A)ldc(M + 3) * sj[k].x + (A)ldc(M + 4) * sj[k].y + (A)ldc(M + 5) * sj[k].z;
-            A f2 = (A)ldc(M + 6) * sj[k].x + (A)ldc(M + 7) * sj[k].y + (A)ldc(M + 8) * sj[k].z;
-            if (k == 0) {
-                F[0] = f0;
-                F[1] = f1;
-                F[2] = f2;
-            } else {
-                F[0] += f0;
-                F[1] += f1;
-                F[2] += f2;
-            }
-        }
+        bilinear_field<R, A, K2C, false>(L, b, sj, F);
     }
     for (int k0 = 0; K2C == 0 && k0 < L.K2P; k0 += 4) {
         int j[4];
@@ -325,41 +352,50 @@ __device__ __forceinline__ A onsite_energy(const LatView<R> &L, int b, A sx, A s
 // single-site updates; X is a spin view, nb the bilinear neighbour view
 // ---------------------------------------------------------------------------
 
-// overrelaxation! body (monte_carlo.jl:128-137)
-template <class R, bool GEN, int K2C, class NB, class SV>
-__device__ __forceinline__ void or_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
-                                          int i) {
-    spin4<R> s = X.ld(i);
-    R F[3], H[3];
-    interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
-    local_field<R, R>(L, b, s, F, H);
-    if (H[0] == R(0) && H[1] == R(0) && H[2] == R(0)) return;
+// overrelaxation! body (monte_carlo.jl:128-137) for a site with spin s and
+// local field H: false (no write) when H == 0
+template <class R>
+__device__ __forceinline__ bool or_reflect(const spin4<R> &s, const R H[3], spin4<R> &n) {
+    if (H[0] == R(0) && H[1] == R(0) && H[2] == R(0)) return false;
     R hh = H[0] * H[0] + H[1] * H[1] + H[2] * H[2];
     R proj = Math<R>::div(R(2) * (s.x * H[0] + s.y * H[1] + s.z * H[2]), hh);
-    spin4<R> n;
     n.x = -s.x + proj * H[0];
     n.y = -s.y + proj * H[1];
     n.z = -s.z + proj * H[2];
     n.w = R(0);
-    X.st(i, n);
+    return true;
 }
 
-// deterministic_updates! body (monte_carlo.jl:203-209)
-template <class R, bool GEN, int K2C, class NB, class SV>
-__device__ __forceinline__ void det_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
-                                           int i) {
-    spin4<R> s = X.ld(i);
-    R F[3], H[3];
-    interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
-    local_field<R, R>(L, b, s, F, H);
-    if (H[0] == R(0) && H[1] == R(0) && H[2] == R(0)) return;
+// deterministic_updates! body (monte_carlo.jl:203-209): s = -S H / |H|
+template <class R>
+__device__ __forceinline__ bool det_align(const LatView<R> &L, const R H[3], spin4<R> &n) {
+    if (H[0] == R(0) && H[1] == R(0) && H[2] == R(0)) return false;
     R nrm = sqrt(H[0] * H[0] + H[1] * H[1] + H[2] * H[2]);
-    spin4<R> n;
     n.x = -H[0] / nrm * L.S;
     n.y = -H[1] / nrm * L.S;
     n.z = -H[2] / nrm * L.S;
     n.w = R(0);
-    X.st(i, n);
+    return true;
+}
+
+template <class R, bool GEN, int K2C, class NB, class SV>
+__device__ __forceinline__ void or_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
+                                          int i) {
+    spin4<R> s = X.ld(i), n;
+    R F[3], H[3];
+    interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
+    local_field<R, R>(L, b, s, F, H);
+    if (or_reflect<R>(s, H, n)) X.st(i, n);
+}
+
+template <class R, bool GEN, int K2C, class NB, class SV>
+__device__ __forceinline__ void det_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
+                                           int i) {
+    spin4<R> s = X.ld(i), n;
+    R F[3], H[3];
+    interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
+    local_field<R, R>(L, b, s, F, H);
+    if (det_align<R>(L, H, n)) X.st(i, n);
 }
 
 // One Metropolis proposal (metropolis.jl:70-78 with calculate_energy_diff!,
@@ -369,15 +405,15 @@ __device__ __forceinline__ void det_update(const LatView<R> &L, const NB &nb, co
 // SELF: some term of this lattice names the site itself, so the new energy
 // is evaluated with the candidate substituted in those slots as the
 // reference does (it writes the spin before calling energy()).
-template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
-__device__ __forceinline__ int metro_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
-                                            int i, R T, R invT, int kind, R sigma, uint32_t k0,
-                                            uint32_t k1, uint32_t stream, uint32_t ctr) {
-    spin4<R> s = X.ld(i);
-    uint4 r = philox4x32_10(make_uint4((uint32_t)L.ref[i], stream, ctr, TAG_METRO), k0, k1);
-    R u1 = Math<R>::u01(r.x), u2 = Math<R>::u01(r.y), u3 = Math<R>::u01(r.z);
+// The pieces of one Metropolis step, shared by every sweep form.
+// Proposal from the draws r.x, r.y: a fresh direction (CSMC_METROPOLIS) or
+// s + sigma * (uniform direction) renormalised (gaussian_move,
+// metropolis.jl:84-87).
+template <class R>
+__device__ __forceinline__ spin4<R> metro_propose(const LatView<R> &L, const spin4<R> &s, uint4 r,
+                                                  int kind, R sigma) {
     R rs[3];
-    Math<R>::rand_spin(L.S, u1, u2, rs);
+    Math<R>::rand_spin(L.S, Math<R>::u01(r.x), Math<R>::u01(r.y), rs);
     spin4<R> n;
     if (kind == 1) {
         n.x = rs[0];
@@ -391,17 +427,59 @@ __device__ __forceinline__ int metro_update(const LatView<R> &L, const NB &nb, c
         n.z = d * inv;
     }
     n.w = R(0);
+    return n;
+}
+
+// energy(new) - energy(old) at a site whose terms do not name the site
+// itself: onsite difference plus (n - s).(F - h)
+template <class R>
+__device__ __forceinline__ R metro_dE(const LatView<R> &L, int b, const spin4<R> &s,
+                                      const spin4<R> &n, const R F[3]) {
     const R *hp = L.field + b * 3;
-    const R h[3] = {ldc(hp), ldc(hp + 1), ldc(hp + 2)};
+    const R h0 = ldc(hp), h1 = ldc(hp + 1), h2 = ldc(hp + 2);
+    R eo_new = onsite_energy<R, R>(L, b, n.x, n.y, n.z);
+    R eo_old = onsite_energy<R, R>(L, b, s.x, s.y, s.z);
+    return (eo_new - eo_old) + (n.x - s.x) * (F[0] - h0) + (n.y - s.y) * (F[1] - h1) +
+           (n.z - s.z) * (F[2] - h2);
+}
+
+// accept if dE < 0, else if u < exp(-dE/T) (metropolis.jl:75-78), u = r.z
+template <class R>
+__device__ __forceinline__ int metro_accept(R dE, uint4 r, R T, R invT) {
+    return (dE < R(0)) ? 1 : (Math<R>::u01(r.z) < Math<R>::boltz(dE, T, invT));
+}
+
+// store the accepted proposal; an FP32 state also rewrites a rejected spin
+// pulled back to |s| = S, so reflections cannot accumulate rounding drift
+template <class R, class SV, class IX>
+__device__ __forceinline__ void metro_commit(const LatView<R> &L, const SV &X, IX i, int acc,
+                                             spin4<R> s, const spin4<R> &n) {
+    if (acc) {
+        X.st(i, n);
+    } else if (sizeof(R) == 4) {
+        R inv = Math<R>::rsqrt_(s.x * s.x + s.y * s.y + s.z * s.z) * L.S;
+        s.x *= inv;
+        s.y *= inv;
+        s.z *= inv;
+        X.st(i, s);
+    }
+}
+
+template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
+__device__ __forceinline__ int metro_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
+                                            int i, R T, R invT, int kind, R sigma, uint32_t k0,
+                                            uint32_t k1, uint32_t stream, uint32_t ctr) {
+    spin4<R> s = X.ld(i);
+    uint4 r = philox4x32_10(make_uint4((uint32_t)L.ref[i], stream, ctr, TAG_METRO), k0, k1);
+    spin4<R> n = metro_propose<R>(L, s, r, kind, sigma);
     R dE;
     if (!SELF) {
         R F[3];
         interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
-        R eo_new = onsite_energy<R, R>(L, b, n.x, n.y, n.z);
-        R eo_old = onsite_energy<R, R>(L, b, s.x, s.y, s.z);
-        dE = (eo_new - eo_old) + (n.x - s.x) * (F[0] - (R)h[0]) + (n.y - s.y) * (F[1] - (R)h[1]) +
-             (n.z - s.z) * (F[2] - (R)h[2]);
+        dE = metro_dE<R>(L, b, s, n, F);
     } else {
+        const R *hp = L.field + b * 3;
+        const R h[3] = {ldc(hp), ldc(hp + 1), ldc(hp + 2)};
         R Fo[3], Fn[3];
         interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, Fo);
         interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return j == i ? n : X.ld(j); }, Fn);
@@ -411,18 +489,8 @@ __device__ __forceinline__ int metro_update(const LatView<R> &L, const NB &nb, c
                (n.x * h[0] + n.y * h[1] + n.z * h[2]);
         dE = en - eo;
     }
-    int acc = (dE < R(0)) ? 1 : (u3 < Math<R>::boltz(dE, T, invT));
-    if (acc) {
-        X.st(i, n);
-    } else if (sizeof(R) == 4) {
-        // FP32 state: pull |s| back to S on every visit so reflections
-        // cannot accumulate rounding drift
-        R inv = Math<R>::rsqrt_(s.x * s.x + s.y * s.y + s.z * s.z) * L.S;
-        s.x *= inv;
-        s.y *= inv;
-        s.z *= inv;
-        X.st(i, s);
-    }
+    int acc = metro_accept<R>(dE, r, T, invT);
+    metro_commit<R>(L, X, i, acc, s, n);
     return acc;
 }
 

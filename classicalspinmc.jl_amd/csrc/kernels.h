@@ -17,7 +17,10 @@ struct DevPools {
     const SysDev *sys;
 };
 
-enum WGMode { WG_SWEEPS = 0, WG_ANNEAL = 1, WG_PT = 2 };
+// WG_PT runs one segment per launch (the last workgroup does the exchange);
+// WG_PT_CHAIN runs a list of segments in one cooperative launch with a grid
+// barrier at each exchange.
+enum WGMode { WG_SWEEPS = 0, WG_ANNEAL = 1, WG_PT = 2, WG_PT_CHAIN = 3 };
 
 // Arguments of the persistent workgroup-per-system kernel.
 struct WGArgs {
@@ -42,6 +45,12 @@ struct WGArgs {
     PTDev pt;
     int64_t seg_begin, seg_end;
     int begin_mid, end_mid, exchange;
+    // WG_PT_CHAIN: nseg segments, the first from seg_begin (begin_mid) to the
+    // exchange at chain_e0, segment g to the exchange at chain_e0 + g * chain_P
+    int64_t chain_e0, chain_P;
+    int nseg;
+    uint32_t spin_limit;  // polls of a barrier wait before it gives up
+    int64_t probe_base;   // series index of this launch's first probe
 };
 
 // Arguments of the one-colour-pass kernel (plain / stream form).
@@ -60,12 +69,24 @@ struct PassArgs {
 };
 
 int max_lds_bytes();
+// raise a kernel's dynamic LDS limit to the device maximum (once per kernel
+// and device; later calls return at once)
+hipError_t allow_large_lds(const void *fn);
+// hipLaunchCooperativeKernel of a persistent kernel taking WGArgs (fails when
+// the grid cannot be resident all at once)
+hipError_t launch_cooperative(const void *fn, int nblocks, int threads, size_t lds_bytes,
+                              const WGArgs &a, hipStream_t st);
 // lm: storage mode (0 global, 1 LDS spins, 2 LDS spins + uint16 neighbour
 // tables); self / gen: some lattice of the batch has a self-referencing
 // term / cubic or quartic terms; k2c: the bilinear slot count shared by
 // every lattice of the batch (0 if they differ)
 hipError_t launch_wg(int prec, int lm, bool self, bool gen, int k2c, int nsys, size_t lds_bytes,
                      const WGArgs &a, hipStream_t st);
+// register-table mode (FP32, bilinear-only): k2c in {3, 4, 6}, spt in {4, 8}
+// register slots per thread (8 only with k2c <= 4), diag: every coupling
+// matrix of the batch is diagonal; lds_bytes = 16 * (max sites + 1)
+hipError_t launch_wg_reg(int k2c, int spt, bool diag, int nsys, size_t lds_bytes, const WGArgs &a,
+                         hipStream_t st);
 hipError_t launch_pass(int prec, bool self, bool gen, int nsys, int max_colour_sites,
                        const PassArgs &a, hipStream_t st);
 hipError_t launch_site_eval(int prec, const DevPools &P, const void *spins, int sys, int N,
@@ -86,6 +107,9 @@ hipError_t launch_grad_spins(int prec, const DevPools &P, const void *spins, int
                              double *dEdS, hipStream_t st);
 hipError_t launch_grad_couplings(int prec, const DevPools &P, const void *spins, int nsys,
                                  int grad_len, double *dEdJ, hipStream_t st);
+// push probes [0, n) of every slot's series into its binners, in order
+hipError_t launch_bin_fold(int R, const double *series, int cap, int n, double *binners,
+                           hipStream_t st);
 hipError_t launch_gd(int prec, const DevPools &P, void *spins, int nsys, int maxN,
                      const double *dEdS, double eta, hipStream_t st);
 

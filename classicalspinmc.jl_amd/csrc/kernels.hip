@@ -11,32 +11,17 @@
 // Parallel-tempering segments end with the replica exchange, done by the last
 // workgroup to finish (a completion counter, no waiting).
 //
+// The register-table mode (neighbour offsets held in registers, spins as
+// float4 in LDS) lives in kernels_reg.hip.
+//
 // k_pass is the plain form: one launch per colour pass, one thread per site.
-#include "device.h"
-#include "kernels.h"
+#include <mutex>
+#include <set>
+#include <utility>
+
+#include "sweep.h"
 
 namespace csmc {
-
-template <class R>
-__device__ __forceinline__ const R *coef_of(const DevPools &P);
-template <>
-__device__ __forceinline__ const float *coef_of<float>(const DevPools &P) {
-    return P.coefF;
-}
-template <>
-__device__ __forceinline__ const double *coef_of<double>(const DevPools &P) {
-    return P.coefD;
-}
-
-template <class R>
-__device__ __forceinline__ spin4<double> to_d(const spin4<R> &s) {
-    spin4<double> d;
-    d.x = s.x;
-    d.y = s.y;
-    d.z = s.z;
-    d.w = 0.0;
-    return d;
-}
 
 // interaction field split by term kind (bilinear, cubic, quartic), FP64,
 // for energies and gradients
@@ -71,12 +56,13 @@ __device__ __forceinline__ void wg_colour_sweep(const LatView<R> &L, OP &&op) {
     }
 }
 
-// FP64 energy parts (bilinear, cubic, quartic, zeeman, onsite) of a system,
-// summed over the workgroup; valid in every thread on return
-template <class R, bool GEN, class NB, class SV>
+// FP64 energy parts (bilinear, cubic, quartic, zeeman, onsite) of a system
+// and its magnetisation, summed over the workgroup in one pass: out[0..4]
+// the parts, out[5..7] the summed spin; valid in every thread on return
+template <class R, bool GEN, bool WITH_M, class NB, class SV>
 __device__ void wg_energy(const LatView<double> &Ld, const NB &nb, const SV &X, double *red,
-                          double out[5]) {
-    double p[5] = {0, 0, 0, 0, 0};
+                          double out[8]) {
+    double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int r = Ld.colrun[0]; r < Ld.colrun[Ld.ncol]; ++r) {
         int4 run = Ld.runs[r];
         int b = __builtin_amdgcn_readfirstlane(run.z);
@@ -90,19 +76,20 @@ __device__ void wg_energy(const LatView<double> &Ld, const NB &nb, const SV &X, 
             p[2] += s.x * F4[0] + s.y * F4[1] + s.z * F4[2];
             p[3] -= s.x * h[0] + s.y * h[1] + s.z * h[2];
             p[4] += onsite_energy<double, double>(Ld, b, s.x, s.y, s.z);
+            if (WITH_M) {
+                p[5] += s.x;
+                p[6] += s.y;
+                p[7] += s.z;
+            }
         }
     }
-    block_sum<5>(p, red);
+    constexpr int NV = WITH_M ? 8 : 5;
+    block_sum<NV>(p, red);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 5; ++k) red[k] = p[k];
+        for (int k = 0; k < NV; ++k) red[k] = p[k];
     __syncthreads();
-    for (int k = 0; k < 5; ++k) out[k] = red[k];
+    for (int k = 0; k < NV; ++k) out[k] = red[k];
     __syncthreads();
-}
-
-__device__ __forceinline__ double combine_energy(const double p[5]) {
-    // hamiltonian.jl:131
-    return p[0] / 2 + p[1] / 3 + p[2] / 4 + p[3] + p[4];
 }
 
 template <class R, class SV>
@@ -140,141 +127,56 @@ __device__ double wg_metro(const LatView<R> &L, const NB &nb, const SV &X, R T, 
     return tot;
 }
 
-// metropolis_adaptive! width update (metropolis.jl:129-131)
-__device__ __forceinline__ double adapt_sigma(double sigma, double acc, double n) {
-    double a = acc / n;
-    double f = 0.5 / fmax(1.0 - a, 0.05);
-    return fmin(fmax(sigma * f, 0.0), 100.0);
-}
+// The sweeper of the global / LDS-array storage modes: sweeps walk the runs
+// of each colour class with one thread per site (wg_colour_sweep).
+template <class R, bool SELF, bool GEN, int K2C, class SV, class NB>
+struct SweeperG {
+    const LatView<R> &L;
+    const LatView<double> &Ld;
+    const SV &X;
+    const NB &nb;
+    double *red;
+    uint32_t stream, k0, k1;
 
-// one log-binning push (E or |M| with its square) into a slot's binner
-__device__ void bin_push(double *st, double v) {
-    double c0 = v, c1 = v * v;
-    for (int l = 0; l < BIN_LVLS; ++l) {
-        double *q = st + l * 10;
-        q[0] += 1.0;
-        q[1] += c0;
-        q[2] += c1;
-        q[3] += c0 * c0;
-        q[4] += c0 * c1;
-        q[5] += c1 * c0;
-        q[6] += c1 * c1;
-        if (q[9] == 0.0) {
-            q[7] = c0;
-            q[8] = c1;
-            q[9] = 1.0;
-            return;
-        }
-        q[9] = 0.0;
-        c0 = 0.5 * (q[7] + c0);
-        c1 = 0.5 * (q[8] + c1);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// k_wg: persistent workgroup-per-system sweeps
-// ---------------------------------------------------------------------------
-template <class R, bool SELF, bool GEN, int MODE, int K2C, class SV, class NB>
-__device__ void wg_run(const WGArgs &a, int sys, const LatView<R> &L, const LatView<double> &Ld,
-                       const SV &X, const NB &nb, double *red) {
-    const uint32_t stream = (uint32_t)a.P.sys[sys].stream;
-    uint32_t mctr = a.mcount[sys];
-
-    auto or_sweep = [&]() {
+    __device__ int n_sites() const { return L.N; }
+    __device__ void or_sweep() {
         wg_colour_sweep<R>(L, [&](int b, int i) { or_update<R, GEN, K2C>(L, nb, X, b, i); });
-    };
-    auto det_sweep = [&]() {
-        wg_colour_sweep<R>(L, [&](int b, int i) { det_update<R, GEN, K2C>(L, nb, X, b, i); });
-    };
-    auto metro_sweep = [&](R T, int kind, double &sigma) {
-        double acc =
-            wg_metro<R, SELF, GEN, K2C>(L, nb, X, T, kind, (R)sigma, a.k0, a.k1, stream, mctr, red);
-        ++mctr;
-        if (kind == CSMC_CONE_ADAPTIVE) sigma = adapt_sigma(sigma, acc, (double)L.N);
-        return acc;
-    };
-
-    if constexpr (MODE == WG_SWEEPS) {
-        const R T = (R)a.T_sys[sys];
-        double sigma = a.sigma_sys ? a.sigma_sys[sys] : 60.0;
-        double acc = 0.0;
-        for (int s = 0; s < a.n_sweeps; ++s) {
-            if (a.kind == CSMC_OVERRELAX) or_sweep();
-            else if (a.kind == CSMC_DETERMINISTIC) det_sweep();
-            else acc += metro_sweep(T, a.kind, sigma);
-        }
-        if (threadIdx.x == 0) {
-            if (a.acc_sys) a.acc_sys[sys] = acc;
-            if (a.sigma_sys) a.sigma_sys[sys] = sigma;
-        }
-    } else if constexpr (MODE == WG_ANNEAL) {
-        // simulated_annealing! (monte_carlo.jl:166-187), t = 1 .. t_th-1
-        for (int k = 0; k < a.n_temps; ++k) {
-            const R T = (R)a.temps[k];
-            double sigma = a.sigma0;
-            double acc = 0.0;
-            for (int64_t t = 1; t < a.t_th; ++t) {
-                if (a.or_rate != 0) {
-                    or_sweep();
-                    if (t % a.or_rate == 0) acc += metro_sweep(T, a.kind, sigma);
-                } else {
-                    acc += metro_sweep(T, a.kind, sigma);
-                }
-            }
-            if (threadIdx.x == 0) a.acc_T[(int64_t)sys * a.n_temps + k] = acc;
-        }
-    } else {
-        // parallel_tempering! sweep loop (monte_carlo.jl:295-388) from the
-        // post-exchange part of seg_begin (begin_mid) to the pre-exchange
-        // part of seg_end (end_mid)
-        const PTDev &pt = a.pt;
-        const int slot = pt.slot_of_sys[sys];
-        const R T = (R)pt.temps[slot];
-        double sigma = pt.sigma[slot];
-        double E = pt.E[sys];
-        double acc_local = 0.0;
-        double *binE = pt.binners + (int64_t)slot * 2 * BIN_DOUBLES;
-        double *binM = binE + BIN_DOUBLES;
-        for (int64_t sw = a.seg_begin; sw <= a.seg_end; ++sw) {
-            const bool head = !(sw == a.seg_begin && a.begin_mid);
-            const bool tail = !(sw == a.seg_end && a.end_mid);
-            if (head) {
-                if (pt.or_rate != 0) or_sweep();
-                if (sw % pt.dosweep == 0) {
-                    acc_local += metro_sweep(T, pt.alg, sigma);
-                    double p[5];
-                    wg_energy<R, GEN>(Ld, nb, X, red, p);
-                    E = combine_energy(p);
-                }
-            }
-            if (tail && pt.measure && sw >= pt.t_th && sw % pt.probe == 0) {
-                double m[3];
-                wg_magnet<R>(L.N, X, red, m);
-                if (threadIdx.x == 0) {
-                    bin_push(binE, E);
-                    bin_push(binM, sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]));
-                }
-            }
-        }
-        if (threadIdx.x == 0) {
-            pt.E[sys] = E;
-            pt.sigma[slot] = sigma;
-            pt.acc_local[slot] += acc_local;
-        }
     }
-    if (threadIdx.x == 0) a.mcount[sys] = mctr;
+    __device__ void det_sweep() {
+        wg_colour_sweep<R>(L, [&](int b, int i) { det_update<R, GEN, K2C>(L, nb, X, b, i); });
+    }
+    __device__ double metro(double T, int kind, double sigma, uint32_t ctr) {
+        return wg_metro<R, SELF, GEN, K2C>(L, nb, X, (R)T, kind, (R)sigma, k0, k1, stream, ctr, red);
+    }
+    __device__ void energy(double p[8], bool with_m) {
+        if (with_m) wg_energy<R, GEN, true>(Ld, nb, X, red, p);
+        else wg_energy<R, GEN, false>(Ld, nb, X, red, p);
+    }
+    __device__ void magnet(double m[3]) { wg_magnet<R>(L.N, X, red, m); }
+};
+
+template <int MODE, class R, bool SELF, bool GEN, int K2C, class SV, class NB>
+__device__ __forceinline__ void wg_run(const WGArgs &a, int sys, const LatView<R> &L,
+                                       const LatView<double> &Ld, const SV &X, const NB &nb,
+                                       double *red) {
+    const uint32_t stream = (uint32_t)a.P.sys[sys].stream;
+    SweeperG<R, SELF, GEN, K2C, SV, NB> sw{L, Ld, X, nb, red, stream, a.k0, a.k1};
+    wg_drive<MODE>(a, sys, sw);
 }
 
 __device__ __forceinline__ size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
 
+// ---------------------------------------------------------------------------
+// k_wg: persistent workgroup-per-system sweeps.
 // GEN (cubic / quartic terms) and SELF (a term names its own site) are
 // launched together: the general variant is exact for every lattice, the
 // fast one for bilinear-only lattices without self references.  K2C > 0
 // specialises the bilinear slot count (all lattices of the batch have it).
+// ---------------------------------------------------------------------------
 template <class R, int LM, bool SELF, bool GEN, int MODE, int K2C>
 __global__ __launch_bounds__((GEN || sizeof(R) == 8) ? 512 : 1024) void k_wg(WGArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    double *red = reinterpret_cast<double *>(smem);  // 1 KB reduction scratch
+    double *red = reinterpret_cast<double *>(smem);  // reduction scratch
 
     const int sys = blockIdx.x;
     const SysDev sd = a.P.sys[sys];
@@ -286,60 +188,33 @@ __global__ __launch_bounds__((GEN || sizeof(R) == 8) ? 512 : 1024) void k_wg(WGA
     const int N = L.N;
 
     if constexpr (LM == 0) {
-        wg_run<R, SELF, GEN, MODE, K2C>(a, sys, L, Ld, SpinsAoS<R>{G}, NbrG{L.nbr2, N}, red);
+        wg_run<MODE, R, SELF, GEN, K2C>(a, sys, L, Ld, SpinsAoS<R>{G}, NbrG{L.nbr2, N}, red);
     } else {
         // N + 1 entries per component: the last one is the zero pad site
-        R *sx = reinterpret_cast<R *>(smem + 1024);
+        R *sx = reinterpret_cast<R *>(smem + RED_BYTES);
         SpinsSoA<R> X{sx, sx + (N + 1), sx + 2 * (N + 1)};
-        for (int i = threadIdx.x; i <= N; i += blockDim.x) X.st(i, G[i]);
+        wg_stage<8>(N + 1, [&](int i) { return G[i]; }, [&](int i, const spin4<R> &v) { X.st(i, v); });
         if constexpr (LM == 1) {
             __syncthreads();
-            wg_run<R, SELF, GEN, MODE, K2C>(a, sys, L, Ld, X, NbrG{L.nbr2, N}, red);
+            wg_run<MODE, R, SELF, GEN, K2C>(a, sys, L, Ld, X, NbrG{L.nbr2, N}, red);
         } else {
             uint16_t *idx =
-                reinterpret_cast<uint16_t *>(smem + 1024 + align16(3 * sizeof(R) * (N + 1)));
-            for (int t = threadIdx.x; t < L.K2 * N; t += blockDim.x) idx[t] = (uint16_t)L.nbr2[t];
+                reinterpret_cast<uint16_t *>(smem + RED_BYTES + align16(3 * sizeof(R) * (N + 1)));
+            const int32_t *nbr2 = L.nbr2;
+            wg_stage<8>(L.K2 * N, [&](int t) { return nbr2[t]; },
+                        [&](int t, int32_t v) { idx[t] = (uint16_t)v; });
             __syncthreads();
-            wg_run<R, SELF, GEN, MODE, K2C>(a, sys, L, Ld, X, NbrL{idx, N, L.K2}, red);
+            wg_run<MODE, R, SELF, GEN, K2C>(a, sys, L, Ld, X, NbrL{idx, N, L.K2}, red);
         }
-        __syncthreads();
-        for (int i = threadIdx.x; i < N; i += blockDim.x) G[i] = X.ld(i);
     }
-
-    if (MODE == WG_PT && a.exchange) {
-        // replica exchange (monte_carlo.jl:308-349) by the last workgroup
-        __shared__ int last;
-        __threadfence();
+    // the exchange goes before the LDS modes store the spins back, so that
+    // its fence writes back only the small PT state
+    if (MODE == WG_PT && a.exchange) pt_exchange_last(a, reinterpret_cast<int *>(red + RED_FLAG));
+    if constexpr (LM != 0) {
         __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned prev = atomicAdd(a.pt.done, 1u);
-            last = (prev == gridDim.x - 1);
-        }
-        __syncthreads();
-        if (last && threadIdx.x == 0) {
-            __threadfence();
-            const PTDev &pt = a.pt;
-            const uint32_t xc = (uint32_t)(a.seg_end / pt.swap);
-            for (int r = (xc % 2 == 0) ? 0 : 1; r + 1 < pt.R; r += 2) {
-                const int p = r + 1;
-                const int sr = pt.sys_of_slot[r], sp = pt.sys_of_slot[p];
-                uint4 rr = philox4x32_10(make_uint4((uint32_t)r, 0xFFFFFFFFu, xc, TAG_EXCH), a.k0,
-                                         a.k1);
-                double u = u01d(rr.x);
-                double dbeta = 1.0 / pt.temps[p] - 1.0 / pt.temps[r];
-                double pacc = fmin(1.0, exp(dbeta * (pt.E[sp] - pt.E[sr])));
-                if (u < pacc) {
-                    pt.sys_of_slot[r] = sp;
-                    pt.sys_of_slot[p] = sr;
-                    pt.slot_of_sys[sp] = r;
-                    pt.slot_of_sys[sr] = p;
-                    pt.acc_exch[r] += 1.0;
-                    pt.acc_exch[p] += 1.0;
-                }
-            }
-            *pt.done = 0u;
-            __threadfence();
-        }
+        R *sx = reinterpret_cast<R *>(smem + RED_BYTES);
+        SpinsSoA<R> X{sx, sx + (N + 1), sx + 2 * (N + 1)};
+        for (int i = threadIdx.x; i < N; i += blockDim.x) G[i] = X.ld(i);
     }
 }
 
@@ -351,15 +226,34 @@ int max_lds_bytes() {
     return v;
 }
 
+hipError_t allow_large_lds(const void *fn) {
+    static std::mutex mu;
+    static std::set<std::pair<const void *, int>> done;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (done.count({fn, dev})) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes());
+    if (e == hipSuccess) done.insert({fn, dev});
+    return e;
+}
+
+hipError_t launch_cooperative(const void *fn, int nblocks, int threads, size_t lds_bytes,
+                              const WGArgs &a, hipStream_t st) {
+    void *args[] = {const_cast<WGArgs *>(&a)};
+    return hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(threads), args, (unsigned)lds_bytes, st);
+}
+
 template <class R, int LM, bool GENERAL, int MODE, int K2C>
 static hipError_t launch_wg_t(int nsys, size_t lds_bytes, const WGArgs &a, hipStream_t st) {
-    size_t bytes = 1024 + (LM ? lds_bytes : 0);
+    size_t bytes = RED_BYTES + (LM ? lds_bytes : 0);
     const void *fn = (const void *)k_wg<R, LM, GENERAL, GENERAL, MODE, K2C>;
     if (bytes > 65536) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        hipError_t e = allow_large_lds(fn);
         if (e != hipSuccess) return e;
     }
     const int threads = (GENERAL || sizeof(R) == 8) ? 512 : 1024;
+    if constexpr (MODE == WG_PT_CHAIN) return launch_cooperative(fn, nsys, threads, bytes, a, st);
     hipLaunchKernelGGL((k_wg<R, LM, GENERAL, GENERAL, MODE, K2C>), dim3(nsys), dim3(threads), bytes,
                        st, a);
     return hipGetLastError();
@@ -381,6 +275,8 @@ template <class R, int LM, bool GENERAL>
 static hipError_t launch_wg_g(int mode, int k2c, int nsys, size_t lds_bytes, const WGArgs &a,
                               hipStream_t st) {
     if (mode == WG_PT) return launch_wg_k<R, LM, GENERAL, WG_PT>(k2c, nsys, lds_bytes, a, st);
+    if (mode == WG_PT_CHAIN)
+        return launch_wg_k<R, LM, GENERAL, WG_PT_CHAIN>(k2c, nsys, lds_bytes, a, st);
     if (mode == WG_ANNEAL) return launch_wg_k<R, LM, GENERAL, WG_ANNEAL>(k2c, nsys, lds_bytes, a, st);
     return launch_wg_k<R, LM, GENERAL, WG_SWEEPS>(k2c, nsys, lds_bytes, a, st);
 }
@@ -509,8 +405,8 @@ __global__ __launch_bounds__(1024) void k_energy(DevPools P, const void *spins, 
     const LatDev ld = P.lats[sd.lat];
     const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
     const SpinsAoS<R> S{const_cast<spin4<R> *>(reinterpret_cast<const spin4<R> *>(spins)) + sd.spin_off};
-    double p[5];
-    wg_energy<R, true>(L, NbrG{L.nbr2, L.N}, S, red, p);
+    double p[8];
+    wg_energy<R, true, false>(L, NbrG{L.nbr2, L.N}, S, red, p);
     if (threadIdx.x == 0) {
         E[sys] = combine_energy(p);
         if (parts)
@@ -811,13 +707,38 @@ hipError_t launch_grad_couplings(int prec, const DevPools &P, const void *spins,
                                  int grad_len, double *dEdJ, hipStream_t st) {
     size_t bytes = (size_t)(grad_len > 0 ? grad_len : 1) * sizeof(double);
     if (bytes > 65536) {
-        hipError_t e = hipFuncSetAttribute(prec == 32 ? (const void *)k_grad_couplings<float>
-                                                      : (const void *)k_grad_couplings<double>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        hipError_t e = allow_large_lds(prec == 32 ? (const void *)k_grad_couplings<float>
+                                                  : (const void *)k_grad_couplings<double>);
         if (e != hipSuccess) return e;
     }
     if (prec == 32) hipLaunchKernelGGL((k_grad_couplings<float>), dim3(nsys), dim3(1024), bytes, st, P, spins, grad_len, dEdJ);
     else hipLaunchKernelGGL((k_grad_couplings<double>), dim3(nsys), dim3(1024), bytes, st, P, spins, grad_len, dEdJ);
+    return hipGetLastError();
+}
+
+// Push probes [0, n) of each slot's series into its binners, in order: one
+// workgroup per slot, the E binner on wave 0 and the |M| binner on wave 1,
+// with the binner state staged in LDS.
+__global__ __launch_bounds__(128) void k_bin_fold(const double *series, int cap, int n,
+                                                   double *binners) {
+    __shared__ double st[2 * BIN_DOUBLES];
+    const int slot = blockIdx.x;
+    double *g = binners + (int64_t)slot * 2 * BIN_DOUBLES;
+    for (int k = threadIdx.x; k < 2 * BIN_DOUBLES; k += blockDim.x) st[k] = g[k];
+    __syncthreads();
+    if (threadIdx.x == 0 || threadIdx.x == 64) {
+        const int w = threadIdx.x / 64;
+        const double *q = series + (int64_t)slot * cap * 2 + w;
+        for (int k = 0; k < n; ++k) bin_push(st + w * BIN_DOUBLES, q[2 * k]);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 2 * BIN_DOUBLES; k += blockDim.x) g[k] = st[k];
+}
+
+hipError_t launch_bin_fold(int R, const double *series, int cap, int n, double *binners,
+                           hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bin_fold, dim3(R), dim3(128), 0, st, series, cap, n, binners);
     return hipGetLastError();
 }
 

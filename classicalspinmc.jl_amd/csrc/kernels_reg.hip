@@ -1,0 +1,289 @@
+// kernels_reg.hip -- register-table storage mode of the persistent
+// workgroup-per-system sweep kernel (FP32 state, bilinear-only lattices).
+//
+// One workgroup of REG_THREADS threads owns one system.  Its spins sit in
+// LDS as float4 records (16 B per site, plus the zero pad site), and each
+// thread keeps, in registers, the LDS byte offsets of the sites it updates
+// and of their K2C bilinear neighbours: thread t owns site first + t of every
+// slot (a REG_THREADS-site piece of one run of one colour class).  A site
+// update is then one 16-byte LDS read per spin involved and one 16-byte
+// write, with no neighbour-table traffic and no address arithmetic in the
+// sweep.  The schedule (wg_drive) and the per-site physics (device.h) are the
+// ones every other storage mode uses.
+#include "sweep.h"
+
+namespace csmc {
+
+// spins as spin4<float> records in LDS, addressed by byte offset (16 * site)
+struct SpinsV4 {
+    unsigned char *base;
+    __device__ __forceinline__ spin4<float> ld(uint32_t off) const {
+        return *reinterpret_cast<const spin4<float> *>(base + off);
+    }
+    __device__ __forceinline__ void st(uint32_t off, const spin4<float> &v) const {
+        *reinterpret_cast<spin4<float> *>(base + off) = v;
+    }
+};
+
+template <int K2C, int SPT, bool DIAG>
+struct SweeperR {
+    const LatView<float> &L;
+    const LatView<double> &Ld;
+    SpinsV4 X;
+    double *red;
+    const int32_t *slots;    // int4 records {first site, run end, basis, colour}
+    const int32_t *colslot;  // first slot of each colour, [ncol + 1]
+    int nslots;
+    uint32_t stream, k0, k1;
+    uint32_t valid;          // bit q: this thread owns a site in slot q
+    uint32_t own[SPT];       // 16 * owned site (the zero pad site if none)
+    uint32_t nb[SPT][K2C];   // 16 * bilinear neighbour (pad site if absent)
+
+    // spins a site update reads: its own and its K2C neighbours
+    struct Site {
+        spin4<float> s;
+        spin4<float> sj[K2C];
+    };
+
+    __device__ SweeperR(const LatView<float> &L_, const LatView<double> &Ld_, SpinsV4 X_,
+                        double *red_, const int32_t *slots_, const int32_t *colslot_, int nslots_,
+                        uint32_t stream_, uint32_t k0_, uint32_t k1_)
+        : L(L_), Ld(Ld_), X(X_), red(red_), slots(slots_), colslot(colslot_), nslots(nslots_),
+          stream(stream_), k0(k0_), k1(k1_), valid(0) {
+        // every table load is unconditional (slot index and site clamped to
+        // valid entries) so all of them are in flight together
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) {
+            const int qq = q < nslots ? q : nslots - 1;
+            const int first = ldc(slots + 4 * qq), end = ldc(slots + 4 * qq + 1);
+            const int i = first + (int)threadIdx.x;
+            const bool ok = q < nslots && i < end;
+            const int ic = ok ? i : first;
+            valid |= ok ? (1u << q) : 0u;
+            own[q] = 16u * (uint32_t)(ok ? i : L.N);
+#pragma unroll
+            for (int k = 0; k < K2C; ++k) nb[q][k] = 16u * (uint32_t)L.nbr2[(int64_t)k * L.N + ic];
+        }
+    }
+
+    __device__ int n_sites() const { return L.N; }
+    __device__ bool owns(int q) const { return (valid >> q) & 1u; }
+    __device__ int basis(int q) const { return ldc(slots + 4 * q + 2); }
+
+    __device__ __forceinline__ Site load(int q) const {
+        Site v;
+        v.s = X.ld(own[q]);
+#pragma unroll
+        for (int k = 0; k < K2C; ++k) v.sj[k] = X.ld(nb[q][k]);
+        return v;
+    }
+
+    // one coloured sweep: colour classes in turn, a workgroup barrier after
+    // each.  Slots of one colour are taken two at a time, both slots' spins
+    // loaded before either is updated (the sites are independent), so two
+    // LDS round trips overlap; upd(q, basis, site) updates slot q.
+    template <class UPD>
+    __device__ __forceinline__ void colour_sweep(UPD &&upd) {
+        for (int c = 0; c < L.ncol; ++c) {
+            const int qb = ldc(colslot + c), qe = ldc(colslot + c + 1);
+#pragma unroll
+            for (int q = 0; q < SPT; q += 2) {
+                const bool in0 = q >= qb && q < qe;
+                const bool in1 = q + 1 < SPT && q + 1 >= qb && q + 1 < qe;
+                if (in0 && in1) {
+                    const Site v0 = load(q), v1 = load(q + 1);
+                    if (owns(q)) upd(q, basis(q), v0);
+                    if (owns(q + 1)) upd(q + 1, basis(q + 1), v1);
+                } else if (in0) {
+                    if (owns(q)) upd(q, basis(q), load(q));
+                } else if (in1) {
+                    if (owns(q + 1)) upd(q + 1, basis(q + 1), load(q + 1));
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    template <class A, class S>
+    __device__ __forceinline__ void field(const LatView<S> &Lv, int b, const Site &v,
+                                          A F[3]) const {
+        bilinear_field<S, A, K2C, DIAG>(Lv, b, v.sj, F);
+    }
+
+    __device__ void or_sweep() {
+        colour_sweep([&](int q, int b, const Site &v) {
+            spin4<float> n;
+            float F[3], H[3];
+            field<float>(L, b, v, F);
+            local_field<float, float>(L, b, v.s, F, H);
+            if (or_reflect<float>(v.s, H, n)) X.st(own[q], n);
+        });
+    }
+
+    __device__ void det_sweep() {
+        colour_sweep([&](int q, int b, const Site &v) {
+            spin4<float> n;
+            float F[3], H[3];
+            field<float>(L, b, v, F);
+            local_field<float, float>(L, b, v.s, F, H);
+            if (det_align<float>(L, H, n)) X.st(own[q], n);
+        });
+    }
+
+    __device__ double metro(double T, int kind, double sigma, uint32_t ctr) {
+        const float Tf = (float)T, invT = 1.0f / Tf, sg = (float)sigma;
+        int acc = 0;
+        colour_sweep([&](int q, int b, const Site &v) {
+            const uint32_t site = (uint32_t)L.ref[own[q] >> 4];  // Philox counter: reference index
+            uint4 r = philox4x32_10(make_uint4(site, stream, ctr, TAG_METRO), k0, k1);
+            spin4<float> n = metro_propose<float>(L, v.s, r, kind, sg);
+            float F[3];
+            field<float>(L, b, v, F);
+            int a = metro_accept<float>(metro_dE<float>(L, b, v.s, n, F), r, Tf, invT);
+            metro_commit<float>(L, X, own[q], a, v.s, n);
+            acc += a;
+        });
+        double w[1] = {(double)acc};
+        block_sum<1>(w, red);
+        if (threadIdx.x == 0) red[0] = w[0];
+        __syncthreads();
+        double tot = red[0];
+        __syncthreads();
+        return tot;
+    }
+
+    // FP64 energy parts of the system (bilinear, cubic = 0, quartic = 0,
+    // Zeeman, onsite) in out[0..4] and its summed spin in out[5..7], valid in
+    // every thread
+    __device__ void energy(double out[8], bool with_m) {
+        if (with_m) energy_t<true>(out);
+        else energy_t<false>(out);
+    }
+
+    template <bool WITH_M>
+    __device__ void energy_t(double out[8]) {
+        double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) {
+            if (owns(q)) {
+                const int b = basis(q);
+                const Site v = load(q);
+                const double sx = v.s.x, sy = v.s.y, sz = v.s.z;
+                double F[3];
+                field<double>(Ld, b, v, F);
+                const double *h = Ld.field + b * 3;
+                p[0] += sx * F[0] + sy * F[1] + sz * F[2];
+                p[3] -= sx * ldc(h) + sy * ldc(h + 1) + sz * ldc(h + 2);
+                p[4] += onsite_energy<double, double>(Ld, b, sx, sy, sz);
+                if (WITH_M) {
+                    p[5] += sx;
+                    p[6] += sy;
+                    p[7] += sz;
+                }
+            }
+        }
+        constexpr int NV = WITH_M ? 8 : 5;
+        block_sum<NV>(p, red);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < NV; ++k) red[k] = p[k];
+        __syncthreads();
+        for (int k = 0; k < NV; ++k) out[k] = red[k];
+        __syncthreads();
+    }
+
+    __device__ void magnet(double out[3]) {
+        double m[3] = {0, 0, 0};
+        for (int i = threadIdx.x; i < L.N; i += REG_THREADS) {
+            spin4<float> s = X.ld(16u * (uint32_t)i);
+            m[0] += s.x;
+            m[1] += s.y;
+            m[2] += s.z;
+        }
+        block_sum<3>(m, red);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 3; ++k) red[k] = m[k];
+        __syncthreads();
+        for (int k = 0; k < 3; ++k) out[k] = red[k];
+        __syncthreads();
+    }
+};
+
+template <int K2C, int SPT, bool DIAG, int MODE>
+__global__ __launch_bounds__(REG_THREADS) void k_wg_reg(WGArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *red = reinterpret_cast<double *>(smem);  // reduction scratch
+
+    const int sys = blockIdx.x;
+    const SysDev sd = a.P.sys[sys];
+    const LatDev ld = a.P.lats[sd.lat];
+    const int4 *runs = a.P.runs + ld.run_off;
+    const LatView<float> L = lat_view<float>(ld, a.P.i32, runs, a.P.coefF);
+    const LatView<double> Ld = lat_view<double>(ld, a.P.i32, runs, a.P.coefD);
+    spin4<float> *G = reinterpret_cast<spin4<float> *>(a.spins) + sd.spin_off;
+    const int N = L.N;
+
+    SpinsV4 X{smem + RED_BYTES};
+    // stage the system and its zero pad site N into LDS
+    wg_stage<8>(N + 1, [&](int i) { return G[i]; },
+                [&](int i, const spin4<float> &v) { X.st(16u * (uint32_t)i, v); });
+    SweeperR<K2C, SPT, DIAG> sw(L, Ld, X, red,
+                                reinterpret_cast<const int32_t *>(a.P.runs + ld.slot_off),
+                                a.P.i32 + ld.colslot_off, ld.nslots, (uint32_t)sd.stream, a.k0,
+                                a.k1);
+    __syncthreads();
+    wg_drive<MODE>(a, sys, sw);
+    // the exchange goes first so that its fence writes back only the small
+    // PT state, not the spins stored below
+    if (MODE == WG_PT && a.exchange) pt_exchange_last(a, reinterpret_cast<int *>(red + RED_FLAG));
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += REG_THREADS) G[i] = X.ld(16u * (uint32_t)i);
+}
+
+template <int K2C, int SPT, bool DIAG, int MODE>
+static hipError_t launch_reg_t(int nsys, size_t lds_bytes, const WGArgs &a, hipStream_t st) {
+    const size_t bytes = RED_BYTES + lds_bytes;
+    const void *fn = (const void *)k_wg_reg<K2C, SPT, DIAG, MODE>;
+    if (bytes > 65536) {
+        hipError_t e = allow_large_lds(fn);
+        if (e != hipSuccess) return e;
+    }
+    if constexpr (MODE == WG_PT_CHAIN) return launch_cooperative(fn, nsys, REG_THREADS, bytes, a, st);
+    hipLaunchKernelGGL((k_wg_reg<K2C, SPT, DIAG, MODE>), dim3(nsys), dim3(REG_THREADS), bytes, st, a);
+    return hipGetLastError();
+}
+
+template <int K2C, int SPT, bool DIAG>
+static hipError_t launch_reg_m(int nsys, size_t lds_bytes, const WGArgs &a, hipStream_t st) {
+    if (a.mode == WG_PT) return launch_reg_t<K2C, SPT, DIAG, WG_PT>(nsys, lds_bytes, a, st);
+    if (a.mode == WG_PT_CHAIN) return launch_reg_t<K2C, SPT, DIAG, WG_PT_CHAIN>(nsys, lds_bytes, a, st);
+    if (a.mode == WG_ANNEAL) return launch_reg_t<K2C, SPT, DIAG, WG_ANNEAL>(nsys, lds_bytes, a, st);
+    return launch_reg_t<K2C, SPT, DIAG, WG_SWEEPS>(nsys, lds_bytes, a, st);
+}
+
+template <int K2C, int SPT>
+static hipError_t launch_reg_d(bool diag, int nsys, size_t lds_bytes, const WGArgs &a,
+                               hipStream_t st) {
+    return diag ? launch_reg_m<K2C, SPT, true>(nsys, lds_bytes, a, st)
+                : launch_reg_m<K2C, SPT, false>(nsys, lds_bytes, a, st);
+}
+
+template <int K2C>
+static hipError_t launch_reg_s(int spt, bool diag, int nsys, size_t lds_bytes, const WGArgs &a,
+                               hipStream_t st) {
+    if (spt == 4) return launch_reg_d<K2C, 4>(diag, nsys, lds_bytes, a, st);
+    // 8 slots of 6 neighbours exceed the register budget of 1024 threads
+    if constexpr (K2C <= 4) {
+        if (spt == 8) return launch_reg_d<K2C, 8>(diag, nsys, lds_bytes, a, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_wg_reg(int k2c, int spt, bool diag, int nsys, size_t lds_bytes, const WGArgs &a,
+                         hipStream_t st) {
+    if (k2c == 3) return launch_reg_s<3>(spt, diag, nsys, lds_bytes, a, st);
+    if (k2c == 4) return launch_reg_s<4>(spt, diag, nsys, lds_bytes, a, st);
+    if (k2c == 6) return launch_reg_s<6>(spt, diag, nsys, lds_bytes, a, st);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace csmc

@@ -177,6 +177,11 @@ int32_t csmc_sweep_plain(csmc_ctx *ctx, int32_t batch_id, int32_t kind, const do
 /* Set / read the per-system Metropolis sweep counters (Philox counter word). */
 int32_t csmc_set_counters(csmc_ctx *ctx, int32_t batch_id, const uint32_t *counters);
 int32_t csmc_get_counters(csmc_ctx *ctx, int32_t batch_id, uint32_t *counters);
+/* no reference counterpart: the storage mode the persistent sweep kernel
+ * uses for this batch (3 register tables + LDS spins, 2 LDS spins and
+ * neighbour tables, 1 LDS spins, 0 global memory); the environment variable
+ * CSMC_LDS=0..3, read at context creation, caps it. */
+int32_t csmc_batch_storage_mode(csmc_ctx *ctx, int32_t batch_id, int32_t *mode);
 
 /* ---- simulated_annealing! (monte_carlo.jl:157-190) ------------------------ */
 /* For each temperature temps[k] (k < n_temps, the caller evaluates the

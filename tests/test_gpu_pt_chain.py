@@ -1,0 +1,71 @@
+"""Chained parallel tempering (one cooperative launch for many exchange
+periods, a grid barrier and per-pair exchange decisions at each exchange)
+against one launch per period: the same schedule, bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+import csmc
+from csmc import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(chain):
+    old = os.environ.get("CSMC_PT_CHAIN")
+    os.environ["CSMC_PT_CHAIN"] = "1" if chain else "0"
+    try:
+        return csmc.Context(0)
+    finally:
+        if old is None:
+            del os.environ["CSMC_PT_CHAIN"]
+        else:
+            os.environ["CSMC_PT_CHAIN"] = old
+
+
+CASES = [  # name, cell, shape, S, precision, replicas, alg, schedule
+    ("honeycomb_fp32_R16", lambda: W.honeycomb_kitaev(1.0, -2.0), (16, 16), 0.5, 32, 16, csmc.METROPOLIS,
+     dict(t_thermalization=40, t_measurement=200, probe_rate=10, swap_rate=10, overrelaxation_rate=10,
+          report_interval=10 ** 9, checkpoint_rate=10 ** 9)),
+    ("square_fp64_R7_events", lambda: W.square_heisenberg(-1.0, (0, 0, 0.1)), (8, 8), 1.0, 64, 7,
+     csmc.CONE_ADAPTIVE,
+     dict(t_thermalization=13, t_measurement=120, probe_rate=3, swap_rate=4, overrelaxation_rate=2,
+          report_interval=50, checkpoint_rate=25)),
+    ("pyrochlore_fp32_R5", W.pyrochlore_mixed, (2, 2, 2), 1.0, 32, 5, csmc.METROPOLIS,
+     dict(t_thermalization=10, t_measurement=60, probe_rate=2, swap_rate=3, overrelaxation_rate=1,
+          report_interval=10 ** 9, checkpoint_rate=10 ** 9)),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_chained_pt_matches_per_period_launches(case):
+    name, mk, shape, S, prec, R, alg, p = case
+    lat = csmc.Lattice(shape, mk(), S, seed=3)
+    temps = W.geometric_temperatures(0.2, 3.0, R)
+    out = []
+    for chain in (True, False):
+        ctx = _ctx(chain)
+        b = csmc.SpinBatch([lat] * R, precision=prec, ctx=ctx)
+        b.randomize(17)
+        b.pt_init(temps, p, alg, sigma0=0.6, seed=23)
+        events = []
+        while True:
+            stopped, ev = b.pt_advance(-1)
+            events.append((stopped, ev))
+            if ev == 3:
+                break
+        out.append((events, b.pt_state(), b.pt_observables(), [b.get_spins(r) for r in range(R)],
+                    b.counters()))
+        b.close()
+        ctx.close()
+    (e1, st1, ob1, sp1, c1), (e2, st2, ob2, sp2, c2) = out
+    assert e1 == e2
+    for k in st1:
+        assert np.array_equal(st1[k], st2[k]), k
+    for k in ob1:
+        assert np.array_equal(ob1[k], ob2[k]), k
+    for x, y in zip(sp1, sp2):
+        assert np.array_equal(x, y)
+    assert np.array_equal(c1, c2)
+    assert st1["accepted_exchange"].sum() > 0
