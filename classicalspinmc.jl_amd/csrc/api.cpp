@@ -1031,10 +1031,14 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
         pt.probe_pos += n;
         return CSMC_OK;
     };
-    auto flush = [&]() -> int {
+    // launch the pending chain; tail >= 0 adds the non-exchange segment from
+    // the chain's last exchange to the end of sweep `tail`
+    auto flush = [&](int64_t tail) -> int {
         if (nchain == 0) return CSMC_OK;
         int rc2 = CSMC_OK;
         bool done_chain = false;
+        const int64_t last_e = chain_e0 + (nchain - 1) * chain_P;
+        if (tail >= 0) chain_probes += probes_in(last_e, tail, false);
         if (nchain > 1 && pt.chain_ok) {
             WGArgs ca = a;
             ca.mode = WG_PT_CHAIN;
@@ -1042,6 +1046,7 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
             ca.begin_mid = chain_bmid;
             ca.chain_e0 = chain_e0;
             ca.chain_P = chain_P;
+            ca.chain_tail = tail;
             ca.nseg = nchain;
             rc2 = reserve(chain_probes);
             if (rc2) return rc2;
@@ -1061,6 +1066,10 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
                 const int64_t se = chain_e0 + g * chain_P;
                 const int64_t sb = g == 0 ? chain_begin : se - chain_P;
                 rc2 = launch_one(sb, se, g == 0 ? chain_bmid : true, true, true);
+                if (rc2) return rc2;
+            }
+            if (tail >= 0) {
+                rc2 = launch_one(last_e, tail, true, false, false);
                 if (rc2) return rc2;
             }
         }
@@ -1096,7 +1105,7 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
             const bool extends = nchain > 0 && chain_probes + np <= PROBE_CAP &&
                                  (nchain == 1 ? s > chain_e0 : s == chain_e0 + nchain * chain_P);
             if (nchain > 0 && !extends) {
-                rc = flush();
+                rc = flush(-1);
                 if (rc) return rc;
             }
             if (nchain == 0) {
@@ -1111,14 +1120,21 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
             pt.sweep = s;
             pt.mid = true;
             if (nchain >= chain_max) {
-                rc = flush();
+                rc = flush(-1);
                 if (rc) return rc;
             }
             continue;
         }
-        rc = flush();
-        if (rc) return rc;
-        rc = launch_one(pt.sweep, s, begin_mid, end_mid, exch);
+        if (nchain > 0 && !exch && !end_mid &&
+            chain_probes + probes_in(pt.sweep, s, false) <= PROBE_CAP) {
+            // the rest of the chain's last sweep (and beyond, up to a host
+            // event) rides in the chain's launch
+            rc = flush(s);
+        } else {
+            rc = flush(-1);
+            if (rc) return rc;
+            rc = launch_one(pt.sweep, s, begin_mid, end_mid, exch);
+        }
         if (rc) return rc;
         if (end_mid) {
             pt.sweep = s;
@@ -1136,7 +1152,7 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
             }
         }
     }
-    rc = flush();
+    rc = flush(-1);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     if (chained) {

@@ -48,6 +48,8 @@ struct WGArgs {
     // WG_PT_CHAIN: nseg segments, the first from seg_begin (begin_mid) to the
     // exchange at chain_e0, segment g to the exchange at chain_e0 + g * chain_P
     int64_t chain_e0, chain_P;
+    int64_t chain_tail;  // >= 0: after the last exchange, run on to the end of
+                         // sweep chain_tail without a further exchange
     int nseg;
     uint32_t spin_limit;  // polls of a barrier wait before it gives up
     int64_t probe_base;   // series index of this launch's first probe

@@ -298,6 +298,7 @@ __device__ void wg_drive(const WGArgs &a, int sys, SW &sw) {
                 sb = se;
                 bmid = true;
             }
+            if (a.chain_tail >= 0) segment(sb, a.chain_tail, bmid, false);
             if (threadIdx.x == 0) {
                 pt.E[sys] = E;
                 pt.sigma[slot] = sigma;
