@@ -111,6 +111,35 @@ def finish(out, args):
     print(json.dumps(out))
 
 
+def fp32_error(csmc, W, lat, temps, ctx, R=8, steps=(1, 10)):
+    """Error of the FP32 state against the FP64 state on the device: the same
+    start (FP32-representable spins), the same schedule and Philox stream, in
+    an FP32 and an FP64 batch; the total energies (both reduced in FP64) after
+    1 and 10 exchange periods, relative, max over R replicas.  Trajectories
+    separate where a Metropolis decision falls differently, so the figure
+    after 10 periods bounds the drift of a run, not an evaluation error."""
+    p = dict(t_thermalization=0, t_measurement=10 ** 6, probe_rate=10, swap_rate=10, overrelaxation_rate=10,
+             report_interval=10 ** 9, checkpoint_rate=10 ** 9)
+    b32 = csmc.SpinBatch([lat] * R, precision=32, ctx=ctx)
+    b64 = csmc.SpinBatch([lat] * R, precision=64, ctx=ctx)
+    b32.randomize(11)
+    for i in range(R):
+        b64.set_spins(i, b32.get_spins(i))
+    t = temps[:: max(1, len(temps) // R)][:R]
+    out = {"start_energy_rel": float(np.max(np.abs(b32.total_energy() - b64.total_energy())
+                                            / np.abs(b64.total_energy())))}
+    for b in (b32, b64):
+        b.pt_init(t, p, csmc.METROPOLIS, seed=4)
+    for k in steps:
+        for b in (b32, b64):
+            b.pt_advance(k * 10)
+        e32, e64 = b32.total_energy(), b64.total_energy()
+        out[f"after_{k}_steps_energy_rel"] = float(np.max(np.abs(e32 - e64) / np.abs(e64)))
+    b32.close()
+    b64.close()
+    return out
+
+
 def run_pt(args, W, csmc, uc, L, S, R, temps, precision, name):
     lat = csmc.Lattice((L, L) if uc.D == 2 else (L, L, L), uc, S, seed=2)
     N = lat.size
@@ -227,6 +256,7 @@ def main():
         "roofline": roofline,
     }
     r["b"].close()
+    out["precision"] = fp32_error(csmc, W, r["lat"], temps, ctx)
 
     if not args.no_plain:
         # plain unbatched FP64 form: one replica at a time, one launch per
