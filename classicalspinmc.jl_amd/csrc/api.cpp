@@ -248,6 +248,15 @@ void append_lattice(csmc_ctx *c, const HostLattice &L) {
     }
     I.push_back(nslots);
     d.nslots = nslots;
+    // basis shared by all runs of a colour (-1 if the colour mixes bases)
+    d.flags |= LAT_COLBASIS;
+    for (int col = 0; col < L.ncol; ++col) {
+        int cb = L.runs[L.colrun[col]][2];
+        for (int r = L.colrun[col]; r < L.colrun[col + 1]; ++r)
+            if (L.runs[r][2] != cb) cb = -1;
+        if (cb < 0) d.flags &= ~LAT_COLBASIS;
+        I.push_back(cb);
+    }
     // incidence lists: participant -> (owner, kind, role, slot)
     std::vector<std::vector<Inc>> inc(N);
     for (int i = 0; i < N; ++i) {
@@ -344,7 +353,9 @@ int choose_lm(csmc_ctx *c, Batch *b, size_t *bytes) {
     size_t idx = al(2 * (size_t)b->maxK2 * b->maxN);
     size_t avail = (size_t)c->max_lds > RED_BYTES ? (size_t)c->max_lds - RED_BYTES : 0;
     const bool k2ok = b->k2c == 3 || b->k2c == 4 || b->k2c == 6;
-    if (c->lds_cap >= 3 && c->specialise && b->prec == 32 && !b->self && !b->gen && k2ok &&
+    bool colbasis = true;
+    for (int id : b->lat) colbasis = colbasis && (c->h_lat[id].flags & LAT_COLBASIS);
+    if (c->lds_cap >= 3 && c->specialise && b->prec == 32 && !b->self && !b->gen && k2ok && colbasis &&
         reg_spt(b->nslots, b->k2c) > 0 && 16 * (size_t)(b->maxN + 1) <= avail) {
         *bytes = 16 * (size_t)(b->maxN + 1);
         return 3;
@@ -574,7 +585,7 @@ int32_t csmc_lattice_update_couplings(csmc_ctx *c, int32_t id, const csmc_unit_c
     if (rc) return fail(rc, err);
     std::vector<double> dc = device_coef(L);
     std::copy(dc.begin(), dc.end(), c->h_coef.begin() + c->h_lat[id].coef_off);
-    c->h_lat[id].flags = coupling_flags(L);
+    c->h_lat[id].flags = coupling_flags(L) | (c->h_lat[id].flags & LAT_COLBASIS);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->dirty = true;
     return CSMC_OK;

@@ -47,7 +47,8 @@ struct LatDev {
     int32_t grad_len;    // coupling-gradient length per system
     int32_t nslots;      // register-table schedule: thread slots per sweep
     int64_t slot_off;    // run pool: int4 {first site, run end, basis, colour}
-    int64_t colslot_off; // i32 pool: first slot of each colour [ncol + 1]
+    int64_t colslot_off; // i32 pool: first slot of each colour [ncol + 1], then
+                         // the basis all slots of a colour share (-1 if mixed) [ncol]
     double S;
 };
 
@@ -71,6 +72,7 @@ struct SysDev {
 constexpr int32_t LAT_ONSITE = 1;
 constexpr int32_t LAT_FIELD = 2;
 constexpr int32_t LAT_DIAG = 4;  // every bilinear coupling matrix is diagonal
+constexpr int32_t LAT_COLBASIS = 8;  // every colour class holds sites of one basis
 
 // incidence entry: owner device site and a packed (kind, slot, role)
 //   code = kind(2 bits: 0 bil, 1 cub, 2 qua) | role << 2 (2 bits) | slot << 4

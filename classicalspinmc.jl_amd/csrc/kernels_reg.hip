@@ -14,14 +14,72 @@
 
 namespace csmc {
 
-// spins as spin4<float> records in LDS, addressed by byte offset (16 * site)
+// spins as spin4<float> records in LDS, addressed by their absolute LDS byte
+// address (so an access needs no base add)
+typedef float lds_f4 __attribute__((ext_vector_type(4)));
+using lds_byte = __attribute__((address_space(3))) unsigned char;
+using lds_vec = __attribute__((address_space(3))) lds_f4;
+
+__device__ __forceinline__ uint32_t lds_address(const void *p) {
+    return (uint32_t)(size_t)(const lds_byte *)p;
+}
+
 struct SpinsV4 {
-    unsigned char *base;
-    __device__ __forceinline__ spin4<float> ld(uint32_t off) const {
-        return *reinterpret_cast<const spin4<float> *>(base + off);
+    __device__ __forceinline__ spin4<float> ld(uint32_t a) const {
+        const lds_f4 v = *(const lds_vec *)(size_t)a;
+        return spin4<float>{v.x, v.y, v.z, v.w};
     }
-    __device__ __forceinline__ void st(uint32_t off, const spin4<float> &v) const {
-        *reinterpret_cast<spin4<float> *>(base + off) = v;
+    __device__ __forceinline__ void st(uint32_t a, const spin4<float> &v) const {
+        *(lds_vec *)(size_t)a = lds_f4{v.x, v.y, v.z, v.w};
+    }
+};
+
+// the bilinear couplings of one basis, K2C slots, held in registers (the
+// values are wave-uniform, so they live in SGPRs); DIAG keeps m11, m22, m33
+template <class S, int K2C, bool DIAG>
+struct BilCoef {
+    static constexpr int W = DIAG ? 3 : 9;
+    S m[K2C][W];
+    __device__ __forceinline__ void load(const LatView<S> &L, int b) {
+#pragma unroll
+        for (int k = 0; k < K2C; ++k)
+#pragma unroll
+            for (int e = 0; e < W; ++e) m[k][e] = ldc(L.c2 + (b * L.K2P + k) * 9 + (DIAG ? 4 * e : e));
+    }
+    // the bilinear field, in the operation order of bilinear_field
+    template <class A>
+    __device__ __forceinline__ void field(const spin4<float> *sj, A F[3]) const {
+#pragma unroll
+        for (int k = 0; k < K2C; ++k) {
+            const A x = sj[k].x, y = sj[k].y, z = sj[k].z;
+            if constexpr (DIAG) {
+                if (k == 0) {
+                    F[0] = (A)m[k][0] * x;
+                    F[1] = (A)m[k][1] * y;
+                    F[2] = (A)m[k][2] * z;
+                } else {
+                    F[0] += (A)m[k][0] * x;
+                    F[1] += (A)m[k][1] * y;
+                    F[2] += (A)m[k][2] * z;
+                }
+            } else {
+                if (k == 0) {
+                    F[0] = (A)m[k][0] * x;
+                    F[1] = (A)m[k][3] * x;
+                    F[2] = (A)m[k][6] * x;
+                } else {
+                    F[0] += (A)m[k][0] * x;
+                    F[1] += (A)m[k][3] * x;
+                    F[2] += (A)m[k][6] * x;
+                }
+                F[0] += (A)m[k][1] * y;
+                F[1] += (A)m[k][4] * y;
+                F[2] += (A)m[k][7] * y;
+                F[0] += (A)m[k][2] * z;
+                F[1] += (A)m[k][5] * z;
+                F[2] += (A)m[k][8] * z;
+            }
+        }
     }
 };
 
@@ -32,12 +90,14 @@ struct SweeperR {
     SpinsV4 X;
     double *red;
     const int32_t *slots;    // int4 records {first site, run end, basis, colour}
-    const int32_t *colslot;  // first slot of each colour, [ncol + 1]
+    const int32_t *colslot;  // first slot of each colour [ncol + 1], then the
+                             // basis shared by a colour's slots (-1 if mixed) [ncol]
     int nslots;
     uint32_t stream, k0, k1;
+    uint32_t base;           // LDS address of site 0
     uint32_t valid;          // bit q: this thread owns a site in slot q
-    uint32_t own[SPT];       // 16 * owned site (the zero pad site if none)
-    uint32_t nb[SPT][K2C];   // 16 * bilinear neighbour (pad site if absent)
+    uint32_t own[SPT];       // LDS address of the owned site (the zero pad site if none)
+    uint32_t nb[SPT][K2C];   // LDS address of each bilinear neighbour (pad site if absent)
 
     // spins a site update reads: its own and its K2C neighbours
     struct Site {
@@ -45,11 +105,11 @@ struct SweeperR {
         spin4<float> sj[K2C];
     };
 
-    __device__ SweeperR(const LatView<float> &L_, const LatView<double> &Ld_, SpinsV4 X_,
+    __device__ SweeperR(const LatView<float> &L_, const LatView<double> &Ld_, uint32_t base_,
                         double *red_, const int32_t *slots_, const int32_t *colslot_, int nslots_,
                         uint32_t stream_, uint32_t k0_, uint32_t k1_)
-        : L(L_), Ld(Ld_), X(X_), red(red_), slots(slots_), colslot(colslot_), nslots(nslots_),
-          stream(stream_), k0(k0_), k1(k1_), valid(0) {
+        : L(L_), Ld(Ld_), X(), red(red_), slots(slots_), colslot(colslot_), nslots(nslots_),
+          stream(stream_), k0(k0_), k1(k1_), base(base_), valid(0) {
         // every table load is unconditional (slot index and site clamped to
         // valid entries) so all of them are in flight together
 #pragma unroll
@@ -60,15 +120,17 @@ struct SweeperR {
             const bool ok = q < nslots && i < end;
             const int ic = ok ? i : first;
             valid |= ok ? (1u << q) : 0u;
-            own[q] = 16u * (uint32_t)(ok ? i : L.N);
+            own[q] = base + 16u * (uint32_t)(ok ? i : L.N);
 #pragma unroll
-            for (int k = 0; k < K2C; ++k) nb[q][k] = 16u * (uint32_t)L.nbr2[(int64_t)k * L.N + ic];
+            for (int k = 0; k < K2C; ++k)
+                nb[q][k] = base + 16u * (uint32_t)L.nbr2[(int64_t)k * L.N + ic];
         }
     }
 
     __device__ int n_sites() const { return L.N; }
     __device__ bool owns(int q) const { return (valid >> q) & 1u; }
     __device__ int basis(int q) const { return ldc(slots + 4 * q + 2); }
+    __device__ int site(int q) const { return (int)((own[q] - base) >> 4); }
 
     __device__ __forceinline__ Site load(int q) const {
         Site v;
@@ -79,52 +141,65 @@ struct SweeperR {
     }
 
     // one coloured sweep: colour classes in turn, a workgroup barrier after
-    // each.  Slots of one colour are taken two at a time, both slots' spins
-    // loaded before either is updated (the sites are independent), so two
-    // LDS round trips overlap; upd(q, basis, site) updates slot q.
+    // each.  Every colour class has a single basis (the host only picks this
+    // mode then), so its couplings are loaded once for the colour.  Slots are
+    // taken two at a time, both slots' spins loaded before either is updated
+    // (the sites are independent), so two LDS round trips overlap;
+    // upd(q, basis, site, M) updates slot q with couplings M.
     template <class UPD>
     __device__ __forceinline__ void colour_sweep(UPD &&upd) {
         for (int c = 0; c < L.ncol; ++c) {
             const int qb = ldc(colslot + c), qe = ldc(colslot + c + 1);
-#pragma unroll
-            for (int q = 0; q < SPT; q += 2) {
-                const bool in0 = q >= qb && q < qe;
-                const bool in1 = q + 1 < SPT && q + 1 >= qb && q + 1 < qe;
-                if (in0 && in1) {
-                    const Site v0 = load(q), v1 = load(q + 1);
-                    if (owns(q)) upd(q, basis(q), v0);
-                    if (owns(q + 1)) upd(q + 1, basis(q + 1), v1);
-                } else if (in0) {
-                    if (owns(q)) upd(q, basis(q), load(q));
-                } else if (in1) {
-                    if (owns(q + 1)) upd(q + 1, basis(q + 1), load(q + 1));
-                }
-            }
+            const int cb = ldc(colslot + L.ncol + 1 + c);
+            BilCoef<float, K2C, DIAG> M;
+            M.load(L, cb);
+            slot_pairs(qb, qe, [&](int q, const Site &v) { upd(q, cb, v, M); });
             __syncthreads();
         }
     }
 
-    template <class A, class S>
-    __device__ __forceinline__ void field(const LatView<S> &Lv, int b, const Site &v,
-                                          A F[3]) const {
-        bilinear_field<S, A, K2C, DIAG>(Lv, b, v.sj, F);
+    template <class F>
+    __device__ __forceinline__ void slot_pairs(int qb, int qe, F &&f) {
+#pragma unroll
+        for (int q = 0; q < SPT; q += 2) {
+            const bool in0 = q >= qb && q < qe;
+            const bool in1 = q + 1 < SPT && q + 1 >= qb && q + 1 < qe;
+            if (in0 && in1) {
+                const Site v0 = load(q), v1 = load(q + 1);
+                if (owns(q)) f(q, v0);
+                if (owns(q + 1)) f(q + 1, v1);
+            } else if (in0) {
+                if (owns(q)) f(q, load(q));
+            } else if (in1) {
+                if (owns(q + 1)) f(q + 1, load(q + 1));
+            }
+        }
     }
 
     __device__ void or_sweep() {
-        colour_sweep([&](int q, int b, const Site &v) {
+        colour_sweep([&](int q, int b, const Site &v, const BilCoef<float, K2C, DIAG> &M) {
             spin4<float> n;
             float F[3], H[3];
-            field<float>(L, b, v, F);
+            M.field(v.sj, F);
             local_field<float, float>(L, b, v.s, F, H);
-            if (or_reflect<float>(v.s, H, n)) X.st(own[q], n);
+            // or_reflect, with the H == 0 case (no change) taken by a select
+            // instead of a branch: the spin is stored back unchanged
+            const bool zero = H[0] == 0.0f && H[1] == 0.0f && H[2] == 0.0f;
+            const float hh = H[0] * H[0] + H[1] * H[1] + H[2] * H[2];
+            const float proj = Math<float>::div(2.0f * (v.s.x * H[0] + v.s.y * H[1] + v.s.z * H[2]), hh);
+            n.x = zero ? v.s.x : -v.s.x + proj * H[0];
+            n.y = zero ? v.s.y : -v.s.y + proj * H[1];
+            n.z = zero ? v.s.z : -v.s.z + proj * H[2];
+            n.w = 0.0f;
+            X.st(own[q], n);
         });
     }
 
     __device__ void det_sweep() {
-        colour_sweep([&](int q, int b, const Site &v) {
+        colour_sweep([&](int q, int b, const Site &v, const BilCoef<float, K2C, DIAG> &M) {
             spin4<float> n;
             float F[3], H[3];
-            field<float>(L, b, v, F);
+            M.field(v.sj, F);
             local_field<float, float>(L, b, v.s, F, H);
             if (det_align<float>(L, H, n)) X.st(own[q], n);
         });
@@ -133,12 +208,13 @@ struct SweeperR {
     __device__ double metro(double T, int kind, double sigma, uint32_t ctr) {
         const float Tf = (float)T, invT = 1.0f / Tf, sg = (float)sigma;
         int acc = 0;
-        colour_sweep([&](int q, int b, const Site &v) {
-            const uint32_t site = (uint32_t)L.ref[own[q] >> 4];  // Philox counter: reference index
-            uint4 r = philox4x32_10(make_uint4(site, stream, ctr, TAG_METRO), k0, k1);
+        colour_sweep([&](int q, int b, const Site &v, const BilCoef<float, K2C, DIAG> &M) {
+            // Philox counter: the reference index of the site
+            const uint32_t ref = (uint32_t)L.ref[site(q)];
+            uint4 r = philox4x32_10(make_uint4(ref, stream, ctr, TAG_METRO), k0, k1);
             spin4<float> n = metro_propose<float>(L, v.s, r, kind, sg);
             float F[3];
-            field<float>(L, b, v, F);
+            M.field(v.sj, F);
             int a = metro_accept<float>(metro_dE<float>(L, b, v.s, n, F), r, Tf, invT);
             metro_commit<float>(L, X, own[q], a, v.s, n);
             acc += a;
@@ -169,8 +245,10 @@ struct SweeperR {
                 const int b = basis(q);
                 const Site v = load(q);
                 const double sx = v.s.x, sy = v.s.y, sz = v.s.z;
+                BilCoef<double, K2C, DIAG> M;
+                M.load(Ld, b);
                 double F[3];
-                field<double>(Ld, b, v, F);
+                M.field(v.sj, F);
                 const double *h = Ld.field + b * 3;
                 p[0] += sx * F[0] + sy * F[1] + sz * F[2];
                 p[3] -= sx * ldc(h) + sy * ldc(h + 1) + sz * ldc(h + 2);
@@ -194,7 +272,7 @@ struct SweeperR {
     __device__ void magnet(double out[3]) {
         double m[3] = {0, 0, 0};
         for (int i = threadIdx.x; i < L.N; i += REG_THREADS) {
-            spin4<float> s = X.ld(16u * (uint32_t)i);
+            spin4<float> s = X.ld(base + 16u * (uint32_t)i);
             m[0] += s.x;
             m[1] += s.y;
             m[2] += s.z;
@@ -222,11 +300,12 @@ __global__ __launch_bounds__(REG_THREADS) void k_wg_reg(WGArgs a) {
     spin4<float> *G = reinterpret_cast<spin4<float> *>(a.spins) + sd.spin_off;
     const int N = L.N;
 
-    SpinsV4 X{smem + RED_BYTES};
+    const uint32_t base = lds_address(smem + RED_BYTES);
+    SpinsV4 X;
     // stage the system and its zero pad site N into LDS
     wg_stage<8>(N + 1, [&](int i) { return G[i]; },
-                [&](int i, const spin4<float> &v) { X.st(16u * (uint32_t)i, v); });
-    SweeperR<K2C, SPT, DIAG> sw(L, Ld, X, red,
+                [&](int i, const spin4<float> &v) { X.st(base + 16u * (uint32_t)i, v); });
+    SweeperR<K2C, SPT, DIAG> sw(L, Ld, base, red,
                                 reinterpret_cast<const int32_t *>(a.P.runs + ld.slot_off),
                                 a.P.i32 + ld.colslot_off, ld.nslots, (uint32_t)sd.stream, a.k0,
                                 a.k1);
@@ -236,7 +315,7 @@ __global__ __launch_bounds__(REG_THREADS) void k_wg_reg(WGArgs a) {
     // PT state, not the spins stored below
     if (MODE == WG_PT && a.exchange) pt_exchange_last(a, reinterpret_cast<int *>(red + RED_FLAG));
     __syncthreads();
-    for (int i = threadIdx.x; i < N; i += REG_THREADS) G[i] = X.ld(16u * (uint32_t)i);
+    for (int i = threadIdx.x; i < N; i += REG_THREADS) G[i] = X.ld(base + 16u * (uint32_t)i);
 }
 
 template <int K2C, int SPT, bool DIAG, int MODE>
