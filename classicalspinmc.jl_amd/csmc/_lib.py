@@ -82,6 +82,7 @@ SIGNATURES = {
     "csmc_timing": [_P, _I32],
     "csmc_timing_read": [_P, _P, _P],
     "csmc_build_tables": [_P, _P, _I32, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "csmc_split_plan": [_P, _P, _I32, _I32, _I32, _P],
     "csmc_lattice_create": [_P, _P, _P, _I32, _D, _P],
     "csmc_lattice_info_get": [_P, _I32, _P],
     "csmc_lattice_tables": [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P],
@@ -105,6 +106,7 @@ SIGNATURES = {
     "csmc_pt_init": [_P, _I32, _P, _P, _D, _U64],
     "csmc_pt_advance": [_P, _I32, _I64, _P, _P],
     "csmc_pt_state": [_P, _I32, _P, _P, _P, _P, _P],
+    "csmc_pt_split_parts": [_P, _I32, _P],
     "csmc_pt_observables": [_P, _I32, _P],
     "csmc_energy_grad_spins": [_P, _I32, _P],
     "csmc_energy_grad_couplings": [_P, _I32, _P],

@@ -153,6 +153,13 @@ class SpinBatch:
                                             _lib.ptr(ax), _lib.ptr(sg)))
         return dict(system_of_slot=sos, E=E, accepted_local=al, accepted_exchange=ax, sigma=sg)
 
+    def pt_split_parts(self):
+        """workgroups per system of the chained PT kernel (1: whole system
+        per workgroup; G > 1: strips with halos)"""
+        g = C.c_int32()
+        _lib.check(_lib.lib().csmc_pt_split_parts(self._h, self.id, C.byref(g)))
+        return g.value
+
     def pt_observables(self):
         out = np.zeros((self.n, 8))
         _lib.check(_lib.lib().csmc_pt_observables(self._h, self.id, _lib.ptr(out)))

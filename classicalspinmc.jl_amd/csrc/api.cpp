@@ -21,6 +21,7 @@
 #include "csmc_internal.h"
 #include "kernels.h"
 #include "lattice.h"
+#include "split.h"
 
 using namespace csmc;
 
@@ -68,16 +69,22 @@ struct PTHost {
     DevBuf<double> temps_d, E, sigma, acc_local, acc_exch, bins;
     DevBuf<int32_t> slot_of_sys, sys_of_slot;
     DevBuf<uint32_t> done;
-    // chained launches: double-buffered exchange state, barrier counter and
+    // chained launches: the exchange records ([PT_RECBUF][R][4]) and the
     // wait-failure flag
-    DevBuf<double> E2, sigma2;
-    DevBuf<int32_t> slot_of_sys2, sys_of_slot2;
-    DevBuf<uint32_t> bar, err;
+    DevBuf<uint64_t> rec;
+    DevBuf<uint32_t> err;
     bool chain_ok = true;  // cleared when a cooperative launch is refused
     // probe series not yet folded into the binners: probes [0, probe_pos)
     DevBuf<double> series;
     int64_t probe_pos = 0;
     uint64_t seed = 0;
+    // split form of the chained kernel: split_G workgroups per system (1:
+    // the whole-system kernel), its tables, exchange words and epochs
+    int split_G = 1, split_nor = 0, split_spt = 0, split_bmax = 0;
+    size_t split_lds = 0;
+    DevBuf<int32_t> split_tab;
+    DevBuf<uint64_t> split_hx, split_px;
+    DevBuf<uint32_t> split_epoch;
 };
 
 constexpr int PROBE_CAP = 4096;  // probes per slot held before a fold
@@ -126,6 +133,9 @@ struct csmc_ctx {
     int lds_cap = 3;  // highest workgroup storage mode allowed
     bool specialise = true;  // use slot-count specialised kernels when they exist
     bool pt_chain = true;    // chain PT exchange periods in cooperative launches
+    int split = 0;           // parts per system of the split PT kernel: 0 chosen
+                             // from the plan's cost, 1 never split, 2..8 forced
+    int ncu = 0;             // compute units of the device
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     double timed_ms = 0.0;
@@ -372,6 +382,95 @@ int choose_lm(csmc_ctx *c, Batch *b, size_t *bytes) {
     return 0;
 }
 
+// LDS of a split part: local spins and the pad site, its boundary list, its
+// halo pairs
+size_t split_lds_bytes(const SplitPlan &p) {
+    return 16 * (size_t)(p.maxloc + 1) + 4 * (size_t)p.maxbnd + 8 * (size_t)p.maxhalo + 8;
+}
+
+// Split form of the chained PT kernel (kernels_split.hip): plan parts for
+// the batch's lattice and upload the tables, or leave split_G = 1 when the
+// whole-system kernel is the better (or the only) choice.  Automatic choice:
+// the plan whose largest part does the least work, if that is at most 3/4 of
+// the whole system's and G * R workgroups fit the device at once.
+int setup_split(csmc_ctx *c, Batch *b) {
+    PTHost &pt = b->pt;
+    pt.split_G = 1;
+    const int R = b->nsys;
+    size_t whole = 0;
+    if (c->split == 1 || !c->pt_chain || R < 2 || choose_lm(c, b, &whole) != 3) return CSMC_OK;
+    const HostLattice &L = *c->lats[b->lat[0]];
+    if (pt.p.overrelaxation_rate > 64) return CSMC_OK;  // the halo would hold the whole system
+    const int nor = pt.p.overrelaxation_rate != 0 ? (int)pt.dosweep : 0;
+    const int gmax = c->split > 1 ? c->split : std::min(SPLIT_MAX, c->ncu / R);
+    SplitPlan best;
+    for (int G = c->split > 1 ? c->split : 2; G <= gmax; ++G) {
+        SplitPlan plan;
+        if (!build_split(L, G, nor, REG_THREADS, plan)) continue;
+        if (RED_BYTES + split_lds_bytes(plan) > (size_t)c->max_lds) continue;
+        if (reg_spt(plan.maxslots, b->k2c) == 0) continue;
+        if (c->split == 0 && plan.work_max * 4 > plan.work_full * 3) continue;
+        if (best.G == 1 || plan.work_max < best.work_max) best = std::move(plan);
+    }
+    if (best.G == 1) return CSMC_OK;
+    const int G = best.G, ncol = best.ncol, K2 = best.K2;
+    std::vector<int32_t> tab((size_t)G * SPLIT_HDR, 0);
+    auto put = [&](const std::vector<int> &v) {
+        const int32_t off = (int32_t)tab.size();
+        tab.insert(tab.end(), v.begin(), v.end());
+        return off;
+    };
+    for (int g = 0; g < G; ++g) {
+        const SplitPart &sp = best.parts[g];
+        const size_t h = (size_t)g * SPLIT_HDR;
+        tab[h + 0] = sp.n_loc;
+        tab[h + 1] = sp.n_own;
+        tab[h + 2] = (int32_t)sp.slots.size();
+        tab[h + 3] = (int32_t)sp.bnd.size();
+        tab[h + 4] = (int32_t)sp.halo.size();
+        tab[h + 5] = put(sp.col_start);
+        tab[h + 6] = put(sp.own_cnt);
+        tab[h + 7] = put(sp.phase_cnt);
+        while (tab.size() % 4) tab.push_back(0);
+        tab[h + 8] = (int32_t)tab.size();
+        for (auto &q : sp.slots) tab.insert(tab.end(), q.begin(), q.end());
+        std::vector<int> cs = sp.colslot;
+        cs.insert(cs.end(), best.colbasis.begin(), best.colbasis.end());
+        tab[h + 9] = put(cs);
+        tab[h + 10] = put(sp.loc_dev);
+        std::vector<int> ref(sp.n_loc);
+        for (int i = 0; i < sp.n_loc; ++i) ref[i] = L.ref_of_dev[sp.loc_dev[i]];
+        tab[h + 11] = put(ref);
+        tab[h + 12] = put(sp.nbl);
+        tab[h + 13] = put(sp.bnd);
+        std::vector<int> hp;
+        for (size_t j = 0; j < sp.halo.size(); ++j) {
+            hp.push_back(sp.halo[j]);
+            hp.push_back(sp.halo_src[j]);
+        }
+        while (tab.size() % 2) tab.push_back(0);
+        tab[h + 14] = put(hp);
+    }
+    (void)ncol;
+    (void)K2;
+    const size_t nhx = 2 * (size_t)R * G * 3 * best.maxbnd, npx = 2 * (size_t)R * SPLIT_MAX * 2 * SPLIT_NV;
+    HIPCHK(pt.split_tab.alloc(tab.size()));
+    HIPCHK(pt.split_hx.alloc(nhx));
+    HIPCHK(pt.split_px.alloc(npx));
+    HIPCHK(pt.split_epoch.alloc(R));
+    HIPCHK(hipMemcpyAsync(pt.split_tab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, c->stream));
+    // flags 0xFFFFFFFF: no epoch a run reaches
+    HIPCHK(hipMemsetAsync(pt.split_hx.p, 0xFF, nhx * 8, c->stream));
+    HIPCHK(hipMemsetAsync(pt.split_px.p, 0xFF, npx * 8, c->stream));
+    HIPCHK(hipMemsetAsync(pt.split_epoch.p, 0, R * 4, c->stream));
+    pt.split_G = G;
+    pt.split_nor = nor;
+    pt.split_spt = reg_spt(best.maxslots, b->k2c);
+    pt.split_bmax = best.maxbnd;
+    pt.split_lds = split_lds_bytes(best);
+    return CSMC_OK;
+}
+
 WGArgs wg_base(csmc_ctx *c, Batch *b, uint64_t seed) {
     WGArgs a{};
     a.P = pools(c, b);
@@ -409,7 +508,31 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, c->stream));
     }
-    if (lm == 3) {
+    bool split_done = false;
+    if (lm == 3 && a.mode == WG_PT_CHAIN && b->pt.split_G > 1) {
+        PTHost &pt = b->pt;
+        bool diag = true;
+        for (int id : b->lat) diag = diag && (c->h_lat[id].flags & LAT_DIAG);
+        WGArgs s = a;
+        s.split.G = pt.split_G;
+        s.split.nor = pt.split_nor;
+        s.split.R = b->nsys;
+        s.split.bmax = pt.split_bmax;
+        s.split.tab = pt.split_tab.p;
+        s.split.hx = pt.split_hx.p;
+        s.split.px = pt.split_px.p;
+        s.split.epoch = pt.split_epoch.p;
+        hipError_t e = launch_wg_split(b->k2c, pt.split_spt, diag, pt.split_G, b->nsys, pt.split_lds, s,
+                                       c->stream);
+        if (e == hipSuccess) {
+            split_done = true;
+        } else {
+            (void)hipGetLastError();  // not co-resident: the whole-system kernel from now on
+            pt.split_G = 1;
+        }
+    }
+    if (split_done) {
+    } else if (lm == 3) {
         bool diag = true;
         for (int id : b->lat) diag = diag && (c->h_lat[id].flags & LAT_DIAG);
         HIPCHK(launch_wg_reg(b->k2c, reg_spt(b->nslots, b->k2c), diag, b->nsys, lds_bytes, a, c->stream));
@@ -462,6 +585,13 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
     if (sp && sp[0] == '0') c->specialise = false;
     const char *ch = getenv("CSMC_PT_CHAIN");
     if (ch && ch[0] == '0') c->pt_chain = false;
+    const char *sl = getenv("CSMC_SPLIT");
+    if (sl && sl[0] >= '0' && sl[0] <= '8' && sl[1] == 0) c->split = sl[0] == '0' ? 1 : sl[0] - '0';
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess)
+            c->ncu = ncu;
+    }
     const char *vb = getenv("CSMC_VERBOSE");
     if (vb && vb[0] == '1') {
         hipDeviceProp_t prop;
@@ -536,6 +666,34 @@ int32_t csmc_build_tables(const csmc_unit_cell *uc, const int32_t *shape, int32_
     export_tables(L, pos, field, onsite, bsite, bmat, csite, ctens, qsite, qtens);
     if (colour)
         for (long i = 0; i < L.N; ++i) colour[i] = L.colour[i];
+    return CSMC_OK;
+}
+
+int32_t csmc_split_plan(const csmc_unit_cell *uc, const int32_t *shape, int32_t periodic,
+                        int32_t parts, int32_t or_sweeps, int64_t *stats) {
+    if (!stats || parts < 1 || parts > SPLIT_MAX || or_sweeps < 0)
+        return fail(CSMC_EINVAL, "split plan: parts in 1..8, or_sweeps >= 0");
+    HostLattice L;
+    std::string err;
+    int rc = build_lattice(uc, shape, periodic, 0.5, L, err);
+    if (rc) return fail(rc, err);
+    SplitPlan plan;
+    if (!build_split(L, parts, or_sweeps, REG_THREADS, plan))
+        return fail(CSMC_EINVAL, "the lattice cannot be split into these parts");
+    stats[0] = plan.G;
+    stats[1] = plan.P;
+    stats[2] = plan.maxloc;
+    stats[3] = plan.maxslots;
+    stats[4] = plan.work_max;
+    stats[5] = plan.work_full;
+    stats[6] = check_split(L, plan);
+    for (int g = 0; g < plan.G; ++g) {
+        const SplitPart &sp = plan.parts[g];
+        stats[7 + 4 * g] = sp.n_loc;
+        stats[8 + 4 * g] = sp.n_own;
+        stats[9 + 4 * g] = (int64_t)sp.halo.size();
+        stats[10 + 4 * g] = (int64_t)sp.bnd.size();
+    }
     return CSMC_OK;
 }
 
@@ -928,11 +1086,9 @@ int32_t csmc_pt_init(csmc_ctx *c, int32_t id, const double *temps, const csmc_pt
     HIPCHK(pt.slot_of_sys.alloc(R));
     HIPCHK(pt.sys_of_slot.alloc(R));
     HIPCHK(pt.done.alloc(1));
-    HIPCHK(pt.E2.alloc(2 * (size_t)R));
-    HIPCHK(pt.sigma2.alloc(2 * (size_t)R));
-    HIPCHK(pt.slot_of_sys2.alloc(2 * (size_t)R));
-    HIPCHK(pt.sys_of_slot2.alloc(2 * (size_t)R));
-    HIPCHK(pt.bar.alloc(1));
+    HIPCHK(pt.rec.alloc((size_t)PT_RECBUF * R * 4));
+    // flags 0xFFFFFFFF: no exchange index a run reaches
+    HIPCHK(hipMemsetAsync(pt.rec.p, 0xFF, (size_t)PT_RECBUF * R * 4 * 8, c->stream));
     HIPCHK(pt.err.alloc(1));
     HIPCHK(hipMemsetAsync(pt.err.p, 0, 4, c->stream));
     HIPCHK(pt.series.alloc((size_t)R * PROBE_CAP * 2));
@@ -951,8 +1107,18 @@ int32_t csmc_pt_init(csmc_ctx *c, int32_t id, const double *temps, const csmc_pt
     HIPCHK(hipMemsetAsync(pt.done.p, 0, 4, c->stream));
     // E = total_energy(mc.lattice) (monte_carlo.jl:265)
     HIPCHK(launch_energy(b->prec, pools(c, b), b->spins.p, R, pt.E.p, nullptr, c->stream));
+    rc = setup_split(c, b);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     pt.init = true;
+    return CSMC_OK;
+}
+
+int32_t csmc_pt_split_parts(csmc_ctx *c, int32_t id, int32_t *parts) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!b->pt.init) return fail(CSMC_ESTATE, "csmc_pt_init was not called for this batch");
+    if (parts) *parts = b->pt.split_G;
     return CSMC_OK;
 }
 
@@ -993,11 +1159,7 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
     a.pt.acc_exch = pt.acc_exch.p;
     a.pt.binners = pt.bins.p;
     a.pt.done = pt.done.p;
-    a.pt.E2 = pt.E2.p;
-    a.pt.sigma2 = pt.sigma2.p;
-    a.pt.slot_of_sys2 = pt.slot_of_sys2.p;
-    a.pt.sys_of_slot2 = pt.sys_of_slot2.p;
-    a.pt.bar = pt.bar.p;
+    a.pt.rec = pt.rec.p;
     a.pt.err = pt.err.p;
     a.spin_limit = 1u << 26;
     a.pt.series = pt.series.p;
@@ -1020,8 +1182,9 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
         return CSMC_OK;
     };
     // Consecutive segments that end with an exchange (and no host event) and
-    // are evenly spaced run as one chain: one cooperative launch, a grid
-    // barrier per exchange, the spins resident on chip throughout.
+    // are evenly spaced run as one chain: one cooperative launch, each
+    // exchange decided pair by pair from published records, the spins
+    // resident on chip throughout.
     const int chain_max = 1024;
     int nchain = 0;
     int64_t chain_begin = 0, chain_e0 = 0, chain_P = 0, chain_probes = 0;
@@ -1062,7 +1225,6 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
             rc2 = reserve(chain_probes);
             if (rc2) return rc2;
             ca.probe_base = pt.probe_pos;
-            HIPCHK(hipMemsetAsync(pt.bar.p, 0, 4, c->stream));
             rc2 = run_wg(c, b, ca);
             if (rc2 == CSMC_OK) {
                 done_chain = chained = true;
@@ -1169,7 +1331,7 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
     if (chained) {
         uint32_t err = 0;
         HIPCHK(hipMemcpy(&err, pt.err.p, 4, hipMemcpyDeviceToHost));
-        if (err) return fail(CSMC_EHIP, "parallel tempering: a grid barrier wait timed out");
+        if (err) return fail(CSMC_EHIP, "parallel tempering: a wait for another workgroup timed out");
     }
     if (stopped) *stopped = pt.sweep;
     if (event) *event = (pt.sweep >= pt.total && !pt.mid) ? 3 : ev;

@@ -53,14 +53,35 @@ struct LatDev {
 };
 
 // LDS scratch at the start of every persistent sweep kernel's dynamic LDS:
-// the workgroup reductions (16 waves x 8 doubles) then a few flags.
-constexpr int RED_BYTES = 2048;
-constexpr int RED_FLAG = 250;   // double index of the exchange flag
-constexpr int RED_XCHG = 248;   // double index of the chain-exchange scratch (2)
+// the workgroup reductions (block_sum of up to 9 values: 17 x 9 doubles,
+// doubles 0..152), the split kernel's partial sums (RED_PART..RED_TOT + 8),
+// then a few flags.
+constexpr int RED_BYTES = 4096;
+constexpr int RED_XCHG = 256;   // double index of the chain-exchange scratch (2)
+constexpr int RED_FLAG = 258;   // double index of the exchange flag
 
 // Workgroup size of the register-table sweep kernel; the slot schedule
 // (one slot = REG_THREADS consecutive device sites of one run) is built for it.
 constexpr int REG_THREADS = 1024;
+
+// Split form of the register-table PT kernel: at most SPLIT_MAX workgroups
+// per system.  LDS doubles RED_PART.. hold the parts' window partial sums
+// (SPLIT_MAX x SPLIT_NV), RED_OWN.. this part's before they are published,
+// RED_TOT.. their sum.
+//
+// Split tables (i32, one block per part, offsets from the table start):
+// header [SPLIT_HDR] = n_loc, n_own, nslots, n_bnd, n_halo, then the offsets
+// of col_start[ncol + 1], own_cnt[ncol], phase_cnt[P], slots (int4 {first,
+// colour end, basis, colour}, 4-int aligned), colslot[ncol + 1] + colbasis
+// [ncol], loc_dev[n_loc], loc_ref[n_loc], nbl[K2][n_loc], bnd[n_bnd] (owned
+// sites another part holds), halo[n_halo][2] (local site, then where its
+// owner publishes it: owner * 3 * maxbnd + position in the owner's bnd).
+constexpr int SPLIT_MAX = 8;
+constexpr int SPLIT_NV = 9;     // accepted, energy parts [5], summed spin [3]
+constexpr int RED_PART = 160;
+constexpr int RED_OWN = 232;
+constexpr int RED_TOT = 241;    // the summed window totals
+constexpr int SPLIT_HDR = 16;   // header ints per part in the split tables
 
 struct SysDev {
     int32_t lat;      // lattice id
@@ -104,14 +125,12 @@ struct PTDev {
     double *acc_exch;      // [R] per slot
     double *binners;       // [R][2][BIN_DOUBLES] per slot: E then |M|
     uint32_t *done;        // [1] WG completion counter for the exchange
-    // chained segments (one cooperative launch, a grid barrier per exchange):
-    // state published at an exchange, double-buffered by exchange parity
-    double *E2;            // [2][R] per system
-    double *sigma2;        // [2][R] per slot
-    int32_t *slot_of_sys2; // [2][R]
-    int32_t *sys_of_slot2; // [2][R]
-    uint32_t *bar;         // [1] grid barrier arrivals (zeroed before a launch)
-    uint32_t *err;         // [1] non-zero: a barrier wait ran out (no result)
+    // chained segments (one cooperative launch, an exchange between each):
+    // at exchange x the system in slot r publishes rec[x % PT_RECBUF][r] =
+    // {E lo, E hi, sigma lo, sigma hi}, each word (x << 32 | payload), and
+    // reads its partner slot's record; no other synchronisation
+    uint64_t *rec;         // [PT_RECBUF][R][4]
+    uint32_t *err;         // [1] non-zero: a wait ran out (no result)
     // probe series: (E, |M|) of each slot per probe sweep, folded into the
     // binners off the sweep path; [R][series_cap][2]
     double *series;
@@ -119,6 +138,13 @@ struct PTDev {
     int32_t pad2_;
 };
 
+
+// records of the chained exchanges: a partner reads exchange x's record
+// while the slot's next writers are at most a few exchanges ahead (each
+// exchange waits on the partner slot, so neighbouring slots stay within two
+// exchanges of each other); a record overwritten before it was read would
+// carry another exchange index and its reader would time out, never read it
+constexpr int PT_RECBUF = 8;
 
 constexpr int BIN_LVLS = 32;
 // per level: count, sum[2], sum2[4], pend[2], has  -> 10 doubles

@@ -504,24 +504,73 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
-// sums NV values per thread over the workgroup; result valid in thread 0.
-// scratch: at least 16*NV doubles of LDS.
+// DPP row shift of a double (both halves): lane i of each row of 16 reads
+// lane i - k of its row, and 0 where that lane lies outside the row.
+template <int K>
+__device__ __forceinline__ double dpp_row_shr(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x110 + K, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x110 + K, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// inclusive prefix sum within each row of 16 lanes: lane 15 of a row holds
+// the row's sum, added in a fixed order
+__device__ __forceinline__ double row_sum16(double v) {
+    v += dpp_row_shr<1>(v);
+    v += dpp_row_shr<2>(v);
+    v += dpp_row_shr<4>(v);
+    v += dpp_row_shr<8>(v);
+    return v;
+}
+
+__device__ __forceinline__ double read_lane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sums NV <= 9 values per thread over the workgroup (every thread active, at
+// most 16 waves); the totals are valid in every thread on return.  scratch:
+// 153 doubles of LDS (per-wave sums in [0, 16 NV), totals in [144, 144 + NV)),
+// not touched by other code between calls.
+//
+// Within a wave: DPP prefix sums over each row of 16 lanes, then the four
+// row sums (read with readlane) added in row order -- VALU latency only, no
+// trip through the LDS permute unit.  Across waves: wave 0 loads the per-wave
+// sums, 16 waves of one value to a row, and sums each row the same way.
+// Every operation and its order is fixed, so the result is the same on every
+// run.  The totals lie past every call's per-wave sums and are rewritten
+// only after the next call's first barrier, so no barrier is needed after
+// they are read.
+constexpr int BLOCK_SUM_SCRATCH = 16 * 9 + 9;
+
 template <int NV>
 __device__ __forceinline__ void block_sum(double *v, double *scratch) {
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    static_assert(NV >= 1 && NV <= 9, "block_sum: at most 9 values");
+    constexpr int TOT = 16 * 9;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
-    __syncthreads();
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) scratch[wave * NV + k] = v[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < nw; ++w)
-#pragma unroll
-            for (int k = 0; k < NV; ++k) v[k] += scratch[w * NV + k];
+    for (int k = 0; k < NV; ++k) {
+        const double r = row_sum16(v[k]);
+        const double t = ((read_lane_d(r, 15) + read_lane_d(r, 31)) + read_lane_d(r, 47)) +
+                         read_lane_d(r, 63);
+        if (lane == 0) scratch[wave * NV + k] = t;
     }
     __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int r0 = 0; r0 < NV; r0 += 4) {
+            const int k = r0 + (lane >> 4), w = lane & 15;
+            double x = (k < NV && w < nw) ? scratch[w * NV + k] : 0.0;
+            x = row_sum16(x);
+            if (w == 15 && k < NV) scratch[TOT + k] = x;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = scratch[TOT + k];
 }
 
 }  // namespace csmc

@@ -18,9 +18,25 @@ struct DevPools {
 };
 
 // WG_PT runs one segment per launch (the last workgroup does the exchange);
-// WG_PT_CHAIN runs a list of segments in one cooperative launch with a grid
-// barrier at each exchange.
+// WG_PT_CHAIN runs a list of segments in one cooperative launch, each exchange
+// decided pair by pair from records the partner slots publish.
 enum WGMode { WG_SWEEPS = 0, WG_ANNEAL = 1, WG_PT = 2, WG_PT_CHAIN = 3 };
+
+// Split form of the chained PT kernel: workgroup b runs part b / R of
+// system b % R.  Parts swap strip edges and partial sums through words of
+// (epoch << 32 | payload), each written and read as one 64-bit atomic, so a
+// reader that sees the epoch it waits for also sees the payload.
+struct SplitDev {
+    int32_t G;            // parts per system (1: the whole-system kernel)
+    int32_t nor;          // overrelaxation sweeps per window
+    int32_t R;            // systems
+    int32_t bmax;         // most published (boundary) sites of a part
+    const int32_t *tab;   // split tables (csmc_internal.h)
+    uint64_t *hx;         // [2][R][G][3][bmax] published spin words (x, y, z
+                          // planes of each part), by epoch parity
+    uint64_t *px;         // [2][R][SPLIT_MAX][2 * SPLIT_NV] partial-sum words
+    uint32_t *epoch;      // [R] epoch of each system's next exchange
+};
 
 // Arguments of the persistent workgroup-per-system kernel.
 struct WGArgs {
@@ -51,8 +67,9 @@ struct WGArgs {
     int64_t chain_tail;  // >= 0: after the last exchange, run on to the end of
                          // sweep chain_tail without a further exchange
     int nseg;
-    uint32_t spin_limit;  // polls of a barrier wait before it gives up
+    uint32_t spin_limit;  // polls of a wait for another workgroup before it gives up
     int64_t probe_base;   // series index of this launch's first probe
+    SplitDev split;       // WG_PT_CHAIN with split.G > 1
 };
 
 // Arguments of the one-colour-pass kernel (plain / stream form).
@@ -89,6 +106,11 @@ hipError_t launch_wg(int prec, int lm, bool self, bool gen, int k2c, int nsys, s
 // matrix of the batch is diagonal; lds_bytes = 16 * (max sites + 1)
 hipError_t launch_wg_reg(int k2c, int spt, bool diag, int nsys, size_t lds_bytes, const WGArgs &a,
                          hipStream_t st);
+// split register-table chain (WG_PT_CHAIN only): G * R workgroups,
+// lds_bytes = 16 * (most local sites of a part + 1) + 4 * (most boundary
+// sites) + 8 * (most halo sites)
+hipError_t launch_wg_split(int k2c, int spt, bool diag, int G, int R, size_t lds_bytes,
+                           const WGArgs &a, hipStream_t st);
 hipError_t launch_pass(int prec, bool self, bool gen, int nsys, int max_colour_sites,
                        const PassArgs &a, hipStream_t st);
 hipError_t launch_site_eval(int prec, const DevPools &P, const void *spins, int sys, int N,

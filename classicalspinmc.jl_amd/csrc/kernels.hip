@@ -85,28 +85,25 @@ __device__ void wg_energy(const LatView<double> &Ld, const NB &nb, const SV &X, 
     }
     constexpr int NV = WITH_M ? 8 : 5;
     block_sum<NV>(p, red);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < NV; ++k) red[k] = p[k];
-    __syncthreads();
-    for (int k = 0; k < NV; ++k) out[k] = red[k];
-    __syncthreads();
+    for (int k = 0; k < NV; ++k) out[k] = p[k];
 }
 
+// summed spin of a system; every thread takes the same sites, in the same
+// order, as in wg_energy, so both give the same FP64 sums
 template <class R, class SV>
-__device__ void wg_magnet(int N, const SV &X, double *red, double out[3]) {
+__device__ void wg_magnet(const LatView<double> &Ld, const SV &X, double *red, double out[3]) {
     double m[3] = {0, 0, 0};
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        spin4<R> s = X.ld(i);
-        m[0] += s.x;
-        m[1] += s.y;
-        m[2] += s.z;
+    for (int r = Ld.colrun[0]; r < Ld.colrun[Ld.ncol]; ++r) {
+        const int4 run = Ld.runs[r];
+        for (int i = run.x + (int)threadIdx.x; i < run.y; i += (int)blockDim.x) {
+            spin4<R> s = X.ld(i);
+            m[0] += s.x;
+            m[1] += s.y;
+            m[2] += s.z;
+        }
     }
     block_sum<3>(m, red);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 3; ++k) red[k] = m[k];
-    __syncthreads();
-    for (int k = 0; k < 3; ++k) out[k] = red[k];
-    __syncthreads();
+    for (int k = 0; k < 3; ++k) out[k] = m[k];
 }
 
 template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
@@ -120,11 +117,7 @@ __device__ double wg_metro(const LatView<R> &L, const NB &nb, const SV &X, R T, 
     });
     double v[1] = {(double)acc};
     block_sum<1>(v, red);
-    if (threadIdx.x == 0) red[0] = v[0];
-    __syncthreads();
-    double tot = red[0];
-    __syncthreads();
-    return tot;
+    return v[0];
 }
 
 // The sweeper of the global / LDS-array storage modes: sweeps walk the runs
@@ -139,6 +132,7 @@ struct SweeperG {
     uint32_t stream, k0, k1;
 
     __device__ int n_sites() const { return L.N; }
+    __device__ bool leader() const { return true; }  // the only workgroup of its system
     __device__ void or_sweep() {
         wg_colour_sweep<R>(L, [&](int b, int i) { or_update<R, GEN, K2C>(L, nb, X, b, i); });
     }
@@ -152,7 +146,7 @@ struct SweeperG {
         if (with_m) wg_energy<R, GEN, true>(Ld, nb, X, red, p);
         else wg_energy<R, GEN, false>(Ld, nb, X, red, p);
     }
-    __device__ void magnet(double m[3]) { wg_magnet<R>(L.N, X, red, m); }
+    __device__ void magnet(double m[3]) { wg_magnet<R>(Ld, X, red, m); }
 };
 
 template <int MODE, class R, bool SELF, bool GEN, int K2C, class SV, class NB>
@@ -240,6 +234,16 @@ hipError_t allow_large_lds(const void *fn) {
 
 hipError_t launch_cooperative(const void *fn, int nblocks, int threads, size_t lds_bytes,
                               const WGArgs &a, hipStream_t st) {
+    // the kernel's waits need every workgroup resident: refuse here, from the
+    // occupancy the compiled kernel allows, before the runtime is asked
+    int per_cu = 0, dev = 0, ncu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds_bytes);
+    if (e != hipSuccess) return e;
+    e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    if ((int64_t)per_cu * ncu < nblocks) return hipErrorCooperativeLaunchTooLarge;
     void *args[] = {const_cast<WGArgs *>(&a)};
     return hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(threads), args, (unsigned)lds_bytes, st);
 }
@@ -399,7 +403,7 @@ hipError_t launch_site_eval(int prec, const DevPools &P, const void *spins, int 
 template <class R>
 __global__ __launch_bounds__(1024) void k_energy(DevPools P, const void *spins, double *E,
                                                  double *parts) {
-    __shared__ double red[16 * 8];
+    __shared__ double red[BLOCK_SUM_SCRATCH];
     const int sys = blockIdx.x;
     const SysDev sd = P.sys[sys];
     const LatDev ld = P.lats[sd.lat];
@@ -423,13 +427,14 @@ hipError_t launch_energy(int prec, const DevPools &P, const void *spins, int nsy
 
 template <class R>
 __global__ __launch_bounds__(1024) void k_magnet(DevPools P, const void *spins, double *M) {
-    __shared__ double red[16 * 8];
+    __shared__ double red[BLOCK_SUM_SCRATCH];
     const int sys = blockIdx.x;
     const SysDev sd = P.sys[sys];
     const LatDev ld = P.lats[sd.lat];
+    const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
     const SpinsAoS<R> S{const_cast<spin4<R> *>(reinterpret_cast<const spin4<R> *>(spins)) + sd.spin_off};
     double m[3];
-    wg_magnet<R>(ld.N, S, red, m);
+    wg_magnet<R>(L, S, red, m);
     if (threadIdx.x == 0)
         for (int k = 0; k < 3; ++k) M[sys * 3 + k] = m[k];
 }

@@ -10,78 +10,9 @@
 // write, with no neighbour-table traffic and no address arithmetic in the
 // sweep.  The schedule (wg_drive) and the per-site physics (device.h) are the
 // ones every other storage mode uses.
-#include "sweep.h"
+#include "reg.h"
 
 namespace csmc {
-
-// spins as spin4<float> records in LDS, addressed by their absolute LDS byte
-// address (so an access needs no base add)
-typedef float lds_f4 __attribute__((ext_vector_type(4)));
-using lds_byte = __attribute__((address_space(3))) unsigned char;
-using lds_vec = __attribute__((address_space(3))) lds_f4;
-
-__device__ __forceinline__ uint32_t lds_address(const void *p) {
-    return (uint32_t)(size_t)(const lds_byte *)p;
-}
-
-struct SpinsV4 {
-    __device__ __forceinline__ spin4<float> ld(uint32_t a) const {
-        const lds_f4 v = *(const lds_vec *)(size_t)a;
-        return spin4<float>{v.x, v.y, v.z, v.w};
-    }
-    __device__ __forceinline__ void st(uint32_t a, const spin4<float> &v) const {
-        *(lds_vec *)(size_t)a = lds_f4{v.x, v.y, v.z, v.w};
-    }
-};
-
-// the bilinear couplings of one basis, K2C slots, held in registers (the
-// values are wave-uniform, so they live in SGPRs); DIAG keeps m11, m22, m33
-template <class S, int K2C, bool DIAG>
-struct BilCoef {
-    static constexpr int W = DIAG ? 3 : 9;
-    S m[K2C][W];
-    __device__ __forceinline__ void load(const LatView<S> &L, int b) {
-#pragma unroll
-        for (int k = 0; k < K2C; ++k)
-#pragma unroll
-            for (int e = 0; e < W; ++e) m[k][e] = ldc(L.c2 + (b * L.K2P + k) * 9 + (DIAG ? 4 * e : e));
-    }
-    // the bilinear field, in the operation order of bilinear_field
-    template <class A>
-    __device__ __forceinline__ void field(const spin4<float> *sj, A F[3]) const {
-#pragma unroll
-        for (int k = 0; k < K2C; ++k) {
-            const A x = sj[k].x, y = sj[k].y, z = sj[k].z;
-            if constexpr (DIAG) {
-                if (k == 0) {
-                    F[0] = (A)m[k][0] * x;
-                    F[1] = (A)m[k][1] * y;
-                    F[2] = (A)m[k][2] * z;
-                } else {
-                    F[0] += (A)m[k][0] * x;
-                    F[1] += (A)m[k][1] * y;
-                    F[2] += (A)m[k][2] * z;
-                }
-            } else {
-                if (k == 0) {
-                    F[0] = (A)m[k][0] * x;
-                    F[1] = (A)m[k][3] * x;
-                    F[2] = (A)m[k][6] * x;
-                } else {
-                    F[0] += (A)m[k][0] * x;
-                    F[1] += (A)m[k][3] * x;
-                    F[2] += (A)m[k][6] * x;
-                }
-                F[0] += (A)m[k][1] * y;
-                F[1] += (A)m[k][4] * y;
-                F[2] += (A)m[k][7] * y;
-                F[0] += (A)m[k][2] * z;
-                F[1] += (A)m[k][5] * z;
-                F[2] += (A)m[k][8] * z;
-            }
-        }
-    }
-};
 
 template <int K2C, int SPT, bool DIAG>
 struct SweeperR {
@@ -128,6 +59,7 @@ struct SweeperR {
     }
 
     __device__ int n_sites() const { return L.N; }
+    __device__ bool leader() const { return true; }  // the only workgroup of its system
     __device__ bool owns(int q) const { return (valid >> q) & 1u; }
     __device__ int basis(int q) const { return ldc(slots + 4 * q + 2); }
     __device__ int site(int q) const { return (int)((own[q] - base) >> 4); }
@@ -221,11 +153,7 @@ struct SweeperR {
         });
         double w[1] = {(double)acc};
         block_sum<1>(w, red);
-        if (threadIdx.x == 0) red[0] = w[0];
-        __syncthreads();
-        double tot = red[0];
-        __syncthreads();
-        return tot;
+        return w[0];
     }
 
     // FP64 energy parts of the system (bilinear, cubic = 0, quartic = 0,
@@ -262,27 +190,24 @@ struct SweeperR {
         }
         constexpr int NV = WITH_M ? 8 : 5;
         block_sum<NV>(p, red);
-        if (threadIdx.x == 0)
-            for (int k = 0; k < NV; ++k) red[k] = p[k];
-        __syncthreads();
-        for (int k = 0; k < NV; ++k) out[k] = red[k];
-        __syncthreads();
+        for (int k = 0; k < NV; ++k) out[k] = p[k];
     }
 
+    // the summed spin over the same per-thread sites, in the same order, as
+    // energy_t, so both give the same FP64 sums
     __device__ void magnet(double out[3]) {
         double m[3] = {0, 0, 0};
-        for (int i = threadIdx.x; i < L.N; i += REG_THREADS) {
-            spin4<float> s = X.ld(base + 16u * (uint32_t)i);
-            m[0] += s.x;
-            m[1] += s.y;
-            m[2] += s.z;
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) {
+            if (owns(q)) {
+                const spin4<float> s = X.ld(own[q]);
+                m[0] += s.x;
+                m[1] += s.y;
+                m[2] += s.z;
+            }
         }
         block_sum<3>(m, red);
-        if (threadIdx.x == 0)
-            for (int k = 0; k < 3; ++k) red[k] = m[k];
-        __syncthreads();
-        for (int k = 0; k < 3; ++k) out[k] = red[k];
-        __syncthreads();
+        for (int k = 0; k < 3; ++k) out[k] = m[k];
     }
 };
 

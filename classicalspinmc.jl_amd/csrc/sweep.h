@@ -4,7 +4,7 @@
 // wg_drive runs one launch's worth of the reference's sweep loops on the
 // system the workgroup owns: n plain sweeps, the whole schedule of an
 // annealing temperature list, one segment of the parallel-tempering loop, or
-// a chain of segments with a grid barrier and the exchange after each.
+// a chain of segments with the pairwise exchange after each.
 // The per-site work comes from a "sweeper" object with the interface
 //   int  n_sites()
 //   void or_sweep(), det_sweep()               one coloured sweep
@@ -14,6 +14,9 @@
 //                              with_m, summed spin p[5..7] (workgroup sums)
 //   void magnet(double m[3])   FP64 summed spin
 //   double *red                RED_BYTES of LDS scratch
+//   bool leader()              this workgroup writes the system's shared
+//                              results (false for all but one part of a
+//                              system shared by several workgroups)
 // so each storage mode only provides its sweeps.
 #pragma once
 #include "device.h"
@@ -94,63 +97,89 @@ __device__ __forceinline__ void wg_stage(int n, GET &&get, PUT &&put) {
     }
 }
 
-// Grid-wide barrier of a cooperative launch: thread 0 of every workgroup
-// arrives with a release add on the arrival counter, polls it (relaxed) until
-// `target` workgroups have arrived, then fences with acquire.  A wait that
-// polls spin_limit times, or sees another wait give up, sets pt.err and
-// returns, so every wave always reaches the end of the kernel.
-__device__ __forceinline__ void grid_barrier(const WGArgs &a, uint32_t target) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(a.pt.bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t n = 0;
-        while (__hip_atomic_load(a.pt.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(a.pt.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
-            __builtin_amdgcn_s_sleep(2);
-            if (++n > a.spin_limit) {
-                __hip_atomic_store(a.pt.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    __syncthreads();
+// 64-bit words of (epoch << 32 | payload), each written and read as one
+// relaxed device-scope atomic: a reader that sees the epoch it waits for sees
+// that payload, with no fence or barrier between writer and reader.
+__device__ __forceinline__ void ll_put(uint64_t *p, uint32_t v, uint32_t e) {
+    __hip_atomic_store(p, ((uint64_t)e << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// The replica exchange (monte_carlo.jl:308-349) of a chained segment, decided
-// by the workgroup of each system for its own pair from the state every
-// workgroup published before the barrier (buffers [par]): the same pairs,
-// Philox draw and FP64 acceptance as the last-workgroup form, so both forms
-// give the same decisions.  On return slot and sigma are those of the
-// system's new temperature slot; the next buffers [par ^ 1] hold the new
-// slot maps.  scratch: 2 doubles of LDS outside the reduction area.
-__device__ __forceinline__ void chain_exchange(const WGArgs &a, int sys, int64_t se, int par,
-                                               int &slot, double &sigma, double *scratch) {
+__device__ __forceinline__ uint64_t ll_load(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the payload of word p once it carries epoch e (v: a first load of it); a
+// wait that polls `limit` times, or sees another wait give up, sets *err and
+// returns what it has, so every wave always reaches the end of the kernel
+__device__ __forceinline__ uint32_t ll_wait(const uint64_t *p, uint64_t v, uint32_t e,
+                                            uint32_t limit, uint32_t *err) {
+    uint32_t n = 0;
+    while ((uint32_t)(v >> 32) != e) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+        if (++n > limit) {
+            __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = ll_load(p);
+    }
+    return (uint32_t)v;
+}
+
+__device__ __forceinline__ void ll_put_d(uint64_t *p, double v, uint32_t e) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    ll_put(p, (uint32_t)b, e);
+    ll_put(p + 1, (uint32_t)(b >> 32), e);
+}
+
+__device__ __forceinline__ double ll_wait_d(const uint64_t *p, uint32_t e, uint32_t limit,
+                                            uint32_t *err) {
+    const uint64_t v0 = ll_load(p), v1 = ll_load(p + 1);
+    const uint32_t lo = ll_wait(p, v0, e, limit, err), hi = ll_wait(p + 1, v1, e, limit, err);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The replica exchange (monte_carlo.jl:308-349) of a chained segment: the
+// system in slot r publishes its energy and sigma for exchange xc = se / swap,
+// waits for its partner slot's record (pairs (r, r+1) from 0 for an even xc,
+// from 1 for an odd one) and decides its own pair: the same Philox draw and
+// FP64 acceptance as the last-workgroup form, and both systems of a pair
+// reach the same decision from the same two records.  Temperature labels
+// move, spins stay.  On return slot and sigma are those of the system's new
+// slot.  Only the leader workgroup of a system publishes and counts.
+// scratch: 2 doubles of LDS outside the reduction area.
+__device__ __forceinline__ void pair_exchange(const WGArgs &a, int64_t se, int &slot,
+                                              double &sigma, double E, double *scratch,
+                                              bool leader) {
     const PTDev &pt = a.pt;
     if (threadIdx.x == 0) {
-        const int R = pt.R, cur = par * R, nxt = (par ^ 1) * R;
-        const uint32_t xc = (uint32_t)(se / pt.swap);
+        const int R = pt.R, r = slot;
+        const uint32_t xc = (uint32_t)(se / pt.swap), lim = a.spin_limit >> 4;
+        uint64_t *rec = pt.rec + (int64_t)(xc % PT_RECBUF) * R * 4;
+        if (leader) {
+            ll_put_d(rec + 4 * r, E, xc);
+            ll_put_d(rec + 4 * r + 2, sigma, xc);
+        }
         const int r0 = (xc % 2 == 0) ? 0 : 1;
-        const int r = slot;
         int lo = -1;
         if (r >= r0) lo = ((r - r0) % 2 == 0) ? r : r - 1;
         int ns = r;
         double nsig = sigma;
         if (lo >= 0 && lo + 1 < R) {
-            const int hi = lo + 1;
-            const int s_lo = pt.sys_of_slot2[cur + lo], s_hi = pt.sys_of_slot2[cur + hi];
+            const int hi = lo + 1, p = (r == lo) ? hi : lo;
+            const double Ep = ll_wait_d(rec + 4 * p, xc, lim, pt.err);
+            const double sp = ll_wait_d(rec + 4 * p + 2, xc, lim, pt.err);
+            const double E_lo = (r == lo) ? E : Ep, E_hi = (r == lo) ? Ep : E;
             uint4 rr = philox4x32_10(make_uint4((uint32_t)lo, 0xFFFFFFFFu, xc, TAG_EXCH), a.k0, a.k1);
             const double u = u01d(rr.x);
             const double dbeta = 1.0 / pt.temps[hi] - 1.0 / pt.temps[lo];
-            const double pacc = fmin(1.0, exp(dbeta * (pt.E2[cur + s_hi] - pt.E2[cur + s_lo])));
+            const double pacc = fmin(1.0, exp(dbeta * (E_hi - E_lo)));
             if (u < pacc) {
-                ns = (r == lo) ? hi : lo;
-                nsig = pt.sigma2[cur + ns];
-                pt.acc_exch[r] += 1.0;
+                ns = p;
+                nsig = sp;
+                if (leader) __hip_atomic_fetch_add(pt.acc_exch + r, 1.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        pt.slot_of_sys2[nxt + sys] = ns;
-        pt.sys_of_slot2[nxt + ns] = sys;
         scratch[0] = (double)ns;
         scratch[1] = nsig;
     }
@@ -161,9 +190,9 @@ __device__ __forceinline__ void chain_exchange(const WGArgs &a, int sys, int64_t
 }
 
 template <int MODE, class SW>
-__device__ void wg_drive(const WGArgs &a, int sys, SW &sw) {
+__device__ __forceinline__ void wg_drive(const WGArgs &a, int sys, SW &sw) {
     uint32_t mctr = a.mcount[sys];
-    auto metro_sweep = [&](double T, int kind, double &sigma) {
+    auto metro_sweep = [&](double T, int kind, double &sigma) __attribute__((always_inline)) {
         double acc = sw.metro(T, kind, sigma, mctr);
         ++mctr;
         if (kind == CSMC_CONE_ADAPTIVE) sigma = adapt_sigma(sigma, acc, (double)sw.n_sites());
@@ -221,7 +250,7 @@ __device__ void wg_drive(const WGArgs &a, int sys, SW &sw) {
         // across an exchange)
         double m[3] = {0, 0, 0};
         bool m_now = false;
-        auto segment = [&](int64_t sb, int64_t se, bool bmid, bool emid) {
+        auto segment = [&](int64_t sb, int64_t se, bool bmid, bool emid) __attribute__((always_inline)) {
             double acc_local = 0.0;
             // swp % dosweep and swp % probe as running remainders (one 64-bit
             // remainder per segment instead of per sweep)
@@ -256,7 +285,7 @@ __device__ void wg_drive(const WGArgs &a, int sys, SW &sw) {
                         sw.magnet(m);
                         m_now = true;
                     }
-                    if (threadIdx.x == 0) {
+                    if (threadIdx.x == 0 && sw.leader()) {
                         double *q = pt.series + ((int64_t)slot * pt.series_cap + probe_idx) * 2;
                         q[0] = E;
                         q[1] = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
@@ -266,40 +295,33 @@ __device__ void wg_drive(const WGArgs &a, int sys, SW &sw) {
                 if (++r_do == pt.dosweep) r_do = 0;
                 if (++r_pr == pt.probe) r_pr = 0;
             }
-            if (threadIdx.x == 0) pt.acc_local[slot] += acc_local;
+            // another system may hold this slot in the next segment, with no
+            // ordering between the two but the exchange records: add atomically
+            if (threadIdx.x == 0 && sw.leader())
+                __hip_atomic_fetch_add(pt.acc_local + slot, acc_local, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
         };
         if constexpr (MODE == WG_PT) {
             segment(a.seg_begin, a.seg_end, a.begin_mid, a.end_mid);
-            if (threadIdx.x == 0) {
+            if (threadIdx.x == 0 && sw.leader()) {
                 pt.E[sys] = E;
                 pt.sigma[slot] = sigma;
             }
         } else {
-            // WG_PT_CHAIN: every segment ends with an exchange, decided by
-            // each workgroup for its own pair after a grid barrier
-            int par = 0;
-            if (threadIdx.x == 0) {
-                pt.slot_of_sys2[sys] = slot;
-                pt.sys_of_slot2[slot] = sys;
-            }
+            // WG_PT_CHAIN: every segment ends with an exchange, decided pair
+            // by pair from the partners' published records
             int64_t sb = a.seg_begin;
             bool bmid = a.begin_mid;
             for (int g = 0; g < a.nseg; ++g) {
                 const int64_t se = a.chain_e0 + (int64_t)g * a.chain_P;
                 segment(sb, se, bmid, true);
-                if (threadIdx.x == 0) {
-                    pt.E2[par * pt.R + sys] = E;
-                    pt.sigma2[par * pt.R + slot] = sigma;
-                }
-                grid_barrier(a, (uint32_t)(g + 1) * gridDim.x);
-                chain_exchange(a, sys, se, par, slot, sigma, sw.red + RED_XCHG);
+                pair_exchange(a, se, slot, sigma, E, sw.red + RED_XCHG, sw.leader());
                 T = pt.temps[slot];
-                par ^= 1;
                 sb = se;
                 bmid = true;
             }
             if (a.chain_tail >= 0) segment(sb, a.chain_tail, bmid, false);
-            if (threadIdx.x == 0) {
+            if (threadIdx.x == 0 && sw.leader()) {
                 pt.E[sys] = E;
                 pt.sigma[slot] = sigma;
                 pt.slot_of_sys[sys] = slot;
@@ -307,7 +329,7 @@ __device__ void wg_drive(const WGArgs &a, int sys, SW &sw) {
             }
         }
     }
-    if (threadIdx.x == 0) a.mcount[sys] = mctr;
+    if (threadIdx.x == 0 && sw.leader()) a.mcount[sys] = mctr;
 }
 
 // Replica exchange (monte_carlo.jl:308-349) at the end of a PT segment, done

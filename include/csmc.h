@@ -113,6 +113,19 @@ int32_t csmc_build_tables(const csmc_unit_cell *uc, const int32_t *shape, int32_
                           double S, csmc_lattice_info *info, double *pos, double *field,
                           double *onsite, int64_t *bsite, double *bmat, int64_t *csite,
                           double *ctens, int64_t *qsite, double *qtens, int32_t *colour);
+/* Plan of the split parallel-tempering kernel for a lattice: `parts`
+ * workgroups share one system, each owning a strip of cells along the first
+ * lattice dimension plus the halo one window (or_sweeps overrelaxation sweeps
+ * and a Metropolis sweep) needs.  No counterpart in the reference: a host-side
+ * inspection of how csmc_pt_advance would split a system.  stats receives
+ * [0] parts, [1] colour phases per window, [2] most local sites of a part,
+ * [3] most register slots of a part, [4] the largest part's wave updates per
+ * window, [5] the unsplit system's, [6] violations found by replaying the
+ * window with a validity bit per site (0 for a sound plan), then per part
+ * [7 + 4p] local sites, owned sites, halo sites, boundary sites.  stats must
+ * hold 7 + 4 * parts values.  CSMC_EINVAL if the lattice cannot be split so. */
+int32_t csmc_split_plan(const csmc_unit_cell *uc, const int32_t *shape, int32_t periodic,
+                        int32_t parts, int32_t or_sweeps, int64_t *stats);
 /* Lattice(shape, uc, S; bc) (lattice.jl:65-291): builds the per-site tables,
  * colours the interaction hypergraph and uploads device tables.
  * periodic: 1 = "periodic", 0 = "open", other = CSMC_EBC. */
@@ -211,6 +224,11 @@ int32_t csmc_pt_advance(csmc_ctx *ctx, int32_t batch_id, int64_t until, int64_t 
  * at that slot, local accepted count, exchange accepted count, sigma. */
 int32_t csmc_pt_state(csmc_ctx *ctx, int32_t batch_id, int32_t *system_of_slot, double *E,
                       double *accepted_local, double *accepted_exchange, double *sigma);
+/* Workgroups per system of the chained PT kernel chosen at csmc_pt_init:
+ * 1 = one workgroup per system; G > 1 = each system split into G strips with
+ * halos (the environment variable CSMC_SPLIT=0/1 disables it, 2..8 forces G).
+ * No counterpart in the reference (its replicas are MPI ranks). */
+int32_t csmc_pt_split_parts(csmc_ctx *ctx, int32_t batch_id, int32_t *parts);
 /* Final observables per slot (hdf5.jl:204-239 / observables.jl:36-63):
  * out[n_sys][8] = C, dC, chi, dchi, <E>, dE, <|M|>, d|M| (absolute errors). */
 int32_t csmc_pt_observables(csmc_ctx *ctx, int32_t batch_id, double *out);
