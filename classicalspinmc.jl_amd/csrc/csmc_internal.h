@@ -77,7 +77,7 @@ constexpr int REG_THREADS = 1024;
 // sites another part holds), halo[n_halo][2] (local site, then where its
 // owner publishes it: owner * 3 * maxbnd + position in the owner's bnd).
 constexpr int SPLIT_MAX = 8;
-constexpr int SPLIT_NV = 9;     // accepted, energy parts [5], summed spin [3]
+constexpr int SPLIT_NV = 5;     // accepted count, energy, summed spin [3]
 constexpr int RED_PART = 160;
 constexpr int RED_OWN = 232;
 constexpr int RED_TOT = 241;    // the summed window totals

@@ -106,16 +106,53 @@ __device__ void wg_magnet(const LatView<double> &Ld, const SV &X, double *red, d
     for (int k = 0; k < 3; ++k) out[k] = m[k];
 }
 
+// this thread's FP64 energy (the parts combined as hamiltonian.jl:131) and
+// summed spin over the same sites, in the same order, as wg_energy and
+// wg_magnet
+template <class R, bool GEN, class NB, class SV>
+__device__ void thread_energy(const LatView<double> &Ld, const NB &nb, const SV &X, double &e,
+                              double m[3]) {
+    double p[5] = {0, 0, 0, 0, 0};
+    m[0] = m[1] = m[2] = 0.0;
+    for (int r = Ld.colrun[0]; r < Ld.colrun[Ld.ncol]; ++r) {
+        int4 run = Ld.runs[r];
+        int b = __builtin_amdgcn_readfirstlane(run.z);
+        const double *h = Ld.field + b * 3;
+        for (int i = run.x + (int)threadIdx.x; i < run.y; i += (int)blockDim.x) {
+            spin4<double> s = to_d<R>(X.ld(i));
+            double F2[3], F3[3], F4[3];
+            field_parts<GEN>(Ld, nb, b, i, [&](int j) { return to_d<R>(X.ld(j)); }, F2, F3, F4);
+            p[0] += s.x * F2[0] + s.y * F2[1] + s.z * F2[2];
+            p[1] += s.x * F3[0] + s.y * F3[1] + s.z * F3[2];
+            p[2] += s.x * F4[0] + s.y * F4[1] + s.z * F4[2];
+            p[3] -= s.x * h[0] + s.y * h[1] + s.z * h[2];
+            p[4] += onsite_energy<double, double>(Ld, b, s.x, s.y, s.z);
+            m[0] += s.x;
+            m[1] += s.y;
+            m[2] += s.z;
+        }
+    }
+    e = combine_energy(p);
+}
+
+// one Metropolis colour sweep; returns this thread's accepted count
 template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
-__device__ double wg_metro(const LatView<R> &L, const NB &nb, const SV &X, R T, int kind, R sigma,
-                           uint32_t k0, uint32_t k1, uint32_t stream, uint32_t ctr, double *red) {
+__device__ int wg_metro_local(const LatView<R> &L, const NB &nb, const SV &X, R T, int kind, R sigma,
+                              uint32_t k0, uint32_t k1, uint32_t stream, uint32_t ctr) {
     int acc = 0;
     const R invT = R(1) / T;
     wg_colour_sweep<R>(L, [&](int b, int i) {
         acc += metro_update<R, SELF, GEN, K2C>(L, nb, X, b, i, T, invT, kind, sigma, k0, k1, stream,
                                                ctr);
     });
-    double v[1] = {(double)acc};
+    return acc;
+}
+
+template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
+__device__ double wg_metro(const LatView<R> &L, const NB &nb, const SV &X, R T, int kind, R sigma,
+                           uint32_t k0, uint32_t k1, uint32_t stream, uint32_t ctr, double *red) {
+    double v[1] = {(double)wg_metro_local<R, SELF, GEN, K2C>(L, nb, X, T, kind, sigma, k0, k1, stream,
+                                                             ctr)};
     block_sum<1>(v, red);
     return v[0];
 }
@@ -142,9 +179,25 @@ struct SweeperG {
     __device__ double metro(double T, int kind, double sigma, uint32_t ctr) {
         return wg_metro<R, SELF, GEN, K2C>(L, nb, X, (R)T, kind, (R)sigma, k0, k1, stream, ctr, red);
     }
-    __device__ void energy(double p[8], bool with_m) {
-        if (with_m) wg_energy<R, GEN, true>(Ld, nb, X, red, p);
-        else wg_energy<R, GEN, false>(Ld, nb, X, red, p);
+    __device__ double metro_energy(double T, int kind, double sigma, uint32_t ctr, bool with_m,
+                                   double &E, double m[3]) {
+        const int acc = wg_metro_local<R, SELF, GEN, K2C>(L, nb, X, (R)T, kind, (R)sigma, k0, k1,
+                                                          stream, ctr);
+        double e, mm[3];
+        thread_energy<R, GEN>(Ld, nb, X, e, mm);
+        if (with_m) {
+            double v[5] = {(double)acc, e, mm[0], mm[1], mm[2]};
+            block_sum<5>(v, red);
+            E = v[1];
+            m[0] = v[2];
+            m[1] = v[3];
+            m[2] = v[4];
+            return v[0];
+        }
+        double v[2] = {(double)acc, e};
+        block_sum<2>(v, red);
+        E = v[1];
+        return v[0];
     }
     __device__ void magnet(double m[3]) { wg_magnet<R>(Ld, X, red, m); }
 };

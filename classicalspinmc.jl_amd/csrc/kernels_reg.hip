@@ -138,6 +138,13 @@ struct SweeperR {
     }
 
     __device__ double metro(double T, int kind, double sigma, uint32_t ctr) {
+        double w[1] = {(double)metro_local(T, kind, sigma, ctr)};
+        block_sum<1>(w, red);
+        return w[0];
+    }
+
+    // one Metropolis sweep; returns this thread's accepted count
+    __device__ int metro_local(double T, int kind, double sigma, uint32_t ctr) {
         const float Tf = (float)T, invT = 1.0f / Tf, sg = (float)sigma;
         int acc = 0;
         colour_sweep([&](int q, int b, const Site &v, const BilCoef<float, K2C, DIAG> &M) {
@@ -151,22 +158,14 @@ struct SweeperR {
             metro_commit<float>(L, X, own[q], a, v.s, n);
             acc += a;
         });
-        double w[1] = {(double)acc};
-        block_sum<1>(w, red);
-        return w[0];
+        return acc;
     }
 
-    // FP64 energy parts of the system (bilinear, cubic = 0, quartic = 0,
-    // Zeeman, onsite) in out[0..4] and its summed spin in out[5..7], valid in
-    // every thread
-    __device__ void energy(double out[8], bool with_m) {
-        if (with_m) energy_t<true>(out);
-        else energy_t<false>(out);
-    }
-
-    template <bool WITH_M>
-    __device__ void energy_t(double out[8]) {
-        double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // this thread's FP64 energy (bilinear / 2 + Zeeman + onsite: the lattice
+    // has no cubic or quartic terms) and summed spin
+    __device__ void thread_energy(double &e, double m[3]) {
+        double bil = 0.0, zee = 0.0, ons = 0.0;
+        m[0] = m[1] = m[2] = 0.0;
 #pragma unroll
         for (int q = 0; q < SPT; ++q) {
             if (owns(q)) {
@@ -178,23 +177,39 @@ struct SweeperR {
                 double F[3];
                 M.field(v.sj, F);
                 const double *h = Ld.field + b * 3;
-                p[0] += sx * F[0] + sy * F[1] + sz * F[2];
-                p[3] -= sx * ldc(h) + sy * ldc(h + 1) + sz * ldc(h + 2);
-                p[4] += onsite_energy<double, double>(Ld, b, sx, sy, sz);
-                if (WITH_M) {
-                    p[5] += sx;
-                    p[6] += sy;
-                    p[7] += sz;
-                }
+                bil += sx * F[0] + sy * F[1] + sz * F[2];
+                zee -= sx * ldc(h) + sy * ldc(h + 1) + sz * ldc(h + 2);
+                ons += onsite_energy<double, double>(Ld, b, sx, sy, sz);
+                m[0] += sx;
+                m[1] += sy;
+                m[2] += sz;
             }
         }
-        constexpr int NV = WITH_M ? 8 : 5;
-        block_sum<NV>(p, red);
-        for (int k = 0; k < NV; ++k) out[k] = p[k];
+        e = bil / 2 + zee + ons;
+    }
+
+    __device__ double metro_energy(double T, int kind, double sigma, uint32_t ctr, bool with_m,
+                                   double &E, double m[3]) {
+        const int acc = metro_local(T, kind, sigma, ctr);
+        double e, mm[3];
+        thread_energy(e, mm);
+        if (with_m) {
+            double v[5] = {(double)acc, e, mm[0], mm[1], mm[2]};
+            block_sum<5>(v, red);
+            E = v[1];
+            m[0] = v[2];
+            m[1] = v[3];
+            m[2] = v[4];
+            return v[0];
+        }
+        double v[2] = {(double)acc, e};
+        block_sum<2>(v, red);
+        E = v[1];
+        return v[0];
     }
 
     // the summed spin over the same per-thread sites, in the same order, as
-    // energy_t, so both give the same FP64 sums
+    // thread_energy, so both give the same FP64 sums
     __device__ void magnet(double out[3]) {
         double m[3] = {0, 0, 0};
 #pragma unroll

@@ -154,9 +154,12 @@ struct SweeperS {
 
     __device__ void det_sweep() {}
 
-    // The Metropolis sweep that ends the window, then the window exchange;
-    // returns the accepted count of the whole system (all threads).
-    __device__ double metro(double T, int kind, double sigma, uint32_t ctr) {
+    __device__ double metro(double, int, double, uint32_t) { return 0.0; }  // PT only
+
+    // The Metropolis sweep that ends the window, then the window exchange,
+    // which also sums the accepted count, energy and spin of the system.
+    __device__ double metro_energy(double T, int kind, double sigma, uint32_t ctr, bool with_m,
+                                   double &E, double m[3]) {
         const float Tf = (float)T, invT = 1.0f / Tf, sg = (float)sigma;
         int acc = 0;
         const uint32_t stream = (uint32_t)a.P.sys[sys].stream;
@@ -177,14 +180,10 @@ struct SweeperS {
         p[0] = (double)acc;
         publish_own(p);
         exchange(true);
-        return red[RED_TOT];
-    }
-
-    // energy parts and summed spin of the whole system at the last window end
-    __device__ void energy(double out[8], bool with_m) {
-        for (int k = 0; k < 5; ++k) out[k] = red[RED_TOT + 1 + k];
+        E = red[RED_TOT + 1];
         if (with_m)
-            for (int k = 0; k < 3; ++k) out[5 + k] = red[RED_TOT + 6 + k];
+            for (int k = 0; k < 3; ++k) m[k] = red[RED_TOT + 2 + k];
+        return red[RED_TOT];
     }
 
     __device__ void magnet(double out[3]) {
@@ -192,14 +191,15 @@ struct SweeperS {
         own_sums(p, false);
         publish_own(p);
         exchange(false);
-        for (int k = 0; k < 3; ++k) out[k] = red[RED_TOT + 6 + k];
+        for (int k = 0; k < 3; ++k) out[k] = red[RED_TOT + 2 + k];
     }
 
-    // per-thread sums over this part's own sites: p[1..5] FP64 energy parts
-    // (bilinear, cubic = 0, quartic = 0, Zeeman, onsite) when with_e,
-    // p[6..8] summed spin; p[0] = 0
+    // per-thread sums over this part's own sites: p[1] the FP64 energy
+    // (bilinear / 2 + Zeeman + onsite) when with_e, p[2..4] summed spin;
+    // p[0] = 0
     __device__ void own_sums(double p[SPLIT_NV], bool with_e) {
         for (int k = 0; k < SPLIT_NV; ++k) p[k] = 0.0;
+        double bil = 0.0, zee = 0.0, ons = 0.0;
         for (int c = 0; c < ncol; ++c) {
             const int qb = ldc(colslot + c), qe = ldc(colslot + c + 1);
             const int b = ldc(colslot + ncol + 1 + c);
@@ -215,21 +215,22 @@ struct SweeperS {
                         double F[3];
                         M.field(v.sj, F);
                         const double *h = Ld.field + b * 3;
-                        p[1] += sx * F[0] + sy * F[1] + sz * F[2];
-                        p[4] -= sx * ldc(h) + sy * ldc(h + 1) + sz * ldc(h + 2);
-                        p[5] += onsite_energy<double, double>(Ld, b, sx, sy, sz);
-                        p[6] += sx;
-                        p[7] += sy;
-                        p[8] += sz;
+                        bil += sx * F[0] + sy * F[1] + sz * F[2];
+                        zee -= sx * ldc(h) + sy * ldc(h + 1) + sz * ldc(h + 2);
+                        ons += onsite_energy<double, double>(Ld, b, sx, sy, sz);
+                        p[2] += sx;
+                        p[3] += sy;
+                        p[4] += sz;
                     } else {
                         const spin4<float> s = X.ld(own[q]);
-                        p[6] += s.x;
-                        p[7] += s.y;
-                        p[8] += s.z;
+                        p[2] += s.x;
+                        p[3] += s.y;
+                        p[4] += s.z;
                     }
                 }
             }
         }
+        if (with_e) p[1] = bil / 2 + zee + ons;
     }
 
     // workgroup sums of p into RED_OWN

@@ -10,9 +10,12 @@
 //   void or_sweep(), det_sweep()               one coloured sweep
 //   double metro(T, kind, sigma, ctr)          one Metropolis sweep, returns
 //                                              the accepted count (all threads)
-//   void energy(double p[8], bool with_m)   FP64 energy parts p[0..4] and,
-//                              with_m, summed spin p[5..7] (workgroup sums)
-//   void magnet(double m[3])   FP64 summed spin
+//   double metro_energy(T, kind, sigma, ctr, with_m, E, m)
+//                              one Metropolis sweep, then the system's FP64
+//                              energy E and, with_m, summed spin m, all in
+//                              one workgroup reduction; returns the accepted
+//                              count (all threads)
+//   void magnet(double m[3])   FP64 summed spin, the same sums as metro_energy's
 //   double *red                RED_BYTES of LDS scratch
 //   bool leader()              this workgroup writes the system's shared
 //                              results (false for all but one part of a
@@ -265,18 +268,14 @@ __device__ __forceinline__ void wg_drive(const WGArgs &a, int sys, SW &sw) {
                         m_now = false;
                     }
                     if (r_do == 0) {
-                        acc_local += metro_sweep(T, pt.alg, sigma);
-                        // total_energy (monte_carlo.jl:305); the same pass
-                        // sums the spins when this sweep has a probe
-                        double p[8];
+                        // the Metropolis sweep and total_energy
+                        // (monte_carlo.jl:303-305); the energy pass also sums
+                        // the spins when this sweep has a probe
                         const bool with_m = probe_here;
-                        sw.energy(p, with_m);
-                        E = combine_energy(p);
-                        if (with_m) {
-                            m[0] = p[5];
-                            m[1] = p[6];
-                            m[2] = p[7];
-                        }
+                        const double acc = sw.metro_energy(T, pt.alg, sigma, mctr, with_m, E, m);
+                        ++mctr;
+                        if (pt.alg == CSMC_CONE_ADAPTIVE) sigma = adapt_sigma(sigma, acc, (double)sw.n_sites());
+                        acc_local += acc;
                         m_now = with_m;
                     }
                 }

@@ -1,6 +1,8 @@
 """Chained parallel tempering (one cooperative launch for many exchange
-periods, a grid barrier and per-pair exchange decisions at each exchange)
-against one launch per period: the same schedule, bit for bit."""
+periods, each exchange decided pair by pair from records the partner slots
+publish) against one launch per period (the last workgroup decides every
+pair): the same schedule -- spins, counters, slot maps and energies bit for
+bit, the binned observables to 1e-12."""
 import os
 
 import numpy as np
@@ -13,15 +15,19 @@ pytestmark = pytest.mark.gpu
 
 
 def _ctx(chain):
-    old = os.environ.get("CSMC_PT_CHAIN")
+    # one workgroup per system in both forms (the split form sums energies
+    # over its parts in another order; it has its own test)
+    old = {k: os.environ.get(k) for k in ("CSMC_PT_CHAIN", "CSMC_SPLIT")}
     os.environ["CSMC_PT_CHAIN"] = "1" if chain else "0"
+    os.environ["CSMC_SPLIT"] = "1"
     try:
         return csmc.Context(0)
     finally:
-        if old is None:
-            del os.environ["CSMC_PT_CHAIN"]
-        else:
-            os.environ["CSMC_PT_CHAIN"] = old
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
 
 
 CASES = [  # name, cell, shape, S, precision, replicas, alg, schedule
@@ -63,8 +69,11 @@ def test_chained_pt_matches_per_period_launches(case):
     assert e1 == e2
     for k in st1:
         assert np.array_equal(st1[k], st2[k]), k
+    # the binned observables come out of two different kernels, whose FP64
+    # |M| = sqrt(mx^2 + my^2 + mz^2) the compiler may contract differently:
+    # equal to the last few bits
     for k in ob1:
-        assert np.array_equal(ob1[k], ob2[k]), k
+        assert np.allclose(ob1[k], ob2[k], rtol=1e-12, atol=0.0), k
     for x, y in zip(sp1, sp2):
         assert np.array_equal(x, y)
     assert np.array_equal(c1, c2)
