@@ -153,7 +153,9 @@ def run_pt(args, W, csmc, uc, L, S, R, temps, precision, name):
     b.randomize(2)
     b.pt_init(temps, params, csmc.METROPOLIS, seed=2)
     mode = b.storage_mode()
-    log(f"[bench] {name}: lattice N={N} replicas={R} colours={lat.info.n_colours} storage mode {mode}")
+    parts = b.pt_split_parts()
+    log(f"[bench] {name}: lattice N={N} replicas={R} colours={lat.info.n_colours} storage mode {mode}, "
+        f"{parts} workgroup(s) per replica")
     b.pt_advance(args.warmup * sweeps_per_step)
     ctx.sync()
     log(f"[bench] warmup {args.warmup} steps done")
@@ -177,7 +179,7 @@ def run_pt(args, W, csmc, uc, L, S, R, temps, precision, name):
     s0 = b.get_spins(0)
     assert np.allclose(np.linalg.norm(s0, axis=1), S, atol=1e-4 * S)
     return dict(b=b, lat=lat, N=N, dt=dt, kern_ms=kern_ms, launches=launches, value=value,
-                updates_per_step=updates_per_step, mode=mode, ctx=ctx)
+                updates_per_step=updates_per_step, mode=mode, parts=parts, ctx=ctx)
 
 
 def main():
@@ -230,13 +232,19 @@ def main():
     avg_ms = kern_ms / launches
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_wg_reg_pt_honeycomb_L64_R64_traffic.json")
+    split = r["parts"] > 1
+    kname = "k_wg_split" if split else "k_wg_reg"
+    tpath = os.path.join(ROOT, "profiles", f"{kname}_pt_honeycomb_L64_R64_traffic.json")
     if L == 64 and R == 64 and os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f)["traffic_bytes_per_step"] * steps_per_launch
+    kdesc = (f"k_wg_split (register-table PT sweep kernel, each replica split over {r['parts']} "
+             "workgroups with halos, chained exchange periods)" if split else
+             "k_wg_reg (register-table PT sweep kernel, one workgroup per replica, chained "
+             "exchange periods)")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                "kernel": "k_wg_reg (register-table PT sweep kernel, chained exchange periods)",
+                "kernel": kdesc,
                 "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 4),
                 "steps_per_launch": round(steps_per_launch, 2),
                 "note": "spins stay in LDS for a whole launch (a chain of up to 1024 exchange "
@@ -252,7 +260,7 @@ def main():
                                "swap_rate=10, overrelaxation_rate=10, probe_rate=10",
                    "replicas": R, "sites": N, "updates_per_step": updates_per_step,
                    "precision": "fp32 spins, fp64 energies/exchange/binning",
-                   "storage_mode": r["mode"]},
+                   "storage_mode": r["mode"], "workgroups_per_replica": r["parts"]},
         "roofline": roofline,
     }
     r["b"].close()
