@@ -133,8 +133,9 @@ struct csmc_ctx {
     int lds_cap = 3;  // highest workgroup storage mode allowed
     bool specialise = true;  // use slot-count specialised kernels when they exist
     bool pt_chain = true;    // chain PT exchange periods in cooperative launches
-    int split = 0;           // parts per system of the split PT kernel: 0 chosen
-                             // from the plan's cost, 1 never split, 2..8 forced
+    int split = 1;           // parts per system of the split PT kernel: 1 never
+                             // split (default), 0 chosen from the plan's cost,
+                             // 2..8 forced
     int ncu = 0;             // compute units of the device
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -389,10 +390,13 @@ size_t split_lds_bytes(const SplitPlan &p) {
 }
 
 // Split form of the chained PT kernel (kernels_split.hip): plan parts for
-// the batch's lattice and upload the tables, or leave split_G = 1 when the
-// whole-system kernel is the better (or the only) choice.  Automatic choice:
-// the plan whose largest part does the least work, if that is at most 3/4 of
-// the whole system's and G * R workgroups fit the device at once.
+// the batch's lattice and upload the tables, or leave split_G = 1.  Off
+// unless asked for: measured on the honeycomb L=64 x 64-replica workload the
+// whole-system kernel is faster (each colour phase has a fixed latency the
+// split does not shorten, and the split adds halo updates and a window
+// exchange).  Automatic choice (CSMC_SPLIT=auto): the plan whose largest part
+// does the least work, if that is at most 3/4 of the whole system's and
+// G * R workgroups fit the device at once.
 int setup_split(csmc_ctx *c, Batch *b) {
     PTHost &pt = b->pt;
     pt.split_G = 1;
@@ -586,7 +590,8 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
     const char *ch = getenv("CSMC_PT_CHAIN");
     if (ch && ch[0] == '0') c->pt_chain = false;
     const char *sl = getenv("CSMC_SPLIT");
-    if (sl && sl[0] >= '0' && sl[0] <= '8' && sl[1] == 0) c->split = sl[0] == '0' ? 1 : sl[0] - '0';
+    if (sl && std::string(sl) == "auto") c->split = 0;
+    else if (sl && sl[0] >= '0' && sl[0] <= '8' && sl[1] == 0) c->split = sl[0] == '0' ? 1 : sl[0] - '0';
     {
         int ncu = 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess)

@@ -51,10 +51,10 @@ struct SweeperR {
             const bool ok = q < nslots && i < end;
             const int ic = ok ? i : first;
             valid |= ok ? (1u << q) : 0u;
-            own[q] = base + 16u * (uint32_t)(ok ? i : L.N);
+            own[q] = pin_address(base + 16u * (uint32_t)(ok ? i : L.N));
 #pragma unroll
             for (int k = 0; k < K2C; ++k)
-                nb[q][k] = base + 16u * (uint32_t)L.nbr2[(int64_t)k * L.N + ic];
+                nb[q][k] = pin_address(base + 16u * (uint32_t)L.nbr2[(int64_t)k * L.N + ic]);
         }
     }
 

@@ -81,9 +81,10 @@ struct SweeperS {
             const int i = first + (int)threadIdx.x;
             const bool ok = q < nslots && i < end;
             const int ic = ok ? i : first;
-            own[q] = base + 16u * (uint32_t)(ok ? i : n_loc);
+            own[q] = pin_address(base + 16u * (uint32_t)(ok ? i : n_loc));
 #pragma unroll
-            for (int k = 0; k < K2C; ++k) nb[q][k] = base + 16u * (uint32_t)nbl[(int64_t)k * n_loc + ic];
+            for (int k = 0; k < K2C; ++k)
+                nb[q][k] = pin_address(base + 16u * (uint32_t)nbl[(int64_t)k * n_loc + ic]);
         }
     }
 

@@ -16,6 +16,15 @@ __device__ __forceinline__ uint32_t lds_address(const void *p) {
     return (uint32_t)(size_t)(const lds_byte *)p;
 }
 
+// An LDS address held as one opaque VGPR value.  Without this the compiler
+// keeps base-relative offsets in the address tables and adds the (uniform)
+// base back before every LDS access, one VALU add per access; an empty asm
+// with a "+v" operand hides the sum, so the tables hold final addresses.
+__device__ __forceinline__ uint32_t pin_address(uint32_t a) {
+    asm volatile("" : "+v"(a));
+    return a;
+}
+
 struct SpinsV4 {
     __device__ __forceinline__ spin4<float> ld(uint32_t a) const {
         const lds_f4 v = *(const lds_vec *)(size_t)a;

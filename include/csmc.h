@@ -225,9 +225,10 @@ int32_t csmc_pt_advance(csmc_ctx *ctx, int32_t batch_id, int64_t until, int64_t 
 int32_t csmc_pt_state(csmc_ctx *ctx, int32_t batch_id, int32_t *system_of_slot, double *E,
                       double *accepted_local, double *accepted_exchange, double *sigma);
 /* Workgroups per system of the chained PT kernel chosen at csmc_pt_init:
- * 1 = one workgroup per system; G > 1 = each system split into G strips with
- * halos (the environment variable CSMC_SPLIT=0/1 disables it, 2..8 forces G).
- * No counterpart in the reference (its replicas are MPI ranks). */
+ * 1 = one workgroup per system (the default); G > 1 = each system split into
+ * G strips with halos, when the environment variable CSMC_SPLIT is "auto" (the
+ * planner decides) or 2..8 (forces G).  No counterpart in the reference (its
+ * replicas are MPI ranks). */
 int32_t csmc_pt_split_parts(csmc_ctx *ctx, int32_t batch_id, int32_t *parts);
 /* Final observables per slot (hdf5.jl:204-239 / observables.jl:36-63):
  * out[n_sys][8] = C, dC, chi, dchi, <E>, dE, <|M|>, d|M| (absolute errors). */

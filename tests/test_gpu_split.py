@@ -44,7 +44,7 @@ def sched(t_th, t_meas, probe, swap, orr, report=10 ** 9, ckpt=10 ** 9):
                 overrelaxation_rate=orr, report_interval=report, checkpoint_rate=ckpt)
 
 
-CASES = [  # name, cell, shape, S, replicas, parts (None: automatic), expected parts, alg, schedule
+CASES = [  # name, cell, shape, S, replicas, CSMC_SPLIT (None: unset), expected parts, alg, schedule
     ("honeycomb_64x8_G4_aligned", lambda: W.honeycomb_kitaev(1.0, -2.0), (64, 8), 0.5, 8, 4, 4,
      csmc.METROPOLIS, sched(40, 400, 10, 10, 10)),
     ("honeycomb_32x8_G3_unaligned_probes_adaptive", lambda: W.honeycomb_kitaev(1.0, -2.0, 0.3, -0.05),
@@ -55,8 +55,10 @@ CASES = [  # name, cell, shape, S, replicas, parts (None: automatic), expected p
      4, 3, 3, csmc.CONE, sched(10, 110, 2, 3, 1, report=37, ckpt=50)),
     ("pyrochlore_8x3x3_G2", pyrochlore_heisenberg, (8, 3, 3), 1.0, 4, 2, 2, csmc.METROPOLIS,
      sched(6, 80, 2, 4, 2)),
-    ("honeycomb_64x64_R64_auto", lambda: W.honeycomb_kitaev(1.0, -2.0), (64, 64), 0.5, 64, None, 4,
+    ("honeycomb_64x64_R64_auto", lambda: W.honeycomb_kitaev(1.0, -2.0), (64, 64), 0.5, 64, "auto", 4,
      csmc.METROPOLIS, sched(20, 200, 10, 10, 10)),
+    ("honeycomb_64x64_R64_default_whole", lambda: W.honeycomb_kitaev(1.0, -2.0), (64, 64), 0.5, 64, None, 1,
+     csmc.METROPOLIS, sched(10, 60, 10, 10, 10)),
     ("honeycomb_64x64_R64_G8_not_resident", lambda: W.honeycomb_kitaev(1.0, -2.0), (64, 64), 0.5, 64, 8,
      1, csmc.METROPOLIS, sched(10, 60, 10, 10, 10)),
 ]
@@ -91,7 +93,7 @@ def test_split_matches_whole_system_kernel(case):
     part = run(lat, R, 32, split, alg, p)
     assert whole["parts0"] == 1
     assert part["parts"] == parts, "split kernel not used"
-    if split is not None and parts == 1:
+    if isinstance(split, int) and parts == 1:
         assert part["parts0"] == split  # planned, refused at launch, fell back
     assert part["events"] == whole["events"]
     for x, y in zip(part["spins"], whole["spins"]):
