@@ -268,6 +268,14 @@ void append_lattice(csmc_ctx *c, const HostLattice &L) {
         if (cb < 0) d.flags &= ~LAT_COLBASIS;
         I.push_back(cb);
     }
+    if ((d.flags & LAT_COLBASIS) && L.ncol == 2) {
+        int size[2] = {0, 0};
+        for (int col = 0; col < 2; ++col)
+            for (int r = L.colrun[col]; r < L.colrun[col + 1]; ++r) size[col] += L.runs[r][1] - L.runs[r][0];
+        const int spc = size[0] / REG_THREADS;
+        if (size[0] == size[1] && size[0] % REG_THREADS == 0 && (spc == 1 || spc == 2 || spc == 4))
+            d.flags |= LAT_TWO_COLOUR;
+    }
     // incidence lists: participant -> (owner, kind, role, slot)
     std::vector<std::vector<Inc>> inc(N);
     for (int i = 0; i < N; ++i) {
@@ -537,9 +545,14 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
     }
     if (split_done) {
     } else if (lm == 3) {
-        bool diag = true;
-        for (int id : b->lat) diag = diag && (c->h_lat[id].flags & LAT_DIAG);
-        HIPCHK(launch_wg_reg(b->k2c, reg_spt(b->nslots, b->k2c), diag, b->nsys, lds_bytes, a, c->stream));
+        bool diag = true, two = true;
+        const int spt = reg_spt(b->nslots, b->k2c);
+        for (int id : b->lat) {
+            diag = diag && (c->h_lat[id].flags & LAT_DIAG);
+            // two full colour classes of spt / 2 slots each
+            two = two && (c->h_lat[id].flags & LAT_TWO_COLOUR) && c->h_lat[id].nslots == spt;
+        }
+        HIPCHK(launch_wg_reg(b->k2c, spt, diag, two, b->nsys, lds_bytes, a, c->stream));
     } else {
         HIPCHK(launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0, b->nsys,
                          lds_bytes, a, c->stream));
@@ -748,7 +761,7 @@ int32_t csmc_lattice_update_couplings(csmc_ctx *c, int32_t id, const csmc_unit_c
     if (rc) return fail(rc, err);
     std::vector<double> dc = device_coef(L);
     std::copy(dc.begin(), dc.end(), c->h_coef.begin() + c->h_lat[id].coef_off);
-    c->h_lat[id].flags = coupling_flags(L) | (c->h_lat[id].flags & LAT_COLBASIS);
+    c->h_lat[id].flags = coupling_flags(L) | (c->h_lat[id].flags & (LAT_COLBASIS | LAT_TWO_COLOUR));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->dirty = true;
     return CSMC_OK;

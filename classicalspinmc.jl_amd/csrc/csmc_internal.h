@@ -94,6 +94,9 @@ constexpr int32_t LAT_ONSITE = 1;
 constexpr int32_t LAT_FIELD = 2;
 constexpr int32_t LAT_DIAG = 4;  // every bilinear coupling matrix is diagonal
 constexpr int32_t LAT_COLBASIS = 8;  // every colour class holds sites of one basis
+// two colour classes, each of one basis and of the same whole number (1, 2
+// or 4) of REG_THREADS-site register-table slots
+constexpr int32_t LAT_TWO_COLOUR = 16;
 
 // incidence entry: owner device site and a packed (kind, slot, role)
 //   code = kind(2 bits: 0 bil, 1 cub, 2 qua) | role << 2 (2 bits) | slot << 4

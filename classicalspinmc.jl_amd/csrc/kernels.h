@@ -103,9 +103,11 @@ hipError_t launch_wg(int prec, int lm, bool self, bool gen, int k2c, int nsys, s
                      const WGArgs &a, hipStream_t st);
 // register-table mode (FP32, bilinear-only): k2c in {3, 4, 6}, spt in {4, 8}
 // register slots per thread (8 only with k2c <= 4), diag: every coupling
-// matrix of the batch is diagonal; lds_bytes = 16 * (max sites + 1)
-hipError_t launch_wg_reg(int k2c, int spt, bool diag, int nsys, size_t lds_bytes, const WGArgs &a,
-                         hipStream_t st);
+// matrix of the batch is diagonal; two_colour: every lattice has two colour
+// classes of spt / 2 full REG_THREADS-site slots each (LAT_TWO_COLOUR);
+// lds_bytes = 16 * (max sites + 1)
+hipError_t launch_wg_reg(int k2c, int spt, bool diag, bool two_colour, int nsys, size_t lds_bytes,
+                         const WGArgs &a, hipStream_t st);
 // split register-table chain (WG_PT_CHAIN only): G * R workgroups,
 // lds_bytes = 16 * (most local sites of a part + 1) + 4 * (most boundary
 // sites) + 8 * (most halo sites)

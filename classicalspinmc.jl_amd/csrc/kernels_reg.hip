@@ -14,7 +14,10 @@
 
 namespace csmc {
 
-template <int K2C, int SPT, bool DIAG>
+// SPC > 0: the lattice has two colour classes of SPC full REG_THREADS-site
+// slots each (SPT = 2 SPC), so every thread has a site in every slot and no
+// update needs an ownership mask; SPC = 0: any layout.
+template <int K2C, int SPT, bool DIAG, int SPC>
 struct SweeperR {
     const LatView<float> &L;
     const LatView<double> &Ld;
@@ -61,6 +64,8 @@ struct SweeperR {
     __device__ int n_sites() const { return L.N; }
     __device__ bool leader() const { return true; }  // the only workgroup of its system
     __device__ bool owns(int q) const { return (valid >> q) & 1u; }
+    // a sweep needs no ownership test when every slot is full
+    __device__ bool sweeps(int q) const { return SPC > 0 || owns(q); }
     __device__ int basis(int q) const { return ldc(slots + 4 * q + 2); }
     __device__ int site(int q) const { return (int)((own[q] - base) >> 4); }
 
@@ -98,12 +103,12 @@ struct SweeperR {
             const bool in1 = q + 1 < SPT && q + 1 >= qb && q + 1 < qe;
             if (in0 && in1) {
                 const Site v0 = load(q), v1 = load(q + 1);
-                if (owns(q)) f(q, v0);
-                if (owns(q + 1)) f(q + 1, v1);
+                if (sweeps(q)) f(q, v0);
+                if (sweeps(q + 1)) f(q + 1, v1);
             } else if (in0) {
-                if (owns(q)) f(q, load(q));
+                if (sweeps(q)) f(q, load(q));
             } else if (in1) {
-                if (owns(q + 1)) f(q + 1, load(q + 1));
+                if (sweeps(q + 1)) f(q + 1, load(q + 1));
             }
         }
     }
@@ -226,7 +231,7 @@ struct SweeperR {
     }
 };
 
-template <int K2C, int SPT, bool DIAG, int MODE>
+template <int K2C, int SPT, bool DIAG, int MODE, int SPC>
 __global__ __launch_bounds__(REG_THREADS) void k_wg_reg(WGArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     double *red = reinterpret_cast<double *>(smem);  // reduction scratch
@@ -245,7 +250,7 @@ __global__ __launch_bounds__(REG_THREADS) void k_wg_reg(WGArgs a) {
     // stage the system and its zero pad site N into LDS
     wg_stage<8>(N + 1, [&](int i) { return G[i]; },
                 [&](int i, const spin4<float> &v) { X.st(base + 16u * (uint32_t)i, v); });
-    SweeperR<K2C, SPT, DIAG> sw(L, Ld, base, red,
+    SweeperR<K2C, SPT, DIAG, SPC> sw(L, Ld, base, red,
                                 reinterpret_cast<const int32_t *>(a.P.runs + ld.slot_off),
                                 a.P.i32 + ld.colslot_off, ld.nslots, (uint32_t)sd.stream, a.k0,
                                 a.k1);
@@ -258,50 +263,56 @@ __global__ __launch_bounds__(REG_THREADS) void k_wg_reg(WGArgs a) {
     for (int i = threadIdx.x; i < N; i += REG_THREADS) G[i] = X.ld(base + 16u * (uint32_t)i);
 }
 
-template <int K2C, int SPT, bool DIAG, int MODE>
+template <int K2C, int SPT, bool DIAG, int MODE, int SPC>
 static hipError_t launch_reg_t(int nsys, size_t lds_bytes, const WGArgs &a, hipStream_t st) {
     const size_t bytes = RED_BYTES + lds_bytes;
-    const void *fn = (const void *)k_wg_reg<K2C, SPT, DIAG, MODE>;
+    const void *fn = (const void *)k_wg_reg<K2C, SPT, DIAG, MODE, SPC>;
     if (bytes > 65536) {
         hipError_t e = allow_large_lds(fn);
         if (e != hipSuccess) return e;
     }
     if constexpr (MODE == WG_PT_CHAIN) return launch_cooperative(fn, nsys, REG_THREADS, bytes, a, st);
-    hipLaunchKernelGGL((k_wg_reg<K2C, SPT, DIAG, MODE>), dim3(nsys), dim3(REG_THREADS), bytes, st, a);
+    hipLaunchKernelGGL((k_wg_reg<K2C, SPT, DIAG, MODE, SPC>), dim3(nsys), dim3(REG_THREADS), bytes, st, a);
     return hipGetLastError();
 }
 
-template <int K2C, int SPT, bool DIAG>
+template <int K2C, int SPT, bool DIAG, int SPC>
 static hipError_t launch_reg_m(int nsys, size_t lds_bytes, const WGArgs &a, hipStream_t st) {
-    if (a.mode == WG_PT) return launch_reg_t<K2C, SPT, DIAG, WG_PT>(nsys, lds_bytes, a, st);
-    if (a.mode == WG_PT_CHAIN) return launch_reg_t<K2C, SPT, DIAG, WG_PT_CHAIN>(nsys, lds_bytes, a, st);
-    if (a.mode == WG_ANNEAL) return launch_reg_t<K2C, SPT, DIAG, WG_ANNEAL>(nsys, lds_bytes, a, st);
-    return launch_reg_t<K2C, SPT, DIAG, WG_SWEEPS>(nsys, lds_bytes, a, st);
+    if (a.mode == WG_PT) return launch_reg_t<K2C, SPT, DIAG, WG_PT, SPC>(nsys, lds_bytes, a, st);
+    if (a.mode == WG_PT_CHAIN)
+        return launch_reg_t<K2C, SPT, DIAG, WG_PT_CHAIN, SPC>(nsys, lds_bytes, a, st);
+    if (a.mode == WG_ANNEAL) return launch_reg_t<K2C, SPT, DIAG, WG_ANNEAL, SPC>(nsys, lds_bytes, a, st);
+    return launch_reg_t<K2C, SPT, DIAG, WG_SWEEPS, SPC>(nsys, lds_bytes, a, st);
 }
 
 template <int K2C, int SPT>
-static hipError_t launch_reg_d(bool diag, int nsys, size_t lds_bytes, const WGArgs &a,
-                               hipStream_t st) {
-    return diag ? launch_reg_m<K2C, SPT, true>(nsys, lds_bytes, a, st)
-                : launch_reg_m<K2C, SPT, false>(nsys, lds_bytes, a, st);
+static hipError_t launch_reg_d(bool diag, bool two_colour, int nsys, size_t lds_bytes,
+                               const WGArgs &a, hipStream_t st) {
+    // the fixed two-colour layout is compiled for diagonal couplings with 3
+    // or 4 neighbours (honeycomb, square and their Kitaev / XXZ variants)
+    if constexpr (K2C <= 4) {
+        if (diag && two_colour) return launch_reg_m<K2C, SPT, true, SPT / 2>(nsys, lds_bytes, a, st);
+    }
+    return diag ? launch_reg_m<K2C, SPT, true, 0>(nsys, lds_bytes, a, st)
+                : launch_reg_m<K2C, SPT, false, 0>(nsys, lds_bytes, a, st);
 }
 
 template <int K2C>
-static hipError_t launch_reg_s(int spt, bool diag, int nsys, size_t lds_bytes, const WGArgs &a,
-                               hipStream_t st) {
-    if (spt == 4) return launch_reg_d<K2C, 4>(diag, nsys, lds_bytes, a, st);
+static hipError_t launch_reg_s(int spt, bool diag, bool two_colour, int nsys, size_t lds_bytes,
+                               const WGArgs &a, hipStream_t st) {
+    if (spt == 4) return launch_reg_d<K2C, 4>(diag, two_colour, nsys, lds_bytes, a, st);
     // 8 slots of 6 neighbours exceed the register budget of 1024 threads
     if constexpr (K2C <= 4) {
-        if (spt == 8) return launch_reg_d<K2C, 8>(diag, nsys, lds_bytes, a, st);
+        if (spt == 8) return launch_reg_d<K2C, 8>(diag, two_colour, nsys, lds_bytes, a, st);
     }
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_wg_reg(int k2c, int spt, bool diag, int nsys, size_t lds_bytes, const WGArgs &a,
-                         hipStream_t st) {
-    if (k2c == 3) return launch_reg_s<3>(spt, diag, nsys, lds_bytes, a, st);
-    if (k2c == 4) return launch_reg_s<4>(spt, diag, nsys, lds_bytes, a, st);
-    if (k2c == 6) return launch_reg_s<6>(spt, diag, nsys, lds_bytes, a, st);
+hipError_t launch_wg_reg(int k2c, int spt, bool diag, bool two_colour, int nsys, size_t lds_bytes,
+                         const WGArgs &a, hipStream_t st) {
+    if (k2c == 3) return launch_reg_s<3>(spt, diag, two_colour, nsys, lds_bytes, a, st);
+    if (k2c == 4) return launch_reg_s<4>(spt, diag, two_colour, nsys, lds_bytes, a, st);
+    if (k2c == 6) return launch_reg_s<6>(spt, diag, two_colour, nsys, lds_bytes, a, st);
     return hipErrorInvalidValue;
 }
 
