@@ -137,6 +137,8 @@ struct csmc_ctx {
                              // split (default), 0 chosen from the plan's cost,
                              // 2..8 forced
     int ncu = 0;             // compute units of the device
+    uint32_t spin_limit = 1u << 26;  // polls of a wait for another workgroup
+                                     // (x 1/16 for the exchange waits)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     double timed_ms = 0.0;
@@ -602,6 +604,10 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
     if (sp && sp[0] == '0') c->specialise = false;
     const char *ch = getenv("CSMC_PT_CHAIN");
     if (ch && ch[0] == '0') c->pt_chain = false;
+    // a smaller limit makes the waits give up early: for tests of the
+    // failure path (every wave still ends, the call reports the error)
+    const char *lim = getenv("CSMC_SPIN_LIMIT");
+    if (lim && lim[0] >= '0' && lim[0] <= '9') c->spin_limit = (uint32_t)strtoul(lim, nullptr, 10);
     const char *sl = getenv("CSMC_SPLIT");
     if (sl && std::string(sl) == "auto") c->split = 0;
     else if (sl && sl[0] >= '0' && sl[0] <= '8' && sl[1] == 0) c->split = sl[0] == '0' ? 1 : sl[0] - '0';
@@ -1179,7 +1185,7 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
     a.pt.done = pt.done.p;
     a.pt.rec = pt.rec.p;
     a.pt.err = pt.err.p;
-    a.spin_limit = 1u << 26;
+    a.spin_limit = c->spin_limit;
     a.pt.series = pt.series.p;
     a.pt.series_cap = PROBE_CAP;
     // probe sweeps of a segment (the tail of sweep se is not run when emid)
