@@ -10,8 +10,10 @@ from ._lib import (CONE, CONE_ADAPTIVE, DETERMINISTIC, FP32, FP64, METROPOLIS, O
 from .batch import SpinBatch
 from .hdf5io import (create_params_file, dump_metadata, h5open, overwrite_keys, read_lattice,
                      read_spin_configuration, read_unit_cell, write_MC_checkpoint)
-from .lattice import (Lattice, energy, energy_density, get_local_field, get_magnetization, get_spin,
-                      local_fields, random_spin_orientation, set_spin, site_energies, total_energy)
+from .lattice import (Lattice, compute_equal_time_correlations, energy, energy_density, get_local_field,
+                      get_magnetization, get_spin, local_fields, random_spin_orientation, set_spin,
+                      site_energies, total_energy)
+from .reciprocal import get_allowed_wavevectors, get_k_path, get_k_plane, reciprocal
 from .montecarlo import (MCParamsBuffer, Metropolis, MetropolisAdaptive, MetropolisConstraint,
                          MetropolisConstraintAdaptive, MetropolisFixedCone, MonteCarlo,
                          SimulationParameters, deterministic_updates, overrelaxation,

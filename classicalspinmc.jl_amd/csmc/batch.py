@@ -153,6 +153,18 @@ class SpinBatch:
                                             _lib.ptr(ax), _lib.ptr(sg)))
         return dict(system_of_slot=sos, E=E, accepted_local=al, accepted_exchange=ax, sigma=sg)
 
+    def spin_correlations(self, ks):
+        """compute_equal_time_correlations (spin_correlations.jl:6-43) of every
+        system (they must share one lattice): (n, 9, n_k), FP64."""
+        ks = np.asarray(ks, dtype=np.float64)
+        if ks.ndim != 2:
+            raise ValueError("ks must be a D x n_k matrix")
+        kt = np.ascontiguousarray(ks.T)
+        out = np.zeros((self.n, ks.shape[1], 9))
+        _lib.check(_lib.lib().csmc_spin_correlations(self._h, self.id, _lib.ptr(kt), ks.shape[1],
+                                                     _lib.ptr(out)))
+        return np.transpose(out, (0, 2, 1)).copy()
+
     def pt_split_parts(self):
         """workgroups per system of the chained PT kernel (1: whole system
         per workgroup; G > 1: strips with halos)"""

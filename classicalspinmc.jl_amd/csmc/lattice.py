@@ -236,3 +236,18 @@ def get_magnetization(lattice):
     M = np.zeros(3)
     _lib.check(_lib.lib().csmc_magnetization(ctx.h, bid, _lib.ptr(M)))
     return float(np.linalg.norm(M))
+
+
+def compute_equal_time_correlations(lattice, ks):
+    """compute_equal_time_correlations (spin_correlations.jl:6-43): the 9 x n_k
+    matrix Suv (rows xx xy xz yx yy yz zx zy zz) of equal-time spin
+    correlations at the momenta ks (D x n_k), divided by lattice.size.  FP64
+    on the GPU (phases and two GEMMs)."""
+    ks = np.asarray(ks, dtype=np.float64)
+    if ks.ndim != 2 or ks.shape[0] != lattice.unit_cell.D:
+        raise ValueError("ks must be a D x n_k matrix")
+    ctx, bid = lattice._eval_batch()
+    kt = np.ascontiguousarray(ks.T)
+    out = np.zeros((ks.shape[1], 9))
+    _lib.check(_lib.lib().csmc_spin_correlations(ctx.h, bid, _lib.ptr(kt), ks.shape[1], _lib.ptr(out)))
+    return out.T.copy()

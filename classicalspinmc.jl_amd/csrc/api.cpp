@@ -107,6 +107,8 @@ struct Batch {
     DevBuf<double> T, sigma, acc;  // per-system scratch
     DevBuf<uint32_t> accU;
     DevBuf<double> stage;          // host<->device staging (total*3 doubles)
+    // equal-time correlations: device positions, momenta and GEMM workspaces
+    DevBuf<double> ssf_pos, ssf_ks, ssf_C, ssf_Q, ssf_B, ssf_Cre, ssf_Cim, ssf_out;
     PTHost pt;
     size_t spin_bytes() const { return prec == 32 ? 16 : 32; }
 };
@@ -137,6 +139,7 @@ struct csmc_ctx {
                              // split (default), 0 chosen from the plan's cost,
                              // 2..8 forced
     int ncu = 0;             // compute units of the device
+    void *blas = nullptr;    // rocBLAS handle (created on first use)
     uint32_t spin_limit = 1u << 26;  // polls of a wait for another workgroup
                                      // (x 1/16 for the exchange waits)
     bool timing = false;
@@ -633,6 +636,7 @@ int32_t csmc_ctx_destroy(csmc_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     c->batches.clear();
+    blas_destroy(c->blas);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return CSMC_OK;
@@ -1445,6 +1449,46 @@ int32_t csmc_pt_observables(csmc_ctx *c, int32_t id, double *out) {
         o[6] = mm[0];
         o[7] = bin_err(bm, g0);
     }
+    return CSMC_OK;
+}
+
+int32_t csmc_spin_correlations(csmc_ctx *c, int32_t id, const double *ks, int32_t n_k, double *out) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (n_k < 1 || !ks || !out) return fail(CSMC_EINVAL, "spin correlations need momenta and an output");
+    for (int s = 1; s < b->nsys; ++s)
+        if (b->lat[s] != b->lat[0])
+            return fail(CSMC_EINVAL, "spin correlations need the systems of the batch to share one lattice");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    const HostLattice &L = *c->lats[b->lat[0]];
+    const int N = (int)L.N, D = L.dim, R = b->nsys;
+    // site positions (compute_site_positions, lattice.jl:38-51) in device order
+    std::vector<double> pref((size_t)N * D), pdev((size_t)N * 3, 0.0), kv((size_t)n_k * 3, 0.0);
+    export_tables(L, pref.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    for (int d = 0; d < N; ++d)
+        for (int q = 0; q < D; ++q) pdev[(size_t)d * 3 + q] = pref[(size_t)L.ref_of_dev[d] * D + q];
+    for (int k = 0; k < n_k; ++k)
+        for (int q = 0; q < D; ++q) kv[(size_t)k * 3 + q] = ks[(size_t)k * D + q];
+    if (!c->blas) HIPCHK(blas_create(&c->blas));
+    // momenta per block: at most 2048, and the two kb x N phase matrices
+    // within 2 x 512 MB
+    const int kb = std::max(1, std::min({n_k, 2048, (int)std::max<int64_t>(64, ((int64_t)1 << 26) / N)}));
+    HIPCHK(b->ssf_pos.alloc(pdev.size()));
+    HIPCHK(b->ssf_ks.alloc(kv.size()));
+    HIPCHK(b->ssf_C.alloc((size_t)kb * N));
+    HIPCHK(b->ssf_Q.alloc((size_t)kb * N));
+    HIPCHK(b->ssf_B.alloc((size_t)3 * R * N));
+    HIPCHK(b->ssf_Cre.alloc((size_t)3 * R * kb));
+    HIPCHK(b->ssf_Cim.alloc((size_t)3 * R * kb));
+    HIPCHK(b->ssf_out.alloc((size_t)R * n_k * 9));
+    HIPCHK(hipMemcpyAsync(b->ssf_pos.p, pdev.data(), pdev.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b->ssf_ks.p, kv.data(), kv.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_ssf(c->blas, b->prec, b->spins.p, b->sys_d.p, R, N, b->ssf_pos.p, b->ssf_ks.p, n_k, kb,
+                      b->ssf_C.p, b->ssf_Q.p, b->ssf_B.p, b->ssf_Cre.p, b->ssf_Cim.p, b->ssf_out.p,
+                      c->stream));
+    HIPCHK(hipMemcpyAsync(out, b->ssf_out.p, (size_t)R * n_k * 9 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return CSMC_OK;
 }
 

@@ -138,5 +138,14 @@ hipError_t launch_bin_fold(int R, const double *series, int cap, int n, double *
                            hipStream_t st);
 hipError_t launch_gd(int prec, const DevPools &P, void *spins, int nsys, int maxN,
                      const double *dEdS, double eta, hipStream_t st);
+// equal-time correlations of nsys systems of one lattice (N sites, device
+// positions pos[N][3]) at momenta ks[nk][3], in blocks of kb momenta:
+// workspaces C, Q [kb][N], B [3 nsys][N], Cre, Cim [3 nsys][kb];
+// out[nsys][nk][9]
+hipError_t launch_ssf(void *blas, int prec, const void *spins, const SysDev *sys, int nsys, int N,
+                      const double *pos, const double *ks, int nk, int kb, double *C, double *Q,
+                      double *B, double *Cre, double *Cim, double *out, hipStream_t st);
+hipError_t blas_create(void **blas);
+void blas_destroy(void *blas);
 
 }  // namespace csmc

@@ -234,6 +234,15 @@ int32_t csmc_pt_split_parts(csmc_ctx *ctx, int32_t batch_id, int32_t *parts);
  * out[n_sys][8] = C, dC, chi, dchi, <E>, dE, <|M|>, d|M| (absolute errors). */
 int32_t csmc_pt_observables(csmc_ctx *ctx, int32_t batch_id, double *out);
 
+/* compute_equal_time_correlations(lat, ks) (spin_correlations.jl:6-43) for
+ * every system of the batch (which must share one lattice): ks is the
+ * reference's D x n_k momentum matrix (double[n_k][D]); out receives, per
+ * system, its 9 x n_k Suv matrix as Julia lays it out (double[n_sys][n_k][9]:
+ * xx xy xz yx yy yz zx zy zz per momentum), already divided by lat.size.  FP64
+ * throughout (phases and two GEMMs per block of momenta). */
+int32_t csmc_spin_correlations(csmc_ctx *ctx, int32_t batch_id, const double *ks, int32_t n_k,
+                               double *out);
+
 /* ---- differentiation (no reference counterpart: adjoint of
  * total_energy, hamiltonian.jl:70-132) ------------------------------------ */
 /* dE/dS_i for every site of every system, double[n_sys][N][3]. */

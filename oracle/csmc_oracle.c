@@ -31,6 +31,12 @@
  *   orc_metropolis_*   single spin in a field: <s.h> = S L(beta h S) (Langevin).
  *   orc_pt_exchange_*  the acceptance formula monte_carlo.jl:326-331 checked
  *                      against detailed balance on a two-level toy.
+ *   orc_equal_time_    closed forms: a ferromagnet along z gives S^zz(0) = N S^2
+ *   correlations       and 0 at the other allowed momenta; a Neel state on the
+ *                      square lattice gives S^zz(pi,pi) = N S^2; Parseval over
+ *                      the ncell allowed momenta of a Bravais lattice,
+ *                      sum_k S^aa(k) = sum_i s_ia^2; S^ab = S^ba and S(k) =
+ *                      S(-k); a direct numpy FFT of the spins on the grid.
  */
 #include <math.h>
 #include <stdint.h>
@@ -900,4 +906,31 @@ long orc_pt_reference_run(const orc_lat *L, double *spins, int R, const double *
     }
     free(order);
     return updates;
+}
+
+/* ------------------------------------------------------------------------ */
+/* compute_equal_time_correlations (spin_correlations.jl:6-43): for each    */
+/* momentum k (column n of the D x n_k matrix ks), s_a(k) = sum_i            */
+/* exp(-i k.r_i) s_{i,a} with r_i the site positions (D x N, lattice.jl:12), */
+/* then Suv[3a+b][k] = Re(s_a(k) conj(s_b(k))) / N.  out is the reference's   */
+/* 9 x n_k column-major matrix: out[9 k + 3 a + b].                          */
+/* ------------------------------------------------------------------------ */
+void orc_equal_time_correlations(const double *pos, long N, int D, const double *spins,
+                                 const double *ks, long nk, double *out)
+{
+    for (long n = 0; n < nk; ++n) {
+        double re[3] = {0, 0, 0}, im[3] = {0, 0, 0};
+        for (long i = 0; i < N; ++i) {
+            double arg = 0.0;
+            for (int d = 0; d < D; ++d) arg += ks[n * D + d] * pos[i * D + d];
+            const double c = cos(arg), sn = -sin(arg);  /* exp(-i arg) */
+            for (int a = 0; a < 3; ++a) {
+                re[a] += c * spins[3 * i + a];
+                im[a] += sn * spins[3 * i + a];
+            }
+        }
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                out[9 * n + 3 * a + b] = (re[a] * re[b] + im[a] * im[b]) / (double)N;
+    }
 }

@@ -284,3 +284,18 @@ def pt_reference_run(T, spins, temps, S, sweep0, n_sweeps, or_rate, swap_rate, s
         T.ref, _p(spins, _D), C.c_int(R), _p(t, _D), C.c_double(S), C.c_long(sweep0),
         C.c_long(n_sweeps), C.c_long(or_rate), C.c_long(swap_rate), C.c_uint64(seed), _p(E, _D),
         _p(perm, _I), _p(al, _L), _p(ax, _L))
+
+
+def equal_time_correlations(pos, spins, ks):
+    """compute_equal_time_correlations (spin_correlations.jl:6-43).
+    pos: D x N site positions, spins: (N, 3), ks: D x n_k; returns the
+    9 x n_k Suv matrix (rows xx xy xz yx yy yz zx zy zz), divided by N."""
+    P = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).T)
+    K = np.ascontiguousarray(np.asarray(ks, dtype=np.float64).T)
+    s = _spins(spins)
+    N, D = P.shape
+    nk = K.shape[0]
+    out = np.zeros((nk, 9))
+    lib().orc_equal_time_correlations(_p(P, _D), C.c_long(N), C.c_int(D), _p(s, _D), _p(K, _D),
+                                      C.c_long(nk), _p(out, _D))
+    return out.T.copy()

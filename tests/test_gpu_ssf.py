@@ -1,0 +1,98 @@
+"""Equal-time spin correlations on the GPU (csmc_spin_correlations: FP64
+phases and two FP64 GEMMs per block of momenta) against the oracle's
+restatement of compute_equal_time_correlations (spin_correlations.jl:6-43),
+itself pinned to closed forms and an FFT on the host.  FP64 and FP32 spin
+storage (the FP32 spins widen exactly, so both compare at FP64 accuracy),
+2D and 3D lattices, several systems in one call, more momenta than one block
+holds, and the ferromagnet's S^zz(0) = N S^2."""
+import numpy as np
+import pytest
+
+import csmc
+import oracle.oracle as orc
+from csmc import _lib, workloads as W
+from csmc.reciprocal import get_allowed_wavevectors
+
+pytestmark = pytest.mark.gpu
+
+
+def _close(a, b, rtol=1e-12):
+    # relative to the largest entry: S(k) spans many orders of magnitude
+    return np.max(np.abs(a - b)) <= rtol * max(1.0, float(np.max(np.abs(b))))
+
+
+CASES = [  # name, cell, shape, S, precision, systems, momenta
+    ("honeycomb_fp64_R3", lambda: W.honeycomb_kitaev(1.0, -2.0), (8, 8), 0.5, 64, 3, "allowed"),
+    ("honeycomb_fp32_R4", lambda: W.honeycomb_kitaev(1.0, -2.0), (6, 10), 0.5, 32, 4, "allowed"),
+    ("pyrochlore_fp32_R2", W.pyrochlore_mixed, (3, 3, 3), 1.0, 32, 2, "allowed"),
+    ("square_fp64_R2_2500k", lambda: W.square_heisenberg(-1.0, (0, 0, 0.1)), (16, 16), 1.0, 64, 2, "random"),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_batch_correlations_match_oracle(case):
+    name, mk, shape, S, prec, R, kind = case
+    print(f"[ssf] {name}", flush=True)
+    uc = mk()
+    lat = csmc.Lattice(shape, uc, S, seed=5)
+    if kind == "allowed":
+        ks = get_allowed_wavevectors(uc, shape, min=-1, max=1)
+    else:
+        # more momenta than one block of the GEMM (2048) holds
+        ks = np.random.default_rng(9).uniform(-7.0, 7.0, size=(len(shape), 2500))
+    b = csmc.SpinBatch([lat] * R, precision=prec)
+    b.randomize(11)
+    got = b.spin_correlations(ks)
+    assert got.shape == (R, 9, ks.shape[1])
+    for r in range(R):
+        ref = orc.equal_time_correlations(lat.site_positions, b.get_spins(r), ks)
+        assert _close(got[r], ref), (name, r)
+    # S^ab = S^ba holds for every system
+    sw = got.reshape(R, 3, 3, -1)
+    assert _close(sw, np.transpose(sw, (0, 2, 1, 3)), 1e-13)
+    b.close()
+
+
+def test_lattice_form_and_ferromagnet():
+    uc = W.honeycomb_kitaev(1.0, -2.0)
+    shape, S = (12, 12), 1.5
+    lat = csmc.Lattice(shape, uc, S, seed=2)
+    ks = get_allowed_wavevectors(uc, shape)
+    got = csmc.compute_equal_time_correlations(lat, ks)
+    ref = orc.equal_time_correlations(lat.site_positions, lat.spins.T, ks)
+    assert got.shape == (9, ks.shape[1])
+    assert _close(got, ref)
+    fm = np.zeros((lat.size, 3))
+    fm[:, 2] = S
+    lat.spins = fm
+    got = csmc.compute_equal_time_correlations(lat, np.zeros((2, 1)))
+    assert got[8, 0] == pytest.approx(lat.size * S * S, rel=1e-13)
+    assert np.max(np.abs(got[:8, 0])) < 1e-12
+
+
+def test_full_size_batch_against_oracle_on_a_subset():
+    """the bench lattice (honeycomb 64 x 64, 8192 sites) with 64 FP32
+    replicas in one call; two systems checked against the oracle"""
+    uc = W.honeycomb_kitaev(1.0, -2.0)
+    lat = csmc.Lattice((64, 64), uc, 0.5, seed=4)
+    R = 64
+    ks = get_allowed_wavevectors(uc, (16, 16), min=-1, max=1)
+    print(f"[ssf] full size N={lat.size} R={R} n_k={ks.shape[1]}", flush=True)
+    b = csmc.SpinBatch([lat] * R, precision=32)
+    b.randomize(21)
+    got = b.spin_correlations(ks)
+    for r in (0, R - 1):
+        ref = orc.equal_time_correlations(lat.site_positions, b.get_spins(r), ks)
+        assert _close(got[r], ref), r
+    b.close()
+
+
+def test_rejects_mixed_lattices_and_bad_momenta():
+    a = csmc.Lattice((4, 4), W.square_heisenberg(-1.0, (0, 0, 0.1)), 1.0, seed=1)
+    c = csmc.Lattice((6, 6), W.square_heisenberg(-1.0, (0, 0, 0.1)), 1.0, seed=1)
+    b = csmc.SpinBatch([a, c], precision=64)
+    with pytest.raises(_lib.CsmcError, match="share one lattice"):
+        b.spin_correlations(np.zeros((2, 3)))
+    with pytest.raises(ValueError):
+        b.spin_correlations(np.zeros(3))
+    b.close()
