@@ -23,7 +23,10 @@ even/odd exchange) on a bounded sample on the host cores.
 --workload anneal_uneven and pt_pyrochlore measure the other batched
 workloads (independent lattices of uneven size annealed in one launch; the
 pyrochlore model with cubic, quartic and onsite terms under parallel
-tempering) with the same JSON fields.
+tempering) with the same JSON fields.  --workload ssf measures the
+equal-time structure factor (compute_equal_time_correlations) of the 64
+replicas of the default workload at the 65 x 65 allowed momenta of one
+Brillouin-zone copy, against the FP64 MFMA roofline.
 """
 import argparse
 import json
@@ -191,7 +194,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1024,
                     help="untimed steps first (1024 = one whole chained launch)")
     ap.add_argument("--workload", default="pt_honeycomb",
-                    choices=["pt_honeycomb", "anneal_uneven", "pt_pyrochlore"])
+                    choices=["pt_honeycomb", "anneal_uneven", "pt_pyrochlore", "ssf"])
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline")
@@ -215,6 +218,8 @@ def main():
         return bench_anneal(args, csmc, W)
     if args.workload == "pt_pyrochlore":
         return bench_pyrochlore(args, csmc, W)
+    if args.workload == "ssf":
+        return bench_ssf(args, csmc, W)
 
     L, R = args.L, args.replicas
     uc = W.honeycomb_kitaev(1.0, -2.0)
@@ -359,6 +364,106 @@ def bench_pyrochlore(args, csmc, W):
                                   "parallel tempering, 32 replicas, T=0.02..2",
                       "storage_mode": r["mode"]}}
     r["b"].close()
+    finish(out, args)
+
+
+PEAK_FP64_TFLOPS = 78.6
+SSF_FLOPS_PER_TERM = 12  # 3 components x (cos, sin) x one multiply-add
+
+
+def ssf_cpu_baseline(pos, spins, ks, seconds, threads):
+    """The oracle's compute_equal_time_correlations on the host: the momenta
+    split over `threads` host threads (the C loop runs without the GIL), one
+    system after another until about `seconds` have passed.  Returns
+    terms/s (momenta x sites per system) and a description of the sample."""
+    import oracle.oracle as orc
+    threads = max(1, threads)
+    parts = np.array_split(np.arange(ks.shape[1]), threads)
+    stop = time.perf_counter() + seconds
+    done = [0] * threads
+    systems = [0] * threads
+
+    def work(k):
+        kp = np.ascontiguousarray(ks[:, parts[k]])
+        r = 0
+        while time.perf_counter() < stop:
+            orc.equal_time_correlations(pos, spins[r % len(spins)], kp)
+            done[k] += kp.shape[1] * pos.shape[1]
+            systems[k] += 1
+            r += 1
+
+    t0 = time.perf_counter()
+    pool = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+    for t in pool:
+        t.start()
+    for t in pool:
+        t.join()
+    dt = time.perf_counter() - t0
+    sample = (f"oracle compute_equal_time_correlations (FP64 sincos per term), {ks.shape[1]} momenta split "
+              f"over {threads} host threads, {min(systems)}-{max(systems)} systems per thread in {dt:.1f} s")
+    return sum(done) / dt, sample
+
+
+def bench_ssf(args, csmc, W):
+    """Equal-time structure factor S^ab(k) of 64 FP32 replicas of the
+    honeycomb L = 64 lattice (8192 sites) at the 65 x 65 allowed momenta
+    (get_allowed_wavevectors, one zone copy): one call per step, inputs and
+    outputs resident in HBM for `value`; the output copy to the host is
+    reported beside it."""
+    from csmc.reciprocal import get_allowed_wavevectors
+    L, R = args.L, args.replicas
+    uc = W.honeycomb_kitaev(1.0, -2.0)
+    lat = csmc.Lattice((L, L), uc, 0.5, seed=3)
+    ks = get_allowed_wavevectors(uc, (L, L))
+    nk, N = ks.shape[1], lat.size
+    ctx = csmc.default_context()
+    b = csmc.SpinBatch([lat] * R, precision=32, ctx=ctx)
+    b.randomize(3)
+    steps = min(args.steps, 50)
+    warm = max(2, min(args.warmup, 5))
+    for _ in range(warm):
+        b.spin_correlations(ks)
+    ctx.sync()
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.spin_correlations(ks)
+    dt = time.perf_counter() - t0
+    kern_ms, launches = ctx.timing_read()
+    ctx.timing(False)
+    terms = nk * N * R
+    avg_ms = kern_ms / launches
+    value = terms / (avg_ms * 1e-3)
+    flops = SSF_FLOPS_PER_TERM * terms
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    kname = "rocBLAS dgemm path" if os.environ.get("CSMC_SSF") == "blas" else "k_ssf_mfma"
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_ssf_mfma_honeycomb_L64_R64_traffic.json")
+    if kname == "k_ssf_mfma" and L == 64 and R == 64 and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes_per_launch"]
+    out = {"metric": "equal-time structure-factor terms/s (momenta x sites x systems) on 1 MI355X; "
+                     "% of FP64 MFMA roofline",
+           "value": value, "unit": "terms/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+           "ms_per_step": avg_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp64", "data": "synthetic",
+           "config": {"workload": f"compute_equal_time_correlations of {R} replicas of the honeycomb L={L} "
+                                  f"lattice ({N} sites, FP32 spins) at the {nk} allowed momenta of one zone "
+                                  "copy, all replicas in one call",
+                      "sites": N, "systems": R, "momenta": nk},
+           "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP64_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+                        "kernel": kname, "flops_per_launch": flops, "avg_launch_ms": round(avg_ms, 4)},
+           "with_host_copy": {"value": terms * steps / dt, "unit": "terms/s",
+                              "note": "wall clock per call, including the output copy to the host "
+                                      f"({R * nk * 9 * 8 / 1e6:.1f} MB over PCIe)"}}
+    if not args.no_cpu:
+        spins = [b.get_spins(r) for r in range(4)]
+        cpu, sample = ssf_cpu_baseline(lat.site_positions, spins, ks, args.cpu_seconds, args.cpu_threads)
+        out["cpu_baseline"] = {"value": cpu, "unit": "terms/s", "cores": args.cpu_threads, "kind": "port",
+                               "sample": sample, "gpu_over_cpu": value / cpu}
+    b.close()
+    log(f"[bench] ssf: {avg_ms:.3f} ms per call, {achieved:.2f} TFLOP/s")
     finish(out, args)
 
 

@@ -50,6 +50,11 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // keeps the buffer when it already holds count elements
+    hipError_t reserve(size_t count) {
+        if (p && n >= count) return hipSuccess;
+        return alloc(count);
+    }
     hipError_t alloc(size_t count) {
         free_();
         n = count;
@@ -108,7 +113,7 @@ struct Batch {
     DevBuf<uint32_t> accU;
     DevBuf<double> stage;          // host<->device staging (total*3 doubles)
     // equal-time correlations: device positions, momenta and GEMM workspaces
-    DevBuf<double> ssf_pos, ssf_ks, ssf_C, ssf_Q, ssf_B, ssf_Cre, ssf_Cim, ssf_out;
+    DevBuf<double> ssf_pos, ssf_ks, ssf_C, ssf_Q, ssf_B, ssf_Cre, ssf_Cim, ssf_P, ssf_out;
     PTHost pt;
     size_t spin_bytes() const { return prec == 32 ? 16 : 32; }
 };
@@ -140,6 +145,8 @@ struct csmc_ctx {
                              // 2..8 forced
     int ncu = 0;             // compute units of the device
     void *blas = nullptr;    // rocBLAS handle (created on first use)
+    bool ssf_blas = false;   // equal-time correlations through rocBLAS
+                             // instead of the fused MFMA kernel
     uint32_t spin_limit = 1u << 26;  // polls of a wait for another workgroup
                                      // (x 1/16 for the exchange waits)
     bool timing = false;
@@ -619,6 +626,8 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess)
             c->ncu = ncu;
     }
+    const char *sf = getenv("CSMC_SSF");
+    if (sf && std::string(sf) == "blas") c->ssf_blas = true;
     const char *vb = getenv("CSMC_VERBOSE");
     if (vb && vb[0] == '1') {
         hipDeviceProp_t prop;
@@ -1463,30 +1472,67 @@ int32_t csmc_spin_correlations(csmc_ctx *c, int32_t id, const double *ks, int32_
     if (rc) return rc;
     const HostLattice &L = *c->lats[b->lat[0]];
     const int N = (int)L.N, D = L.dim, R = b->nsys;
-    // site positions (compute_site_positions, lattice.jl:38-51) in device order
-    std::vector<double> pref((size_t)N * D), pdev((size_t)N * 3, 0.0), kv((size_t)n_k * 3, 0.0);
+    // site positions (compute_site_positions, lattice.jl:38-51);
+    // max |k| max |r| bounds every phase argument |k.r|
+    std::vector<double> pref((size_t)N * D);
     export_tables(L, pref.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    double rmax = 0.0, kmax = 0.0;
+    for (int i = 0; i < N; ++i) {
+        double r2 = 0.0;
+        for (int q = 0; q < D; ++q) r2 += pref[(size_t)i * D + q] * pref[(size_t)i * D + q];
+        rmax = std::max(rmax, std::sqrt(r2));
+    }
+    for (int k = 0; k < n_k; ++k) {
+        double k2 = 0.0;
+        for (int q = 0; q < D; ++q) k2 += ks[(size_t)k * D + q] * ks[(size_t)k * D + q];
+        if (!std::isfinite(k2)) return fail(CSMC_EINVAL, "momenta must be finite");
+        kmax = std::max(kmax, std::sqrt(k2));
+    }
+    SsfPlan plan;
+    ssf_plan(N, n_k, R, c->ncu, kmax * rmax * (1.0 + 1e-12) < SSF_FAST_ARG, &plan);
+    const int np = c->ssf_blas ? N : plan.npad, mp = c->ssf_blas ? n_k : plan.mpad;
+    // positions in device order and the momenta, 3 components each,
+    // zero-padded
+    std::vector<double> pdev((size_t)np * 3, 0.0), kv((size_t)mp * 3, 0.0);
     for (int d = 0; d < N; ++d)
         for (int q = 0; q < D; ++q) pdev[(size_t)d * 3 + q] = pref[(size_t)L.ref_of_dev[d] * D + q];
     for (int k = 0; k < n_k; ++k)
         for (int q = 0; q < D; ++q) kv[(size_t)k * 3 + q] = ks[(size_t)k * D + q];
-    if (!c->blas) HIPCHK(blas_create(&c->blas));
-    // momenta per block: at most 2048, and the two kb x N phase matrices
-    // within 2 x 512 MB
-    const int kb = std::max(1, std::min({n_k, 2048, (int)std::max<int64_t>(64, ((int64_t)1 << 26) / N)}));
-    HIPCHK(b->ssf_pos.alloc(pdev.size()));
-    HIPCHK(b->ssf_ks.alloc(kv.size()));
-    HIPCHK(b->ssf_C.alloc((size_t)kb * N));
-    HIPCHK(b->ssf_Q.alloc((size_t)kb * N));
-    HIPCHK(b->ssf_B.alloc((size_t)3 * R * N));
-    HIPCHK(b->ssf_Cre.alloc((size_t)3 * R * kb));
-    HIPCHK(b->ssf_Cim.alloc((size_t)3 * R * kb));
-    HIPCHK(b->ssf_out.alloc((size_t)R * n_k * 9));
+    HIPCHK(b->ssf_pos.reserve(pdev.size()));
+    HIPCHK(b->ssf_ks.reserve(kv.size()));
+    HIPCHK(b->ssf_out.reserve((size_t)R * n_k * 9));
     HIPCHK(hipMemcpyAsync(b->ssf_pos.p, pdev.data(), pdev.size() * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(b->ssf_ks.p, kv.data(), kv.size() * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_ssf(c->blas, b->prec, b->spins.p, b->sys_d.p, R, N, b->ssf_pos.p, b->ssf_ks.p, n_k, kb,
-                      b->ssf_C.p, b->ssf_Q.p, b->ssf_B.p, b->ssf_Cre.p, b->ssf_Cim.p, b->ssf_out.p,
-                      c->stream));
+    // with timing on, the events bracket the dominant work: the MFMA kernel,
+    // or the phases and GEMMs of the rocBLAS path
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+    }
+    if (c->ssf_blas) {
+        if (e0) HIPCHK(hipEventRecord(e0, c->stream));
+        // momenta per block: at most 2048, and the two kb x N phase matrices
+        // within 2 x 512 MB
+        const int kb =
+            std::max(1, std::min({n_k, 2048, (int)std::max<int64_t>(64, ((int64_t)1 << 26) / N)}));
+        if (!c->blas) HIPCHK(blas_create(&c->blas));
+        HIPCHK(b->ssf_C.reserve((size_t)kb * N));
+        HIPCHK(b->ssf_Q.reserve((size_t)kb * N));
+        HIPCHK(b->ssf_B.reserve((size_t)3 * R * N));
+        HIPCHK(b->ssf_Cre.reserve((size_t)3 * R * kb));
+        HIPCHK(b->ssf_Cim.reserve((size_t)3 * R * kb));
+        HIPCHK(launch_ssf_blas(c->blas, b->prec, b->spins.p, b->sys_d.p, R, N, b->ssf_pos.p, b->ssf_ks.p, n_k,
+                               kb, b->ssf_C.p, b->ssf_Q.p, b->ssf_B.p, b->ssf_Cre.p, b->ssf_Cim.p,
+                               b->ssf_out.p, c->stream));
+        if (e1) HIPCHK(hipEventRecord(e1, c->stream));
+    } else {
+        HIPCHK(b->ssf_B.reserve((size_t)plan.npad * plan.ldb));
+        HIPCHK(b->ssf_P.reserve((size_t)plan.ksplit * 2 * plan.mpad * plan.ldb));
+        HIPCHK(launch_ssf_mfma(plan, b->prec, b->spins.p, b->sys_d.p, b->ssf_pos.p, b->ssf_ks.p, b->ssf_B.p,
+                               b->ssf_P.p, b->ssf_out.p, c->stream, e0, e1));
+    }
+    if (c->timing) c->events.push_back({e0, e1});
     HIPCHK(hipMemcpyAsync(out, b->ssf_out.p, (size_t)R * n_k * 9 * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return CSMC_OK;

@@ -142,7 +142,30 @@ hipError_t launch_gd(int prec, const DevPools &P, void *spins, int nsys, int max
 // positions pos[N][3]) at momenta ks[nk][3], in blocks of kb momenta:
 // workspaces C, Q [kb][N], B [3 nsys][N], Cre, Cim [3 nsys][kb];
 // out[nsys][nk][9]
-hipError_t launch_ssf(void *blas, int prec, const void *spins, const SysDev *sys, int nsys, int N,
+// equal-time correlations, fused FP64 MFMA path: sizes of the padded
+// operands and of the split of the site sum
+constexpr int SSF_KC = 16;  // sites per LDS chunk
+// the fused path's own branch-free sincos is used when every |k.r| is
+// below this bound (OCML's sincos otherwise)
+constexpr double SSF_FAST_ARG = 1e5;
+struct SsfPlan {
+    int N, nk, nsys;
+    bool fast;                // the branch-free sincos
+    int nct;                  // 16-column tiles per wave (1, 2, 3, 4, 6, 8, 12)
+    int colblocks, ldb;       // column blocks of 16 nct; ldb = colblocks 16 nct
+    int mblocks, mpad;        // blocks of 64 momenta; mpad = 64 mblocks
+    int nchunks, npad;        // chunks of SSF_KC sites; npad = SSF_KC nchunks
+    int ksplit, cps;          // splits of the site sum, chunks per split
+};
+void ssf_plan(int N, int nk, int nsys, int ncu, bool fast, SsfPlan *p);
+// pos: npad x 3, ks: mpad x 3 (zero-padded); B: npad x ldb and
+// P: ksplit x 2 x mpad x ldb scratch; out: nsys x nk x 9; ev0 / ev1 (may be
+// null) are recorded around the MFMA kernel alone
+hipError_t launch_ssf_mfma(const SsfPlan &p, int prec, const void *spins, const SysDev *sys, const double *pos,
+                           const double *ks, double *B, double *P, double *out, hipStream_t st,
+                           hipEvent_t ev0, hipEvent_t ev1);
+// the rocBLAS path: phase matrices in memory, two dgemm per block of momenta
+hipError_t launch_ssf_blas(void *blas, int prec, const void *spins, const SysDev *sys, int nsys, int N,
                       const double *pos, const double *ks, int nk, int kb, double *C, double *Q,
                       double *B, double *Cre, double *Cim, double *out, hipStream_t st);
 hipError_t blas_create(void **blas);

@@ -5,15 +5,26 @@
 // For momenta k and sites i at positions r_i,
 //   s_a(k) = sum_i exp(-i k.r_i) s_{i,a},   S^{ab}(k) = Re(s_a(k) conj(s_b(k))) / N.
 // With C = cos(k.r) and Q = sin(k.r) as (n_k x N) matrices and the spins of
-// all R systems as one (N x 3R) matrix B, Re s = C B and Im s = -Q B: two
-// FP64 GEMMs (rocBLAS; MFMA on gfx950) per block of momenta, so the phases
-// are generated once for all systems.  S^{ab} = (Re_a Re_b + Im_a Im_b) / N
-// (the sign of Im cancels).  Phases are FP64 sincos of the FP64 argument.
+// all R systems as one (N x 3R) matrix B, Re s = C B and Im s = -Q B, and
+// S^{ab} = (Re_a Re_b + Im_a Im_b) / N (the sign of Im cancels).
+//
+// Default path, k_ssf_mfma: one FP64 MFMA kernel.  Each wave owns 16
+// momenta and makes their phases in registers (FP64 sine and cosine of the
+// FP64 argument, one per lane per 4 sites: the A operand of both products), the
+// spins are staged through LDS in chunks of 16 sites shared by the 4 waves
+// (the B operand), and the N sum is split over workgroups to fill the
+// device; a small kernel sums the partial products in a fixed order and forms
+// the 9 correlations.  Nothing of size n_k x N ever reaches memory.
+//
+// The alternative path (the phase matrices written to memory and two
+// rocBLAS dgemm per block of momenta) is kept for comparison.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
 #include "device.h"
 #include "kernels.h"
+
+#include <algorithm>
 
 namespace csmc {
 
@@ -61,11 +72,346 @@ __global__ void k_ssf_combine(const double *Cre, const double *Cim, int kb, int 
         for (int b = 0; b < 3; ++b) o[3 * a + b] = (re[a] * re[b] + im[a] * im[b]) * inv_n;
 }
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// B[i * ldb + c] = component c % 3 of site i (device order) of system c / 3;
+// zero for i >= N or c >= 3 nsys (the padding the MFMA tiles read)
+template <class R>
+__global__ void k_ssf_spins_rows(const void *spins, const SysDev *sys, int nsys, int N, int npad, int ldb,
+                                 double *B) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)npad * ldb) return;
+    const int i = (int)(t / ldb), c = (int)(t % ldb);
+    double v = 0.0;
+    if (i < N && c < 3 * nsys) {
+        const spin4<R> s = reinterpret_cast<const spin4<R> *>(spins)[sys[c / 3].spin_off + i];
+        const int a = c % 3;
+        v = (double)(a == 0 ? s.x : a == 1 ? s.y : s.z);
+    }
+    B[t] = v;
+}
+
+// sin and cos of x for |x| <= SSF_FAST_ARG (the host checks the bound):
+// x = n pi/2 + r with pi/2 split in three parts (Cody and Waite; the first two
+// have 33 significant bits, so n P1 and n P2 are exact for |n| < 2^20), the
+// Taylor series of sin to r^15 and of cos to r^16 on |r| <= pi/4 (truncation
+// below 5e-17), and the quadrant n mod 4.  No branch, so the phase of the
+// next sites can be scheduled between the MFMAs of the current ones.
+__device__ __forceinline__ void sincos_reduced(double x, double *sn, double *cs) {
+    const double n = __builtin_rint(x * 0x1.45f306dc9c883p-1);  // 2 / pi
+    double r = __builtin_fma(-n, 0x1.921fb54400000p+0, x);
+    r = __builtin_fma(-n, 0x1.0b4611a600000p-34, r);
+    r = __builtin_fma(-n, 0x1.3198a2e037073p-69, r);
+    const double z = r * r;
+    double sp = -1.0 / 1307674368000.0;  // -1 / 15!
+    sp = __builtin_fma(sp, z, 1.0 / 6227020800.0);
+    sp = __builtin_fma(sp, z, -1.0 / 39916800.0);
+    sp = __builtin_fma(sp, z, 1.0 / 362880.0);
+    sp = __builtin_fma(sp, z, -1.0 / 5040.0);
+    sp = __builtin_fma(sp, z, 1.0 / 120.0);
+    sp = __builtin_fma(sp, z, -1.0 / 6.0);
+    const double s = __builtin_fma(r * z, sp, r);
+    double cp = 1.0 / 20922789888000.0;  // 1 / 16!
+    cp = __builtin_fma(cp, z, -1.0 / 87178291200.0);
+    cp = __builtin_fma(cp, z, 1.0 / 479001600.0);
+    cp = __builtin_fma(cp, z, -1.0 / 3628800.0);
+    cp = __builtin_fma(cp, z, 1.0 / 40320.0);
+    cp = __builtin_fma(cp, z, -1.0 / 720.0);
+    cp = __builtin_fma(cp, z, 1.0 / 24.0);
+    cp = __builtin_fma(cp, z, -0.5);
+    const double c = __builtin_fma(z, cp, 1.0);
+    const int q = (int)n & 3;
+    const double sv = (q & 1) ? c : s, cv = (q & 1) ? s : c;
+    *sn = (q & 2) ? -sv : sv;
+    *cs = ((q + 1) & 2) ? -cv : cv;
+}
+
+template <bool FAST>
+__device__ __forceinline__ void phase(double arg, double *sn, double *cs) {
+    if constexpr (FAST)
+        sincos_reduced(arg, sn, cs);
+    else
+        sincos(arg, sn, cs);
+}
+
+// Partial products of a block of 64 momenta (16 per wave) x 16 NCT columns
+// over the chunks [ch0, ch1) of 16 sites:
+//   P[((z 2 + 0) mpad + m) ldb + c] = sum_i cos(k_m.r_i) B[i][c]
+//   P[((z 2 + 1) mpad + m) ldb + c] = sum_i sin(k_m.r_i) B[i][c]
+// v_mfma_f64_16x16x4f64 operands: lane l gives A[l % 16][l / 16] (momentum
+// l % 16 of the wave's 16, site l / 16 of the 4) and B[l / 16][l % 16];
+// it holds D[l / 16 + 4 v][l % 16] in element v of its accumulator.
+// pos (npad x 3), ks (mpad x 3) and B (npad x ldb) are zero-padded, so no
+// load needs a bound check; padded sites carry zero spins.  Within a chunk
+// the phases and B fragments of the next 4 sites are fetched before the
+// MFMAs of the current 4 are issued.
+template <int NCT, bool FAST>
+__global__ __launch_bounds__(256) void k_ssf_mfma(const double *__restrict__ pos, const double *__restrict__ ks,
+                                                  const double *__restrict__ B, int ldb, int mpad, int nchunks,
+                                                  int cps, double *__restrict__ P) {
+    constexpr int KC = SSF_KC, BN = 16 * NCT;
+    __shared__ double Bs[2][KC][BN];
+    __shared__ double Ps[2][KC][3];
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int m = blockIdx.x * 64 + w * 16 + (l & 15);
+    const double kx = ks[3 * m], ky = ks[3 * m + 1], kz = ks[3 * m + 2];
+    const int c0 = blockIdx.y * BN;
+    const int ch0 = blockIdx.z * cps;
+    const int ch1 = min(ch0 + cps, nchunks);  // the host makes every split non-empty
+    dbl4 ac[NCT], as[NCT];
+#pragma unroll
+    for (int t = 0; t < NCT; ++t) {
+        ac[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+        as[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    }
+    // staging: element e = tid + 256 q of a chunk is row e / BN, column e % BN
+    double stage[NCT];
+    double pstage = 0.0;
+    auto load = [&](int ch) __attribute__((always_inline)) {
+        const double *src = B + (int64_t)ch * KC * ldb + c0;
+#pragma unroll
+        for (int q = 0; q < NCT; ++q) {
+            const int e = tid + 256 * q;
+            stage[q] = src[(int64_t)(e / BN) * ldb + e % BN];
+        }
+        if (tid < 3 * KC) pstage = pos[(int64_t)ch * KC * 3 + tid];
+    };
+    auto store = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int q = 0; q < NCT; ++q) {
+            const int e = tid + 256 * q;
+            Bs[buf][e / BN][e % BN] = stage[q];
+        }
+        if (tid < 3 * KC) Ps[buf][tid / 3][tid % 3] = pstage;
+    };
+    // the A operands (cos, sin) and B fragments of sites 4 kk .. 4 kk + 3
+    auto fetch = [&](int buf, int kk, double *b, double *sn, double *cs) __attribute__((always_inline)) {
+        const int row = 4 * kk + (l >> 4);
+#pragma unroll
+        for (int t = 0; t < NCT; ++t) b[t] = Bs[buf][row][16 * t + (l & 15)];
+        phase<FAST>(kx * Ps[buf][row][0] + ky * Ps[buf][row][1] + kz * Ps[buf][row][2], sn, cs);
+    };
+    load(ch0);
+    store(0);
+    __syncthreads();
+    for (int ch = ch0; ch < ch1; ++ch) {
+        const int buf = (ch - ch0) & 1;
+        if (ch + 1 < ch1) load(ch + 1);
+        double bf[NCT], sn, cs;
+        fetch(buf, 0, bf, &sn, &cs);
+        if constexpr (FAST) {
+#pragma unroll
+            for (int kk = 0; kk < KC / 4; ++kk) {
+                // six segments, each with the MFMAs of NCT / 6 column tiles
+                // of these 4 sites, one stage of the phase of the next 4 and
+                // the reads of their B fragments; the scheduling barriers
+                // keep the stages between the MFMAs
+                const bool next = kk + 1 < KC / 4;
+                const int row = 4 * (kk + 1) + (l >> 4);
+                double bn[NCT], x = 0.0, n = 0.0, r = 0.0, z = 0.0, sp = 0.0, cp = 0.0, sv = 0.0, cv = 0.0;
+#pragma unroll
+                for (int seg = 0; seg < 6; ++seg) {
+#pragma unroll
+                    for (int t = seg * NCT / 6; t < (seg + 1) * NCT / 6; ++t) {
+                        ac[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(cs, bf[t], ac[t], 0, 0, 0);
+                        as[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(sn, bf[t], as[t], 0, 0, 0);
+                        if (next) bn[t] = Bs[buf][row][16 * t + (l & 15)];
+                    }
+                    if (next) {
+                        if (seg == 0) {
+                            x = kx * Ps[buf][row][0] + ky * Ps[buf][row][1] + kz * Ps[buf][row][2];
+                            n = __builtin_rint(x * 0x1.45f306dc9c883p-1);
+                        } else if (seg == 1) {
+                            r = __builtin_fma(-n, 0x1.921fb54400000p+0, x);
+                            r = __builtin_fma(-n, 0x1.0b4611a600000p-34, r);
+                            r = __builtin_fma(-n, 0x1.3198a2e037073p-69, r);
+                            z = r * r;
+                        } else if (seg == 2) {
+                            sp = __builtin_fma(-1.0 / 1307674368000.0, z, 1.0 / 6227020800.0);
+                            sp = __builtin_fma(sp, z, -1.0 / 39916800.0);
+                            sp = __builtin_fma(sp, z, 1.0 / 362880.0);
+                            cp = __builtin_fma(1.0 / 20922789888000.0, z, -1.0 / 87178291200.0);
+                            cp = __builtin_fma(cp, z, 1.0 / 479001600.0);
+                            cp = __builtin_fma(cp, z, -1.0 / 3628800.0);
+                        } else if (seg == 3) {
+                            sp = __builtin_fma(sp, z, -1.0 / 5040.0);
+                            sp = __builtin_fma(sp, z, 1.0 / 120.0);
+                            sp = __builtin_fma(sp, z, -1.0 / 6.0);
+                            cp = __builtin_fma(cp, z, 1.0 / 40320.0);
+                            cp = __builtin_fma(cp, z, -1.0 / 720.0);
+                        } else if (seg == 4) {
+                            cp = __builtin_fma(cp, z, 1.0 / 24.0);
+                            cp = __builtin_fma(cp, z, -0.5);
+                            sv = __builtin_fma(r * z, sp, r);
+                            cv = __builtin_fma(z, cp, 1.0);
+                        }
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (next) {
+                    // the quadrant (the same steps as sincos_reduced)
+                    const int q = (int)n & 3;
+                    const double s1 = (q & 1) ? cv : sv, c1 = (q & 1) ? sv : cv;
+                    sn = (q & 2) ? -s1 : s1;
+                    cs = ((q + 1) & 2) ? -c1 : c1;
+#pragma unroll
+                    for (int t = 0; t < NCT; ++t) bf[t] = bn[t];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < KC / 4; ++kk) {
+                double bn[NCT], sn2 = 0.0, cs2 = 0.0;
+                if (kk + 1 < KC / 4) fetch(buf, kk + 1, bn, &sn2, &cs2);
+#pragma unroll
+                for (int t = 0; t < NCT; ++t) {
+                    ac[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(cs, bf[t], ac[t], 0, 0, 0);
+                    as[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(sn, bf[t], as[t], 0, 0, 0);
+                }
+                if (kk + 1 < KC / 4) {
+#pragma unroll
+                    for (int t = 0; t < NCT; ++t) bf[t] = bn[t];
+                    sn = sn2;
+                    cs = cs2;
+                }
+            }
+        }
+        if (ch + 1 < ch1) store(buf ^ 1);
+        __syncthreads();
+    }
+    const int mrow = blockIdx.x * 64 + w * 16 + (l >> 4);
+    double *pc = P + ((int64_t)(2 * blockIdx.z) * mpad + mrow) * ldb + c0 + (l & 15);
+    double *ps = pc + (int64_t)mpad * ldb;
+#pragma unroll
+    for (int t = 0; t < NCT; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            pc[(int64_t)4 * v * ldb + 16 * t] = ac[t][v];
+            ps[(int64_t)4 * v * ldb + 16 * t] = as[t][v];
+        }
+}
+
+// out[r][k][3 a + b] = (Re_a Re_b + Im_a Im_b) / N, the split partial sums
+// added in split order; consecutive threads take consecutive systems, so
+// the partial sums are read in order of memory
+__global__ void k_ssf_finish(const double *P, int ksplit, int mpad, int ldb, int nk, int nsys, double inv_n,
+                             double *out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nk * nsys) return;
+    const int r = (int)(t % nsys), k = (int)(t / nsys);
+    double re[3] = {0.0, 0.0, 0.0}, im[3] = {0.0, 0.0, 0.0};
+    for (int z = 0; z < ksplit; ++z) {
+        const double *pc = P + ((int64_t)(2 * z) * mpad + k) * ldb + 3 * r;
+        const double *ps = pc + (int64_t)mpad * ldb;
+        for (int a = 0; a < 3; ++a) {
+            re[a] += pc[a];
+            im[a] += ps[a];
+        }
+    }
+    double *o = out + ((int64_t)r * nk + k) * 9;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) o[3 * a + b] = (re[a] * re[b] + im[a] * im[b]) * inv_n;
+}
+
+template <int NCT>
+static const void *mfma_kernel(bool fast) {
+    return fast ? (const void *)k_ssf_mfma<NCT, true> : (const void *)k_ssf_mfma<NCT, false>;
+}
+
+static const void *mfma_kernel(int nct, bool fast) {
+    switch (nct) {
+    case 1: return mfma_kernel<1>(fast);
+    case 2: return mfma_kernel<2>(fast);
+    case 3: return mfma_kernel<3>(fast);
+    case 4: return mfma_kernel<4>(fast);
+    case 6: return mfma_kernel<6>(fast);
+    case 8: return mfma_kernel<8>(fast);
+    default: return mfma_kernel<12>(fast);
+    }
+}
+
+void ssf_plan(int N, int nk, int nsys, int ncu, bool fast, SsfPlan *p) {
+    static const int ncts[] = {1, 2, 3, 4, 6, 8, 12};
+    const int cols = 3 * nsys;
+    int nct = 12;
+    for (int c : ncts)
+        if (16 * c >= cols) {
+            nct = c;
+            break;
+        }
+    p->N = N;
+    p->nk = nk;
+    p->nsys = nsys;
+    p->nct = nct;
+    p->fast = fast;
+    p->colblocks = (cols + 16 * nct - 1) / (16 * nct);
+    p->ldb = p->colblocks * 16 * nct;
+    p->mblocks = (nk + 63) / 64;
+    p->mpad = p->mblocks * 64;
+    p->nchunks = (N + SSF_KC - 1) / SSF_KC;
+    p->npad = p->nchunks * SSF_KC;
+    // split of the site sum: the fewest rounds of workgroups over the
+    // resident slots times the chunks per workgroup, plus the cost of writing
+    // and reading back one more set of partial sums
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mfma_kernel(nct, fast), 256, 0) != hipSuccess ||
+        occ < 1)
+        occ = 1;
+    const int64_t slots = (int64_t)(ncu > 0 ? ncu : 256) * occ;
+    const int64_t tiles = (int64_t)p->mblocks * p->colblocks;
+    const double t_chunk = (2.84 * nct + 1.0) * 1e-7 * occ;             // s per chunk per workgroup
+    const double t_split = 2.0 * 2.0 * p->mpad * p->ldb * 8.0 / 4e12;  // s per set of partials
+    const int64_t max_split = std::max<int64_t>(1, ((int64_t)1 << 27) / (2 * (int64_t)p->mpad * p->ldb));
+    double best = 1e300;
+    p->ksplit = 1;
+    p->cps = p->nchunks;
+    const int64_t ks_max = std::min<int64_t>(p->nchunks, std::min<int64_t>(256, max_split));
+    for (int ks = 1; ks <= ks_max; ++ks) {
+        const int cps = (p->nchunks + ks - 1) / ks;
+        const int kse = (p->nchunks + cps - 1) / cps;  // no empty split
+        const int64_t rounds = (tiles * kse + slots - 1) / slots;
+        const double t = rounds * cps * t_chunk + kse * t_split;
+        if (t < best * (1.0 - 1e-9)) {
+            best = t;
+            p->ksplit = kse;
+            p->cps = cps;
+        }
+    }
+}
+
+hipError_t launch_ssf_mfma(const SsfPlan &p, int prec, const void *spins, const SysDev *sys, const double *pos,
+                           const double *ks, double *B, double *P, double *out, hipStream_t st,
+                           hipEvent_t ev0, hipEvent_t ev1) {
+    const int64_t nb = (int64_t)p.npad * p.ldb;
+    if (prec == 32)
+        hipLaunchKernelGGL(k_ssf_spins_rows<float>, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, spins,
+                           sys, p.nsys, p.N, p.npad, p.ldb, B);
+    else
+        hipLaunchKernelGGL(k_ssf_spins_rows<double>, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st,
+                           spins, sys, p.nsys, p.N, p.npad, p.ldb, B);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
+    {
+        int ldb = p.ldb, mpad = p.mpad, nchunks = p.nchunks, cps = p.cps;
+        void *args[] = {(void *)&pos, (void *)&ks, (void *)&B, &ldb, &mpad, &nchunks, &cps, (void *)&P};
+        e = hipLaunchKernel(mfma_kernel(p.nct, p.fast), dim3(p.mblocks, p.colblocks, p.ksplit), dim3(256), args,
+                            0, st);
+        if (e != hipSuccess) return e;
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (ev1 && (e = hipEventRecord(ev1, st)) != hipSuccess) return e;
+    const int64_t nc = (int64_t)p.nk * p.nsys;
+    hipLaunchKernelGGL(k_ssf_finish, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, P, p.ksplit, p.mpad,
+                       p.ldb, p.nk, p.nsys, 1.0 / p.N, out);
+    return hipGetLastError();
+}
+
 static hipError_t blas_err(rocblas_status s) {
     return s == rocblas_status_success ? hipSuccess : hipErrorLaunchFailure;
 }
 
-hipError_t launch_ssf(void *blas, int prec, const void *spins, const SysDev *sys, int nsys, int N,
+hipError_t launch_ssf_blas(void *blas, int prec, const void *spins, const SysDev *sys, int nsys, int N,
                       const double *pos, const double *ks, int nk, int kb, double *C, double *Q,
                       double *B, double *Cre, double *Cim, double *out, hipStream_t st) {
     rocblas_handle h = (rocblas_handle)blas;
