@@ -14,6 +14,7 @@ from .lattice import (Lattice, compute_equal_time_correlations, energy, energy_d
                       get_magnetization, get_spin, local_fields, random_spin_orientation, set_spin,
                       site_energies, total_energy)
 from .reciprocal import get_allowed_wavevectors, get_k_path, get_k_plane, reciprocal
+from .spin_correlations import compute_equal_time_structure_factor, runEqualTimeStructureFactor
 from .montecarlo import (MCParamsBuffer, Metropolis, MetropolisAdaptive, MetropolisConstraint,
                          MetropolisConstraintAdaptive, MetropolisFixedCone, MonteCarlo,
                          SimulationParameters, deterministic_updates, overrelaxation,

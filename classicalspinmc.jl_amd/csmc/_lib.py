@@ -109,6 +109,8 @@ SIGNATURES = {
     "csmc_pt_split_parts": [_P, _I32, _P],
     "csmc_pt_observables": [_P, _I32, _P],
     "csmc_spin_correlations": [_P, _I32, _P, _I32, _P],
+    "csmc_pt_set_correlations": [_P, _I32, _P, _I32],
+    "csmc_pt_correlations": [_P, _I32, _P, _P],
     "csmc_energy_grad_spins": [_P, _I32, _P],
     "csmc_energy_grad_couplings": [_P, _I32, _P],
     "csmc_gradient_descent": [_P, _I32, _D, _I32],

@@ -165,6 +165,25 @@ class SpinBatch:
                                                      _lib.ptr(out)))
         return np.transpose(out, (0, 2, 1)).copy()
 
+    def pt_set_correlations(self, ks):
+        """measure compute_equal_time_correlations at the momenta ks (D x n_k)
+        at every probe sweep of the PT run (corr = true,
+        monte_carlo.jl:371-375); call after pt_init"""
+        ks = np.asarray(ks, dtype=np.float64)
+        if ks.ndim != 2:
+            raise ValueError("ks must be a D x n_k matrix")
+        kt = np.ascontiguousarray(ks.T)
+        _lib.check(_lib.lib().csmc_pt_set_correlations(self._h, self.id, _lib.ptr(kt), ks.shape[1]))
+        self._corr_nk = ks.shape[1]
+
+    def pt_correlations(self):
+        """(mean over probes per temperature slot, (n, 9, n_k); probe count)"""
+        nk = getattr(self, "_corr_nk", 0)
+        out = np.zeros((self.n, max(nk, 1), 9))
+        cnt = C.c_int64()
+        _lib.check(_lib.lib().csmc_pt_correlations(self._h, self.id, _lib.ptr(out), C.byref(cnt)))
+        return np.transpose(out, (0, 2, 1)).copy(), cnt.value
+
     def pt_split_parts(self):
         """workgroups per system of the chained PT kernel (1: whole system
         per workgroup; G > 1: strips with halos)"""

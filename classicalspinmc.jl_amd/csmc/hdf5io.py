@@ -324,6 +324,13 @@ def write_final_observables(mc):
             g.write("magnetization_err", o["magnetization_err"])
             g.write("energy", o["energy"])
             g.write("energy_err", o["energy_err"])
+        if mc.corr:
+            # Julia's 9 x n_k and D x n_k matrices, column-major, are the
+            # row-major n_k x 9 and n_k x D arrays written here
+            with f.group("spin_correlations", create=True) as g:
+                overwrite_keys(g, {"SSF": np.ascontiguousarray(np.asarray(mc.correlations).T),
+                                   "SSF_momentum": np.ascontiguousarray(np.asarray(mc.momentum_vectors).T)})
+            print(f"Writing spin correlations to {mc.outpath}")
 
 
 def overwrite_keys(fid, d):

@@ -110,8 +110,6 @@ class MonteCarlo:
             warnings.warn("No momentum vectors provided for correlation calculations!")
         elif not corr and ks is not None and np.size(ks) != 0:
             warnings.warn("Momentum vectors provided but correlation calculations not requested!")
-        if corr:
-            raise NotImplementedError("spin correlations (corr=true) are not part of this engine")
         self.T = float(T)
         self.lattice = copy.deepcopy(lattice)
         self.parameters = MCParamsBuffer(parameters)
@@ -121,8 +119,11 @@ class MonteCarlo:
         self.constraint = constraint
         self.sigma = sigma0
         self.sigma0 = sigma0
-        self.corr = corr
-        self.momentum_vectors = ks
+        self.corr = bool(corr)
+        D = lattice.unit_cell.D
+        self.momentum_vectors = (np.zeros((D, 0)) if ks is None or np.size(ks) == 0
+                                 else np.asarray(ks, dtype=np.float64).reshape(D, -1))
+        self.correlations = None
         self.precision = precision
         self.seed = int(seed)
         self.ctx = ctx
@@ -258,6 +259,12 @@ def parallel_tempering(mcs, saveIC=(), alg=None, verbose=True):
         b.set_spins(r, m.lattice._spins)
     params = p.as_dict()
     b.pt_init(temps, params, alg.kind, sigma0=mcs[0].sigma0, seed=mcs[0].seed)
+    # corr = true: S(k) at every probe sweep, averaged per temperature slot
+    # on the device (monte_carlo.jl:371-375)
+    ks = mcs[0].momentum_vectors
+    corr = mcs[0].corr and ks.shape[1] > 0
+    if corr:
+        b.pt_set_correlations(ks)
     total = p.t_thermalization + p.t_measurement
     out = any(m.outpath for m in mcs)
     if verbose:
@@ -282,11 +289,17 @@ def parallel_tempering(mcs, saveIC=(), alg=None, verbose=True):
             break
     st = b.pt_state()
     obs = b.pt_observables()
+    if corr:
+        cmean, ccount = b.pt_correlations()
     for r, m in enumerate(mcs):
         m.lattice._spins[:] = b.get_spins(int(st["system_of_slot"][r]))
         m.sigma = float(st["sigma"][r])
         m.observables = {k: float(v[r]) for k, v in obs.items()}
         m.pt_stats = {k: (v[r] if np.ndim(v) else v) for k, v in st.items()}
+        if m.corr:
+            # mean(mc.observables.correlations): 9 x n_k (9 x 0 without momenta)
+            m.correlations = cmean[r] if corr else np.zeros((9, 0))
+            m.correlation_count = ccount if corr else 0
     if out:
         from . import hdf5io
         for m in mcs:

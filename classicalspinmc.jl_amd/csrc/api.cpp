@@ -90,6 +90,12 @@ struct PTHost {
     DevBuf<int32_t> split_tab;
     DevBuf<uint64_t> split_hx, split_px;
     DevBuf<uint32_t> split_epoch;
+    // equal-time correlations at every probe (corr = true): the sum over
+    // probes per temperature slot, [R][corr_nk][9], and the probe count
+    int corr_nk = 0;
+    int64_t corr_count = 0;
+    SsfPlan corr_plan{};
+    DevBuf<double> corr_acc;
 };
 
 constexpr int PROBE_CAP = 4096;  // probes per slot held before a fold
@@ -1110,6 +1116,8 @@ int32_t csmc_pt_init(csmc_ctx *c, int32_t id, const double *temps, const csmc_pt
     pt.seed = seed;
     pt.sweep = 0;
     pt.mid = false;
+    pt.corr_nk = 0;
+    pt.corr_count = 0;
     pt.total = p->t_thermalization + p->t_measurement;
     pt.dosweep = p->overrelaxation_rate == 0 ? 1 : p->overrelaxation_rate;
     pt.temps.assign(temps, temps + b->nsys);
@@ -1174,6 +1182,20 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
         return p.checkpoint_rate > 0 && s >= p.t_thermalization && s % p.checkpoint_rate == 0;
     };
     auto report_at = [&](int64_t s1) { return p.report_interval > 0 && s1 % p.report_interval == 0; };
+    // correlations are measured at the probe sweeps, after the exchange
+    // (monte_carlo.jl:368-375)
+    auto corr_at = [&](int64_t s) {
+        return pt.corr_nk > 0 && p.measure && p.probe_rate > 0 && s >= p.t_thermalization &&
+               s % p.probe_rate == 0;
+    };
+    // S(k) of every system, added to the sum of the temperature slot it holds
+    auto corr_probe = [&]() -> int {
+        HIPCHK(launch_ssf_mfma(pt.corr_plan, b->prec, b->spins.p, b->sys_d.p, b->ssf_pos.p, b->ssf_ks.p,
+                               b->ssf_B.p, b->ssf_P.p, b->ssf_out.p, c->stream, nullptr, nullptr));
+        HIPCHK(launch_ssf_accumulate(b->ssf_out.p, pt.sys_of_slot.p, R, pt.corr_nk, pt.corr_acc.p, c->stream));
+        ++pt.corr_count;
+        return CSMC_OK;
+    };
     const int64_t max_seg = 2048;
     WGArgs a = wg_base(c, b, pt.seed);
     a.mode = WG_PT;
@@ -1302,7 +1324,7 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
         bool begin_mid = pt.mid, end_mid = false, exch = false;
         while (true) {
             bool head = !(s == pt.sweep && begin_mid);
-            if (head && (exchange_at(s) || checkpoint_at(s))) {
+            if (head && (exchange_at(s) || checkpoint_at(s) || corr_at(s))) {
                 end_mid = true;
                 exch = exchange_at(s);
                 break;
@@ -1310,7 +1332,7 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
             if (report_at(s + 1) || s + 1 >= pt.total || s + 1 >= until || s - pt.sweep + 1 >= max_seg) break;
             ++s;
         }
-        if (c->pt_chain && R > 1 && exch && !checkpoint_at(s)) {
+        if (c->pt_chain && R > 1 && exch && !checkpoint_at(s) && !corr_at(s)) {
             const int64_t np = probes_in(pt.sweep, s, true);
             const bool extends = nchain > 0 && chain_probes + np <= PROBE_CAP &&
                                  (nchain == 1 ? s > chain_e0 : s == chain_e0 + nchain * chain_P);
@@ -1349,6 +1371,10 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
         if (end_mid) {
             pt.sweep = s;
             pt.mid = true;
+            if (corr_at(s)) {
+                rc = corr_probe();
+                if (rc) return rc;
+            }
             if (checkpoint_at(s)) {
                 ev = 1;
                 break;
@@ -1461,15 +1487,18 @@ int32_t csmc_pt_observables(csmc_ctx *c, int32_t id, double *out) {
     return CSMC_OK;
 }
 
-int32_t csmc_spin_correlations(csmc_ctx *c, int32_t id, const double *ks, int32_t n_k, double *out) {
-    CHECK_CTX(c);
-    GET_BATCH(c, id, b);
-    if (n_k < 1 || !ks || !out) return fail(CSMC_EINVAL, "spin correlations need momenta and an output");
+}  // extern "C"
+
+namespace {
+
+// the positions and momenta of a correlation call on the device (3
+// components each, zero-padded to the plan's sizes) and the plan; every
+// system of the batch must be on one lattice
+int ssf_prepare(csmc_ctx *c, Batch *b, const double *ks, int n_k, bool for_blas, SsfPlan *plan) {
+    if (n_k < 1 || !ks) return fail(CSMC_EINVAL, "spin correlations need momenta");
     for (int s = 1; s < b->nsys; ++s)
         if (b->lat[s] != b->lat[0])
             return fail(CSMC_EINVAL, "spin correlations need the systems of the batch to share one lattice");
-    int rc = ensure_pools(c);
-    if (rc) return rc;
     const HostLattice &L = *c->lats[b->lat[0]];
     const int N = (int)L.N, D = L.dim, R = b->nsys;
     // site positions (compute_site_positions, lattice.jl:38-51);
@@ -1488,11 +1517,9 @@ int32_t csmc_spin_correlations(csmc_ctx *c, int32_t id, const double *ks, int32_
         if (!std::isfinite(k2)) return fail(CSMC_EINVAL, "momenta must be finite");
         kmax = std::max(kmax, std::sqrt(k2));
     }
-    SsfPlan plan;
-    ssf_plan(N, n_k, R, c->ncu, kmax * rmax * (1.0 + 1e-12) < SSF_FAST_ARG, &plan);
-    const int np = c->ssf_blas ? N : plan.npad, mp = c->ssf_blas ? n_k : plan.mpad;
-    // positions in device order and the momenta, 3 components each,
-    // zero-padded
+    ssf_plan(N, n_k, R, c->ncu, kmax * rmax * (1.0 + 1e-12) < SSF_FAST_ARG, plan);
+    const int np = for_blas ? N : plan->npad, mp = for_blas ? n_k : plan->mpad;
+    // positions in device order and the momenta, zero-padded
     std::vector<double> pdev((size_t)np * 3, 0.0), kv((size_t)mp * 3, 0.0);
     for (int d = 0; d < N; ++d)
         for (int q = 0; q < D; ++q) pdev[(size_t)d * 3 + q] = pref[(size_t)L.ref_of_dev[d] * D + q];
@@ -1503,6 +1530,29 @@ int32_t csmc_spin_correlations(csmc_ctx *c, int32_t id, const double *ks, int32_
     HIPCHK(b->ssf_out.reserve((size_t)R * n_k * 9));
     HIPCHK(hipMemcpyAsync(b->ssf_pos.p, pdev.data(), pdev.size() * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(b->ssf_ks.p, kv.data(), kv.size() * 8, hipMemcpyHostToDevice, c->stream));
+    // the host vectors go out of scope: wait for the copies
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!for_blas) {
+        HIPCHK(b->ssf_B.reserve((size_t)plan->npad * plan->ldb));
+        HIPCHK(b->ssf_P.reserve((size_t)plan->ksplit * 2 * plan->mpad * plan->ldb));
+    }
+    return CSMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t csmc_spin_correlations(csmc_ctx *c, int32_t id, const double *ks, int32_t n_k, double *out) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!out) return fail(CSMC_EINVAL, "spin correlations need an output");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    SsfPlan plan;
+    rc = ssf_prepare(c, b, ks, n_k, c->ssf_blas, &plan);
+    if (rc) return rc;
+    const int N = plan.N, R = b->nsys;
     // with timing on, the events bracket the dominant work: the MFMA kernel,
     // or the phases and GEMMs of the rocBLAS path
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1527,14 +1577,51 @@ int32_t csmc_spin_correlations(csmc_ctx *c, int32_t id, const double *ks, int32_
                                b->ssf_out.p, c->stream));
         if (e1) HIPCHK(hipEventRecord(e1, c->stream));
     } else {
-        HIPCHK(b->ssf_B.reserve((size_t)plan.npad * plan.ldb));
-        HIPCHK(b->ssf_P.reserve((size_t)plan.ksplit * 2 * plan.mpad * plan.ldb));
         HIPCHK(launch_ssf_mfma(plan, b->prec, b->spins.p, b->sys_d.p, b->ssf_pos.p, b->ssf_ks.p, b->ssf_B.p,
                                b->ssf_P.p, b->ssf_out.p, c->stream, e0, e1));
     }
     if (c->timing) c->events.push_back({e0, e1});
     HIPCHK(hipMemcpyAsync(out, b->ssf_out.p, (size_t)R * n_k * 9 * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_pt_set_correlations(csmc_ctx *c, int32_t id, const double *ks, int32_t n_k) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    PTHost &pt = b->pt;
+    if (!pt.init) return fail(CSMC_ESTATE, "csmc_pt_init was not called for this batch");
+    if (pt.sweep != 0 || pt.mid)
+        return fail(CSMC_ESTATE, "correlations must be set before the first csmc_pt_advance");
+    pt.corr_nk = 0;
+    pt.corr_count = 0;
+    if (n_k == 0) return CSMC_OK;
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    rc = ssf_prepare(c, b, ks, n_k, false, &pt.corr_plan);
+    if (rc) return rc;
+    HIPCHK(pt.corr_acc.alloc((size_t)b->nsys * n_k * 9));
+    HIPCHK(hipMemsetAsync(pt.corr_acc.p, 0, (size_t)b->nsys * n_k * 9 * 8, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    pt.corr_nk = n_k;
+    return CSMC_OK;
+}
+
+int32_t csmc_pt_correlations(csmc_ctx *c, int32_t id, double *mean, int64_t *count) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    PTHost &pt = b->pt;
+    if (!pt.init) return fail(CSMC_ESTATE, "csmc_pt_init was not called for this batch");
+    if (pt.corr_nk == 0) return fail(CSMC_ESTATE, "correlations were not requested (csmc_pt_set_correlations)");
+    const size_t n = (size_t)b->nsys * pt.corr_nk * 9;
+    if (mean) {
+        HIPCHK(hipMemcpyAsync(mean, pt.corr_acc.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        // the mean of the probes (an empty mean is NaN, as 0 / 0)
+        const double inv = pt.corr_count > 0 ? 1.0 / (double)pt.corr_count : std::nan("");
+        for (size_t i = 0; i < n; ++i) mean[i] = pt.corr_count > 0 ? mean[i] * inv : inv;
+    }
+    if (count) *count = pt.corr_count;
     return CSMC_OK;
 }
 

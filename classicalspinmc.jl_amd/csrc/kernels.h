@@ -164,6 +164,9 @@ void ssf_plan(int N, int nk, int nsys, int ncu, bool fast, SsfPlan *p);
 hipError_t launch_ssf_mfma(const SsfPlan &p, int prec, const void *spins, const SysDev *sys, const double *pos,
                            const double *ks, double *B, double *P, double *out, hipStream_t st,
                            hipEvent_t ev0, hipEvent_t ev1);
+// acc[T][k][j] += out[sys_of_slot[T]][k][j] for the R temperature slots
+hipError_t launch_ssf_accumulate(const double *out, const int32_t *sys_of_slot, int R, int nk, double *acc,
+                                 hipStream_t st);
 // the rocBLAS path: phase matrices in memory, two dgemm per block of momenta
 hipError_t launch_ssf_blas(void *blas, int prec, const void *spins, const SysDev *sys, int nsys, int N,
                       const double *pos, const double *ks, int nk, int kb, double *C, double *Q,

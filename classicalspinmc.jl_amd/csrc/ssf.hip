@@ -407,6 +407,22 @@ hipError_t launch_ssf_mfma(const SsfPlan &p, int prec, const void *spins, const 
     return hipGetLastError();
 }
 
+__global__ void k_ssf_accumulate(const double *out, const int32_t *sys_of_slot, int R, int nk, double *acc) {
+    const int64_t per = (int64_t)nk * 9;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * R) return;
+    const int T = (int)(t / per);
+    acc[t] += out[(int64_t)sys_of_slot[T] * per + t % per];
+}
+
+hipError_t launch_ssf_accumulate(const double *out, const int32_t *sys_of_slot, int R, int nk, double *acc,
+                                 hipStream_t st) {
+    const int64_t n = (int64_t)R * nk * 9;
+    hipLaunchKernelGGL(k_ssf_accumulate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, sys_of_slot, R,
+                       nk, acc);
+    return hipGetLastError();
+}
+
 static hipError_t blas_err(rocblas_status s) {
     return s == rocblas_status_success ? hipSuccess : hipErrorLaunchFailure;
 }

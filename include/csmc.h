@@ -239,9 +239,21 @@ int32_t csmc_pt_observables(csmc_ctx *ctx, int32_t batch_id, double *out);
  * reference's D x n_k momentum matrix (double[n_k][D]); out receives, per
  * system, its 9 x n_k Suv matrix as Julia lays it out (double[n_sys][n_k][9]:
  * xx xy xz yx yy yz zx zy zz per momentum), already divided by lat.size.  FP64
- * throughout (phases and two GEMMs per block of momenta). */
+ * throughout: one fused MFMA kernel (phases in registers), or with
+ * CSMC_SSF=blas set at csmc_ctx_create, phase matrices and rocBLAS GEMMs. */
 int32_t csmc_spin_correlations(csmc_ctx *ctx, int32_t batch_id, const double *ks, int32_t n_k,
                                double *out);
+
+/* MonteCarlo(...; corr = true, ks) for parallel tempering
+ * (monte_carlo.jl:61-81,371-375): after csmc_pt_init and before the first
+ * csmc_pt_advance, measure the equal-time correlations at ks (double[n_k][D])
+ * at every probe sweep, after that sweep's exchange, and add them to the sum
+ * of the temperature slot each system holds.  n_k = 0 turns it off. */
+int32_t csmc_pt_set_correlations(csmc_ctx *ctx, int32_t batch_id, const double *ks, int32_t n_k);
+/* mean(mc.observables.correlations) per temperature slot (hdf5.jl:229-233):
+ * mean[n_sys][n_k][9] (each slot's 9 x n_k Suv in Julia's layout; NaN before
+ * the first probe) and the number of probes. */
+int32_t csmc_pt_correlations(csmc_ctx *ctx, int32_t batch_id, double *mean, int64_t *count);
 
 /* ---- differentiation (no reference counterpart: adjoint of
  * total_energy, hamiltonian.jl:70-132) ------------------------------------ */

@@ -119,3 +119,52 @@ def test_k_path_along_a_direction_and_through_points():
     assert np.allclose(path[:, 0], [0, 0]) and np.allclose(path[:, -1], [np.pi, np.pi])
     steps = np.linalg.norm(np.diff(path, axis=1), axis=0)
     assert np.allclose(steps, 2 * np.pi / 8)
+
+
+def test_structure_factor_average_over_files(tmp_path):
+    """compute_equal_time_structure_factor (spin_correlations.jl:116-145):
+    the mean of every file's SSF (files in name order), written with the
+    first file's momenta to dest; the 9 x n_k matrices are stored as Julia
+    lays them out (row-major n_k x 9)"""
+    from csmc import hdf5io as H
+    from csmc.spin_correlations import compute_equal_time_structure_factor
+    rng = np.random.default_rng(1)
+    src = tmp_path / "ics"
+    src.mkdir()
+    ks = rng.normal(size=(2, 7))
+    mats = []
+    for i in range(4):
+        S = rng.normal(size=(9, 7))
+        mats.append(S)
+        with H.h5open(str(src / f"IC_{i}.h5"), "w") as f:
+            with f.group("spin_correlations", create=True) as g:
+                g.write("SSF", np.ascontiguousarray(S.T))
+                g.write("SSF_momentum", np.ascontiguousarray(ks.T))
+    dest = str(tmp_path / "dest.h5")
+    with H.h5open(dest, "w") as f:
+        f.write("spins", np.zeros((3, 3)))
+    mean, kout = compute_equal_time_structure_factor(str(src), dest)
+    ref = (mats[0] + mats[1] + mats[2] + mats[3]) / 4
+    assert np.allclose(mean, ref, rtol=1e-15, atol=0)
+    assert np.array_equal(kout, ks)
+    with H.h5open(dest, "r") as f:
+        assert np.allclose(f.read("spin_correlations/SSF"), ref.T, rtol=1e-15, atol=0)
+        assert np.array_equal(f.read("spin_correlations/SSF_momentum"), ks.T)
+    # a second run overwrites the keys in place
+    compute_equal_time_structure_factor(str(src), dest)
+    with H.h5open(dest, "r") as f:
+        assert np.allclose(f.read("spin_correlations/SSF"), ref.T, rtol=1e-15, atol=0)
+
+
+def test_montecarlo_corr_arguments():
+    """MonteCarlo(...; corr, ks) keeps the momenta as the D x n_k matrix and
+    warns as the reference does (monte_carlo.jl:64-68)"""
+    from csmc import Lattice, MonteCarlo
+    lat = Lattice((4, 4), W.square_heisenberg(-1.0, (0, 0, 0.1)), 1.0, seed=1)
+    ks = np.zeros((2, 5))
+    mc = MonteCarlo(1.0, lat, {"t_thermalization": 10}, corr=True, ks=ks)
+    assert mc.corr and mc.momentum_vectors.shape == (2, 5)
+    with pytest.warns(UserWarning, match="No momentum vectors"):
+        MonteCarlo(1.0, lat, {}, corr=True)
+    with pytest.warns(UserWarning, match="not requested"):
+        MonteCarlo(1.0, lat, {}, corr=False, ks=ks)
