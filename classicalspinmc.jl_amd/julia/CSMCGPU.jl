@@ -248,6 +248,14 @@ function parallel_tempering!(mcs::Vector{MonteCarlo}; alg::Int32=METROPOLIS,
     check(ccall((:csmc_pt_init, libcsmc), Int32,
                 (Ptr{Cvoid}, Int32, Ptr{Float64}, Ref{CPTParams}, Float64, UInt64),
                 ctx.h, b.id, temps, cp, Float64(mcs[1].sigma0), seed))
+    # corr = true (monte_carlo.jl:371-375): S(k) at every probe, on the device;
+    # the momenta go as double[n_k][D], i.e. permutedims of the D x n_k matrix
+    ks = mcs[1].momentum_vectors
+    corr = mcs[1].corr && size(ks, 2) > 0
+    if corr
+        check(ccall((:csmc_pt_set_correlations, libcsmc), Int32, (Ptr{Cvoid}, Int32, Ptr{Float64}, Int32),
+                    ctx.h, b.id, Matrix{Float64}(permutedims(ks)), size(ks, 2)))
+    end
     stopped, event = Ref{Int64}(0), Ref{Int32}(0)
     while event[] != 3
         check(ccall((:csmc_pt_advance, libcsmc), Int32,
@@ -267,7 +275,27 @@ function parallel_tempering!(mcs::Vector{MonteCarlo}; alg::Int32=METROPOLIS,
     for (r, mc) in enumerate(mcs)
         get_spins!(b, sys[r] + 1, mc.lattice.spins)
     end
+    if corr
+        # mean(mc.observables.correlations) of each slot: 9 x n_k
+        nk = size(ks, 2)
+        ssf = zeros(Float64, 9, nk, R)
+        count = Ref{Int64}(0)
+        check(ccall((:csmc_pt_correlations, libcsmc), Int32, (Ptr{Cvoid}, Int32, Ptr{Float64}, Ref{Int64}),
+                    ctx.h, b.id, ssf, count))
+        return obs, ssf
+    end
     return obs  # rows: C, dC, chi, dchi, <E>, dE, <|M|>, d|M| per temperature slot
+end
+
+"""compute_equal_time_correlations (spin_correlations.jl:6-43) of every
+system of `b` (one lattice): 9 x n_k x n_sys."""
+function compute_equal_time_correlations(b::Batch, ks::Matrix{Float64})
+    nk = size(ks, 2)
+    out = zeros(Float64, 9, nk, length(b.lattices))
+    check(ccall((:csmc_spin_correlations, libcsmc), Int32,
+                (Ptr{Cvoid}, Int32, Ptr{Float64}, Int32, Ptr{Float64}),
+                b.ctx.h, b.id, Matrix{Float64}(permutedims(ks)), nk, out))
+    return out
 end
 
 # ---- differentiation ------------------------------------------------------------
