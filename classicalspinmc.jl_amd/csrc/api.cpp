@@ -64,6 +64,13 @@ struct DevBuf {
     ~DevBuf() { free_(); }
 };
 
+// the inputs of a correlation call on the device: positions (in the order
+// the kernels visit the sites), the momenta and the device site of each row
+struct SsfBufs {
+    DevBuf<double> pos, ks;
+    DevBuf<int32_t> perm;
+};
+
 struct PTHost {
     bool init = false;
     csmc_pt_params p{};
@@ -95,6 +102,7 @@ struct PTHost {
     int corr_nk = 0;
     int64_t corr_count = 0;
     SsfPlan corr_plan{};
+    SsfBufs corr_in;
     DevBuf<double> corr_acc;
 };
 
@@ -119,7 +127,8 @@ struct Batch {
     DevBuf<uint32_t> accU;
     DevBuf<double> stage;          // host<->device staging (total*3 doubles)
     // equal-time correlations: device positions, momenta and GEMM workspaces
-    DevBuf<double> ssf_pos, ssf_ks, ssf_C, ssf_Q, ssf_B, ssf_Cre, ssf_Cim, ssf_P, ssf_out;
+    SsfBufs ssf_in;
+    DevBuf<double> ssf_C, ssf_Q, ssf_B, ssf_Cre, ssf_Cim, ssf_P, ssf_out;
     PTHost pt;
     size_t spin_bytes() const { return prec == 32 ? 16 : 32; }
 };
@@ -1190,8 +1199,8 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
     };
     // S(k) of every system, added to the sum of the temperature slot it holds
     auto corr_probe = [&]() -> int {
-        HIPCHK(launch_ssf_mfma(pt.corr_plan, b->prec, b->spins.p, b->sys_d.p, b->ssf_pos.p, b->ssf_ks.p,
-                               b->ssf_B.p, b->ssf_P.p, b->ssf_out.p, c->stream, nullptr, nullptr));
+        HIPCHK(launch_ssf_mfma(pt.corr_plan, b->prec, b->spins.p, b->sys_d.p, pt.corr_in.perm.p, pt.corr_in.pos.p,
+                               pt.corr_in.ks.p, b->ssf_B.p, b->ssf_P.p, b->ssf_out.p, c->stream, nullptr, nullptr));
         HIPCHK(launch_ssf_accumulate(b->ssf_out.p, pt.sys_of_slot.p, R, pt.corr_nk, pt.corr_acc.p, c->stream));
         ++pt.corr_count;
         return CSMC_OK;
@@ -1493,8 +1502,11 @@ namespace {
 
 // the positions and momenta of a correlation call on the device (3
 // components each, zero-padded to the plan's sizes) and the plan; every
-// system of the batch must be on one lattice
-int ssf_prepare(csmc_ctx *c, Batch *b, const double *ks, int n_k, bool for_blas, SsfPlan *plan) {
+// system of the batch must be on one lattice.  The MFMA path visits the
+// sites in the reference's order (lattice.jl:29-33, the last coordinate
+// fastest), where sites 4 apart in a chunk are usually one lattice step
+// apart; the rocBLAS path keeps the device order.
+int ssf_prepare(csmc_ctx *c, Batch *b, const double *ks, int n_k, bool for_blas, SsfPlan *plan, SsfBufs &in) {
     if (n_k < 1 || !ks) return fail(CSMC_EINVAL, "spin correlations need momenta");
     for (int s = 1; s < b->nsys; ++s)
         if (b->lat[s] != b->lat[0])
@@ -1517,19 +1529,46 @@ int ssf_prepare(csmc_ctx *c, Batch *b, const double *ks, int n_k, bool for_blas,
         if (!std::isfinite(k2)) return fail(CSMC_EINVAL, "momenta must be finite");
         kmax = std::max(kmax, std::sqrt(k2));
     }
-    ssf_plan(N, n_k, R, c->ncu, kmax * rmax * (1.0 + 1e-12) < SSF_FAST_ARG, plan);
+    const bool fast = kmax * rmax * (1.0 + 1e-12) < SSF_FAST_ARG;
+    // the step d between sites 4 apart, if every chunk of SSF_KC sites in
+    // reference order has it between all its sites 4 apart (to 1e-9 of the
+    // lattice's extent); then the phases rotate
+    double d[3] = {0.0, 0.0, 0.0};
+    bool regular = !for_blas && fast && N > 4;
+    if (regular) {
+        for (int q = 0; q < D; ++q) d[q] = pref[(size_t)4 * D + q] - pref[q];
+        const double tol = 1e-9 * (1.0 + rmax);
+        for (int i = 0; i < N && regular; ++i) {
+            if (i % SSF_KC < 4) continue;  // the first 4 of a chunk come from sincos
+            for (int q = 0; q < D; ++q)
+                if (std::fabs(pref[(size_t)i * D + q] - pref[(size_t)(i - 4) * D + q] - d[q]) > tol) {
+                    regular = false;
+                    break;
+                }
+        }
+    }
+    ssf_plan(N, n_k, R, c->ncu, fast ? (regular ? 2 : 1) : 0, plan);
+    for (int q = 0; q < 3; ++q) plan->d[q] = regular ? d[q] : 0.0;
     const int np = for_blas ? N : plan->npad, mp = for_blas ? n_k : plan->mpad;
-    // positions in device order and the momenta, zero-padded
-    std::vector<double> pdev((size_t)np * 3, 0.0), kv((size_t)mp * 3, 0.0);
-    for (int d = 0; d < N; ++d)
-        for (int q = 0; q < D; ++q) pdev[(size_t)d * 3 + q] = pref[(size_t)L.ref_of_dev[d] * D + q];
+    // positions in the visiting order and the momenta, zero-padded; perm[i]
+    // is the device site of row i
+    std::vector<double> pv((size_t)np * 3, 0.0), kv((size_t)mp * 3, 0.0);
+    std::vector<int32_t> perm(N);
+    for (int d0 = 0; d0 < N; ++d0) {
+        const int ref = L.ref_of_dev[d0];
+        const int row = for_blas ? d0 : ref;
+        perm[row] = d0;
+        for (int q = 0; q < D; ++q) pv[(size_t)row * 3 + q] = pref[(size_t)ref * D + q];
+    }
     for (int k = 0; k < n_k; ++k)
         for (int q = 0; q < D; ++q) kv[(size_t)k * 3 + q] = ks[(size_t)k * D + q];
-    HIPCHK(b->ssf_pos.reserve(pdev.size()));
-    HIPCHK(b->ssf_ks.reserve(kv.size()));
+    HIPCHK(in.pos.reserve(pv.size()));
+    HIPCHK(in.ks.reserve(kv.size()));
+    HIPCHK(in.perm.reserve(perm.size()));
     HIPCHK(b->ssf_out.reserve((size_t)R * n_k * 9));
-    HIPCHK(hipMemcpyAsync(b->ssf_pos.p, pdev.data(), pdev.size() * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(b->ssf_ks.p, kv.data(), kv.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(in.pos.p, pv.data(), pv.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(in.ks.p, kv.data(), kv.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(in.perm.p, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, c->stream));
     // the host vectors go out of scope: wait for the copies
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!for_blas) {
@@ -1550,7 +1589,7 @@ int32_t csmc_spin_correlations(csmc_ctx *c, int32_t id, const double *ks, int32_
     int rc = ensure_pools(c);
     if (rc) return rc;
     SsfPlan plan;
-    rc = ssf_prepare(c, b, ks, n_k, c->ssf_blas, &plan);
+    rc = ssf_prepare(c, b, ks, n_k, c->ssf_blas, &plan, b->ssf_in);
     if (rc) return rc;
     const int N = plan.N, R = b->nsys;
     // with timing on, the events bracket the dominant work: the MFMA kernel,
@@ -1572,13 +1611,13 @@ int32_t csmc_spin_correlations(csmc_ctx *c, int32_t id, const double *ks, int32_
         HIPCHK(b->ssf_B.reserve((size_t)3 * R * N));
         HIPCHK(b->ssf_Cre.reserve((size_t)3 * R * kb));
         HIPCHK(b->ssf_Cim.reserve((size_t)3 * R * kb));
-        HIPCHK(launch_ssf_blas(c->blas, b->prec, b->spins.p, b->sys_d.p, R, N, b->ssf_pos.p, b->ssf_ks.p, n_k,
+        HIPCHK(launch_ssf_blas(c->blas, b->prec, b->spins.p, b->sys_d.p, R, N, b->ssf_in.pos.p, b->ssf_in.ks.p, n_k,
                                kb, b->ssf_C.p, b->ssf_Q.p, b->ssf_B.p, b->ssf_Cre.p, b->ssf_Cim.p,
                                b->ssf_out.p, c->stream));
         if (e1) HIPCHK(hipEventRecord(e1, c->stream));
     } else {
-        HIPCHK(launch_ssf_mfma(plan, b->prec, b->spins.p, b->sys_d.p, b->ssf_pos.p, b->ssf_ks.p, b->ssf_B.p,
-                               b->ssf_P.p, b->ssf_out.p, c->stream, e0, e1));
+        HIPCHK(launch_ssf_mfma(plan, b->prec, b->spins.p, b->sys_d.p, b->ssf_in.perm.p, b->ssf_in.pos.p,
+                               b->ssf_in.ks.p, b->ssf_B.p, b->ssf_P.p, b->ssf_out.p, c->stream, e0, e1));
     }
     if (c->timing) c->events.push_back({e0, e1});
     HIPCHK(hipMemcpyAsync(out, b->ssf_out.p, (size_t)R * n_k * 9 * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1598,7 +1637,7 @@ int32_t csmc_pt_set_correlations(csmc_ctx *c, int32_t id, const double *ks, int3
     if (n_k == 0) return CSMC_OK;
     int rc = ensure_pools(c);
     if (rc) return rc;
-    rc = ssf_prepare(c, b, ks, n_k, false, &pt.corr_plan);
+    rc = ssf_prepare(c, b, ks, n_k, false, &pt.corr_plan, pt.corr_in);
     if (rc) return rc;
     HIPCHK(pt.corr_acc.alloc((size_t)b->nsys * n_k * 9));
     HIPCHK(hipMemsetAsync(pt.corr_acc.p, 0, (size_t)b->nsys * n_k * 9 * 8, c->stream));

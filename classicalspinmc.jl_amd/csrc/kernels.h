@@ -144,25 +144,28 @@ hipError_t launch_gd(int prec, const DevPools &P, void *spins, int nsys, int max
 // out[nsys][nk][9]
 // equal-time correlations, fused FP64 MFMA path: sizes of the padded
 // operands and of the split of the site sum
-constexpr int SSF_KC = 16;  // sites per LDS chunk
+constexpr int SSF_KC = 32;  // sites per LDS chunk
 // the fused path's own branch-free sincos is used when every |k.r| is
 // below this bound (OCML's sincos otherwise)
 constexpr double SSF_FAST_ARG = 1e5;
 struct SsfPlan {
     int N, nk, nsys;
-    bool fast;                // the branch-free sincos
+    int ph;                   // phases: 0 library sincos, 1 branch-free
+                              // sincos, 2 rotations by exp(i k.d)
+    double d[3];              // the step between sites 4 apart (ph 2)
     int nct;                  // 16-column tiles per wave (1, 2, 3, 4, 6, 8, 12)
     int colblocks, ldb;       // column blocks of 16 nct; ldb = colblocks 16 nct
     int mblocks, mpad;        // blocks of 64 momenta; mpad = 64 mblocks
     int nchunks, npad;        // chunks of SSF_KC sites; npad = SSF_KC nchunks
     int ksplit, cps;          // splits of the site sum, chunks per split
 };
-void ssf_plan(int N, int nk, int nsys, int ncu, bool fast, SsfPlan *p);
-// pos: npad x 3, ks: mpad x 3 (zero-padded); B: npad x ldb and
-// P: ksplit x 2 x mpad x ldb scratch; out: nsys x nk x 9; ev0 / ev1 (may be
-// null) are recorded around the MFMA kernel alone
-hipError_t launch_ssf_mfma(const SsfPlan &p, int prec, const void *spins, const SysDev *sys, const double *pos,
-                           const double *ks, double *B, double *P, double *out, hipStream_t st,
+void ssf_plan(int N, int nk, int nsys, int ncu, int ph, SsfPlan *p);
+// perm: N device site indices in the order of pos; pos: npad x 3, ks:
+// mpad x 3 (zero-padded); B: npad x ldb and P: ksplit x 2 x mpad x ldb
+// scratch; out: nsys x nk x 9; ev0 / ev1 (may be null) are recorded around
+// the MFMA kernel alone
+hipError_t launch_ssf_mfma(const SsfPlan &p, int prec, const void *spins, const SysDev *sys, const int32_t *perm,
+                           const double *pos, const double *ks, double *B, double *P, double *out, hipStream_t st,
                            hipEvent_t ev0, hipEvent_t ev1);
 // acc[T][k][j] += out[sys_of_slot[T]][k][j] for the R temperature slots
 hipError_t launch_ssf_accumulate(const double *out, const int32_t *sys_of_slot, int R, int nk, double *acc,

@@ -10,8 +10,10 @@
 //
 // Default path, k_ssf_mfma: one FP64 MFMA kernel.  Each wave owns 16
 // momenta and makes their phases in registers (FP64 sine and cosine of the
-// FP64 argument, one per lane per 4 sites: the A operand of both products), the
-// spins are staged through LDS in chunks of 16 sites shared by the 4 waves
+// FP64 argument, or rotations of the previous phase where the sites are
+// regularly spaced; one per lane per 4 sites: the A operand of both
+// products), the
+// spins are staged through LDS in chunks of SSF_KC sites shared by the 4 waves
 // (the B operand), and the N sum is split over workgroups to fill the
 // device; a small kernel sums the partial products in a fixed order and forms
 // the 9 correlations.  Nothing of size n_k x N ever reaches memory.
@@ -74,17 +76,17 @@ __global__ void k_ssf_combine(const double *Cre, const double *Cim, int kb, int 
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-// B[i * ldb + c] = component c % 3 of site i (device order) of system c / 3;
+// B[i * ldb + c] = component c % 3 of device site perm[i] of system c / 3;
 // zero for i >= N or c >= 3 nsys (the padding the MFMA tiles read)
 template <class R>
-__global__ void k_ssf_spins_rows(const void *spins, const SysDev *sys, int nsys, int N, int npad, int ldb,
-                                 double *B) {
+__global__ void k_ssf_spins_rows(const void *spins, const SysDev *sys, const int32_t *perm, int nsys, int N,
+                                 int npad, int ldb, double *B) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)npad * ldb) return;
     const int i = (int)(t / ldb), c = (int)(t % ldb);
     double v = 0.0;
     if (i < N && c < 3 * nsys) {
-        const spin4<R> s = reinterpret_cast<const spin4<R> *>(spins)[sys[c / 3].spin_off + i];
+        const spin4<R> s = reinterpret_cast<const spin4<R> *>(spins)[sys[c / 3].spin_off + perm[i]];
         const int a = c % 3;
         v = (double)(a == 0 ? s.x : a == 1 ? s.y : s.z);
     }
@@ -126,30 +128,73 @@ __device__ __forceinline__ void sincos_reduced(double x, double *sn, double *cs)
     *cs = ((q + 1) & 2) ? -cv : cv;
 }
 
-template <bool FAST>
-__device__ __forceinline__ void phase(double arg, double *sn, double *cs) {
-    if constexpr (FAST)
-        sincos_reduced(arg, sn, cs);
-    else
-        sincos(arg, sn, cs);
+// The branch-free sincos in six stages (the same operations in the same
+// order as sincos_reduced), so a caller can place them between MFMAs:
+// stage 0 takes x, stages 1-4 carry n, r, z and the two polynomials,
+// phase_finish applies the quadrant.
+__device__ __forceinline__ void phase_stage(int seg, double x, double &n, double &r, double &z, double &sp,
+                                            double &cp, double &sv, double &cv) {
+    if (seg == 0) {
+        n = __builtin_rint(x * 0x1.45f306dc9c883p-1);
+        r = __builtin_fma(-n, 0x1.921fb54400000p+0, x);
+    } else if (seg == 1) {
+        r = __builtin_fma(-n, 0x1.0b4611a600000p-34, r);
+        r = __builtin_fma(-n, 0x1.3198a2e037073p-69, r);
+        z = r * r;
+    } else if (seg == 2) {
+        sp = __builtin_fma(-1.0 / 1307674368000.0, z, 1.0 / 6227020800.0);
+        sp = __builtin_fma(sp, z, -1.0 / 39916800.0);
+        sp = __builtin_fma(sp, z, 1.0 / 362880.0);
+        cp = __builtin_fma(1.0 / 20922789888000.0, z, -1.0 / 87178291200.0);
+        cp = __builtin_fma(cp, z, 1.0 / 479001600.0);
+        cp = __builtin_fma(cp, z, -1.0 / 3628800.0);
+    } else if (seg == 3) {
+        sp = __builtin_fma(sp, z, -1.0 / 5040.0);
+        sp = __builtin_fma(sp, z, 1.0 / 120.0);
+        sp = __builtin_fma(sp, z, -1.0 / 6.0);
+        cp = __builtin_fma(cp, z, 1.0 / 40320.0);
+        cp = __builtin_fma(cp, z, -1.0 / 720.0);
+    } else if (seg == 4) {
+        cp = __builtin_fma(cp, z, 1.0 / 24.0);
+        cp = __builtin_fma(cp, z, -0.5);
+        sv = __builtin_fma(r * z, sp, r);
+        cv = __builtin_fma(z, cp, 1.0);
+    }
+}
+
+__device__ __forceinline__ void phase_finish(double n, double sv, double cv, double *sn, double *cs) {
+    const int q = (int)n & 3;
+    const double s1 = (q & 1) ? cv : sv, c1 = (q & 1) ? sv : cv;
+    *sn = (q & 2) ? -s1 : s1;
+    *cs = ((q + 1) & 2) ? -c1 : c1;
 }
 
 // Partial products of a block of 64 momenta (16 per wave) x 16 NCT columns
-// over the chunks [ch0, ch1) of 16 sites:
+// over the chunks [ch0, ch1) of SSF_KC sites:
 //   P[((z 2 + 0) mpad + m) ldb + c] = sum_i cos(k_m.r_i) B[i][c]
 //   P[((z 2 + 1) mpad + m) ldb + c] = sum_i sin(k_m.r_i) B[i][c]
 // v_mfma_f64_16x16x4f64 operands: lane l gives A[l % 16][l / 16] (momentum
 // l % 16 of the wave's 16, site l / 16 of the 4) and B[l / 16][l % 16];
 // it holds D[l / 16 + 4 v][l % 16] in element v of its accumulator.
 // pos (npad x 3), ks (mpad x 3) and B (npad x ldb) are zero-padded, so no
-// load needs a bound check; padded sites carry zero spins.  Within a chunk
-// the phases and B fragments of the next 4 sites are fetched before the
-// MFMAs of the current 4 are issued.
-template <int NCT, bool FAST>
+// load needs a bound check; padded sites carry zero spins.
+//
+// The phases of the lane's 4 sites at a time (PH):
+//   0  the library sincos for every 4 sites;
+//   1  the branch-free sincos for every 4 sites, its stages placed between
+//      the MFMAs of the previous 4 (all |k.r| < SSF_FAST_ARG);
+//   2  as 1, and every site 4 places later in the chunk is 4 places further
+//      along the same vector d (the host checked): the phases of sites 4,
+//      8 and 12 of a chunk are the previous ones rotated by exp(i k.d) (two
+//      multiplies and two multiply-adds), and the branch-free sincos of the
+//      next chunk's first 4 sites runs between the last MFMAs of the chunk.
+template <int NCT, int PH>
 __global__ __launch_bounds__(256) void k_ssf_mfma(const double *__restrict__ pos, const double *__restrict__ ks,
                                                   const double *__restrict__ B, int ldb, int mpad, int nchunks,
-                                                  int cps, double *__restrict__ P) {
-    constexpr int KC = SSF_KC, BN = 16 * NCT;
+                                                  int cps, double dx, double dy, double dz,
+                                                  double *__restrict__ P) {
+    constexpr int KC = SSF_KC, BN = 16 * NCT, NK = KC / 4, NST = KC * BN / 256;
+    static_assert(KC * BN % 256 == 0, "a chunk is a whole number of elements per thread");
     __shared__ double Bs[2][KC][BN];
     __shared__ double Ps[2][KC][3];
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
@@ -165,48 +210,93 @@ __global__ __launch_bounds__(256) void k_ssf_mfma(const double *__restrict__ pos
         as[t] = dbl4{0.0, 0.0, 0.0, 0.0};
     }
     // staging: element e = tid + 256 q of a chunk is row e / BN, column e % BN
-    double stage[NCT];
+    double stage[NST];
     double pstage = 0.0;
     auto load = [&](int ch) __attribute__((always_inline)) {
         const double *src = B + (int64_t)ch * KC * ldb + c0;
 #pragma unroll
-        for (int q = 0; q < NCT; ++q) {
+        for (int q = 0; q < NST; ++q) {
             const int e = tid + 256 * q;
             stage[q] = src[(int64_t)(e / BN) * ldb + e % BN];
         }
-        if (tid < 3 * KC) pstage = pos[(int64_t)ch * KC * 3 + tid];
+        if (PH != 2 && tid < 3 * KC) pstage = pos[(int64_t)ch * KC * 3 + tid];
     };
     auto store = [&](int buf) __attribute__((always_inline)) {
 #pragma unroll
-        for (int q = 0; q < NCT; ++q) {
+        for (int q = 0; q < NST; ++q) {
             const int e = tid + 256 * q;
             Bs[buf][e / BN][e % BN] = stage[q];
         }
-        if (tid < 3 * KC) Ps[buf][tid / 3][tid % 3] = pstage;
+        if (PH != 2 && tid < 3 * KC) Ps[buf][tid / 3][tid % 3] = pstage;
     };
-    // the A operands (cos, sin) and B fragments of sites 4 kk .. 4 kk + 3
-    auto fetch = [&](int buf, int kk, double *b, double *sn, double *cs) __attribute__((always_inline)) {
-        const int row = 4 * kk + (l >> 4);
-#pragma unroll
-        for (int t = 0; t < NCT; ++t) b[t] = Bs[buf][row][16 * t + (l & 15)];
-        phase<FAST>(kx * Ps[buf][row][0] + ky * Ps[buf][row][1] + kz * Ps[buf][row][2], sn, cs);
+    // the argument k.r of the lane's site among the first 4 of chunk ch
+    auto first_arg = [&](int ch) __attribute__((always_inline)) {
+        const double *pp = pos + ((int64_t)ch * KC + (l >> 4)) * 3;
+        return kx * pp[0] + ky * pp[1] + kz * pp[2];
     };
+    double cw = 1.0, sw = 0.0;  // exp(i k.d)
+    double cs = 0.0, sn = 0.0;  // PH 2: the phase of the lane's current site
+    if constexpr (PH == 2) {
+        sincos_reduced(kx * dx + ky * dy + kz * dz, &sw, &cw);
+        sincos_reduced(first_arg(ch0), &sn, &cs);
+    }
     load(ch0);
     store(0);
     __syncthreads();
     for (int ch = ch0; ch < ch1; ++ch) {
         const int buf = (ch - ch0) & 1;
         if (ch + 1 < ch1) load(ch + 1);
-        double bf[NCT], sn, cs;
-        fetch(buf, 0, bf, &sn, &cs);
-        if constexpr (FAST) {
+        double bf[NCT];
+        if constexpr (PH == 2) {
 #pragma unroll
-            for (int kk = 0; kk < KC / 4; ++kk) {
+            for (int t = 0; t < NCT; ++t) bf[t] = Bs[buf][l >> 4][16 * t + (l & 15)];
+            // the last chunk computes a phase it does not use (of a chunk in
+            // bounds)
+            const double xn = first_arg(min(ch + 1, nchunks - 1));
+            double cs_n = 0.0, sn_n = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < NK; ++kk) {
+                const bool next = kk + 1 < NK;
+                const int row = 4 * (kk + 1) + (l >> 4);
+                double bn[NCT], n = 0.0, r = 0.0, z = 0.0, sp = 0.0, cp = 0.0, sv = 0.0, cv = 0.0;
+#pragma unroll
+                for (int seg = 0; seg < 6; ++seg) {
+#pragma unroll
+                    for (int t = seg * NCT / 6; t < (seg + 1) * NCT / 6; ++t) {
+                        ac[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(cs, bf[t], ac[t], 0, 0, 0);
+                        as[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(sn, bf[t], as[t], 0, 0, 0);
+                        if (next) bn[t] = Bs[buf][row][16 * t + (l & 15)];
+                    }
+                    if (!next) phase_stage(seg, xn, n, r, z, sp, cp, sv, cv);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (next) {
+                    const double c1 = __builtin_fma(cs, cw, -(sn * sw));
+                    const double s1 = __builtin_fma(sn, cw, cs * sw);
+                    cs = c1;
+                    sn = s1;
+#pragma unroll
+                    for (int t = 0; t < NCT; ++t) bf[t] = bn[t];
+                } else {
+                    phase_finish(n, sv, cv, &sn_n, &cs_n);
+                }
+            }
+            cs = cs_n;
+            sn = sn_n;
+        } else if constexpr (PH == 1) {
+            {
+                const int row = l >> 4;
+#pragma unroll
+                for (int t = 0; t < NCT; ++t) bf[t] = Bs[buf][row][16 * t + (l & 15)];
+                sincos_reduced(kx * Ps[buf][row][0] + ky * Ps[buf][row][1] + kz * Ps[buf][row][2], &sn, &cs);
+            }
+#pragma unroll
+            for (int kk = 0; kk < NK; ++kk) {
                 // six segments, each with the MFMAs of NCT / 6 column tiles
                 // of these 4 sites, one stage of the phase of the next 4 and
                 // the reads of their B fragments; the scheduling barriers
                 // keep the stages between the MFMAs
-                const bool next = kk + 1 < KC / 4;
+                const bool next = kk + 1 < NK;
                 const int row = 4 * (kk + 1) + (l >> 4);
                 double bn[NCT], x = 0.0, n = 0.0, r = 0.0, z = 0.0, sp = 0.0, cp = 0.0, sv = 0.0, cv = 0.0;
 #pragma unroll
@@ -218,61 +308,28 @@ __global__ __launch_bounds__(256) void k_ssf_mfma(const double *__restrict__ pos
                         if (next) bn[t] = Bs[buf][row][16 * t + (l & 15)];
                     }
                     if (next) {
-                        if (seg == 0) {
-                            x = kx * Ps[buf][row][0] + ky * Ps[buf][row][1] + kz * Ps[buf][row][2];
-                            n = __builtin_rint(x * 0x1.45f306dc9c883p-1);
-                        } else if (seg == 1) {
-                            r = __builtin_fma(-n, 0x1.921fb54400000p+0, x);
-                            r = __builtin_fma(-n, 0x1.0b4611a600000p-34, r);
-                            r = __builtin_fma(-n, 0x1.3198a2e037073p-69, r);
-                            z = r * r;
-                        } else if (seg == 2) {
-                            sp = __builtin_fma(-1.0 / 1307674368000.0, z, 1.0 / 6227020800.0);
-                            sp = __builtin_fma(sp, z, -1.0 / 39916800.0);
-                            sp = __builtin_fma(sp, z, 1.0 / 362880.0);
-                            cp = __builtin_fma(1.0 / 20922789888000.0, z, -1.0 / 87178291200.0);
-                            cp = __builtin_fma(cp, z, 1.0 / 479001600.0);
-                            cp = __builtin_fma(cp, z, -1.0 / 3628800.0);
-                        } else if (seg == 3) {
-                            sp = __builtin_fma(sp, z, -1.0 / 5040.0);
-                            sp = __builtin_fma(sp, z, 1.0 / 120.0);
-                            sp = __builtin_fma(sp, z, -1.0 / 6.0);
-                            cp = __builtin_fma(cp, z, 1.0 / 40320.0);
-                            cp = __builtin_fma(cp, z, -1.0 / 720.0);
-                        } else if (seg == 4) {
-                            cp = __builtin_fma(cp, z, 1.0 / 24.0);
-                            cp = __builtin_fma(cp, z, -0.5);
-                            sv = __builtin_fma(r * z, sp, r);
-                            cv = __builtin_fma(z, cp, 1.0);
-                        }
+                        if (seg == 0) x = kx * Ps[buf][row][0] + ky * Ps[buf][row][1] + kz * Ps[buf][row][2];
+                        phase_stage(seg, x, n, r, z, sp, cp, sv, cv);
                     }
                     __builtin_amdgcn_sched_barrier(0);
                 }
                 if (next) {
-                    // the quadrant (the same steps as sincos_reduced)
-                    const int q = (int)n & 3;
-                    const double s1 = (q & 1) ? cv : sv, c1 = (q & 1) ? sv : cv;
-                    sn = (q & 2) ? -s1 : s1;
-                    cs = ((q + 1) & 2) ? -c1 : c1;
+                    phase_finish(n, sv, cv, &sn, &cs);
 #pragma unroll
                     for (int t = 0; t < NCT; ++t) bf[t] = bn[t];
                 }
             }
         } else {
 #pragma unroll
-            for (int kk = 0; kk < KC / 4; ++kk) {
-                double bn[NCT], sn2 = 0.0, cs2 = 0.0;
-                if (kk + 1 < KC / 4) fetch(buf, kk + 1, bn, &sn2, &cs2);
+            for (int kk = 0; kk < NK; ++kk) {
+                const int row = 4 * kk + (l >> 4);
+#pragma unroll
+                for (int t = 0; t < NCT; ++t) bf[t] = Bs[buf][row][16 * t + (l & 15)];
+                sincos(kx * Ps[buf][row][0] + ky * Ps[buf][row][1] + kz * Ps[buf][row][2], &sn, &cs);
 #pragma unroll
                 for (int t = 0; t < NCT; ++t) {
                     ac[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(cs, bf[t], ac[t], 0, 0, 0);
                     as[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(sn, bf[t], as[t], 0, 0, 0);
-                }
-                if (kk + 1 < KC / 4) {
-#pragma unroll
-                    for (int t = 0; t < NCT; ++t) bf[t] = bn[t];
-                    sn = sn2;
-                    cs = cs2;
                 }
             }
         }
@@ -314,23 +371,25 @@ __global__ void k_ssf_finish(const double *P, int ksplit, int mpad, int ldb, int
 }
 
 template <int NCT>
-static const void *mfma_kernel(bool fast) {
-    return fast ? (const void *)k_ssf_mfma<NCT, true> : (const void *)k_ssf_mfma<NCT, false>;
+static const void *mfma_kernel(int ph) {
+    return ph == 2   ? (const void *)k_ssf_mfma<NCT, 2>
+           : ph == 1 ? (const void *)k_ssf_mfma<NCT, 1>
+                     : (const void *)k_ssf_mfma<NCT, 0>;
 }
 
-static const void *mfma_kernel(int nct, bool fast) {
+static const void *mfma_kernel(int nct, int ph) {
     switch (nct) {
-    case 1: return mfma_kernel<1>(fast);
-    case 2: return mfma_kernel<2>(fast);
-    case 3: return mfma_kernel<3>(fast);
-    case 4: return mfma_kernel<4>(fast);
-    case 6: return mfma_kernel<6>(fast);
-    case 8: return mfma_kernel<8>(fast);
-    default: return mfma_kernel<12>(fast);
+    case 1: return mfma_kernel<1>(ph);
+    case 2: return mfma_kernel<2>(ph);
+    case 3: return mfma_kernel<3>(ph);
+    case 4: return mfma_kernel<4>(ph);
+    case 6: return mfma_kernel<6>(ph);
+    case 8: return mfma_kernel<8>(ph);
+    default: return mfma_kernel<12>(ph);
     }
 }
 
-void ssf_plan(int N, int nk, int nsys, int ncu, bool fast, SsfPlan *p) {
+void ssf_plan(int N, int nk, int nsys, int ncu, int ph, SsfPlan *p) {
     static const int ncts[] = {1, 2, 3, 4, 6, 8, 12};
     const int cols = 3 * nsys;
     int nct = 12;
@@ -343,7 +402,7 @@ void ssf_plan(int N, int nk, int nsys, int ncu, bool fast, SsfPlan *p) {
     p->nk = nk;
     p->nsys = nsys;
     p->nct = nct;
-    p->fast = fast;
+    p->ph = ph;
     p->colblocks = (cols + 16 * nct - 1) / (16 * nct);
     p->ldb = p->colblocks * 16 * nct;
     p->mblocks = (nk + 63) / 64;
@@ -354,8 +413,7 @@ void ssf_plan(int N, int nk, int nsys, int ncu, bool fast, SsfPlan *p) {
     // resident slots times the chunks per workgroup, plus the cost of writing
     // and reading back one more set of partial sums
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mfma_kernel(nct, fast), 256, 0) != hipSuccess ||
-        occ < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mfma_kernel(nct, ph), 256, 0) != hipSuccess || occ < 1)
         occ = 1;
     const int64_t slots = (int64_t)(ncu > 0 ? ncu : 256) * occ;
     const int64_t tiles = (int64_t)p->mblocks * p->colblocks;
@@ -379,24 +437,26 @@ void ssf_plan(int N, int nk, int nsys, int ncu, bool fast, SsfPlan *p) {
     }
 }
 
-hipError_t launch_ssf_mfma(const SsfPlan &p, int prec, const void *spins, const SysDev *sys, const double *pos,
-                           const double *ks, double *B, double *P, double *out, hipStream_t st,
+hipError_t launch_ssf_mfma(const SsfPlan &p, int prec, const void *spins, const SysDev *sys, const int32_t *perm,
+                           const double *pos, const double *ks, double *B, double *P, double *out, hipStream_t st,
                            hipEvent_t ev0, hipEvent_t ev1) {
     const int64_t nb = (int64_t)p.npad * p.ldb;
     if (prec == 32)
         hipLaunchKernelGGL(k_ssf_spins_rows<float>, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, spins,
-                           sys, p.nsys, p.N, p.npad, p.ldb, B);
+                           sys, perm, p.nsys, p.N, p.npad, p.ldb, B);
     else
         hipLaunchKernelGGL(k_ssf_spins_rows<double>, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st,
-                           spins, sys, p.nsys, p.N, p.npad, p.ldb, B);
+                           spins, sys, perm, p.nsys, p.N, p.npad, p.ldb, B);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
     {
         int ldb = p.ldb, mpad = p.mpad, nchunks = p.nchunks, cps = p.cps;
-        void *args[] = {(void *)&pos, (void *)&ks, (void *)&B, &ldb, &mpad, &nchunks, &cps, (void *)&P};
-        e = hipLaunchKernel(mfma_kernel(p.nct, p.fast), dim3(p.mblocks, p.colblocks, p.ksplit), dim3(256), args,
-                            0, st);
+        double dx = p.d[0], dy = p.d[1], dz = p.d[2];
+        void *args[] = {(void *)&pos, (void *)&ks, (void *)&B, &ldb, &mpad, &nchunks, &cps, &dx, &dy, &dz,
+                        (void *)&P};
+        e = hipLaunchKernel(mfma_kernel(p.nct, p.ph), dim3(p.mblocks, p.colblocks, p.ksplit), dim3(256), args, 0,
+                            st);
         if (e != hipSuccess) return e;
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -42,6 +42,8 @@ CASES = [  # name, cell, shape, S, precision, systems, momenta
     ("honeycomb_fp32_R4_N120", lambda: W.honeycomb_kitaev(1.0, -2.0), (6, 10), 0.5, 32, 4, "allowed"),
     ("honeycomb_fp32_R6_N70", lambda: W.honeycomb_kitaev(1.0, -2.0), (5, 7), 0.5, 32, 6, "allowed"),
     ("pyrochlore_fp32_R2", W.pyrochlore_mixed, (3, 3, 3), 1.0, 32, 2, "allowed"),
+    # rows of 32 sites: every chunk of 16 is regular, the phases rotate
+    ("honeycomb_fp32_R5_rotating", lambda: W.honeycomb_kitaev(1.0, -2.0), (16, 32), 0.5, 32, 5, "allowed"),
     ("square_fp64_R1", lambda: W.square_heisenberg(-1.0, (0, 0, 0.1)), (9, 9), 1.0, 64, 1, "allowed"),
     ("square_fp64_R2_2500k", lambda: W.square_heisenberg(-1.0, (0, 0, 0.1)), (16, 16), 1.0, 64, 2, "random"),
     ("square_fp64_R2_bigk", lambda: W.square_heisenberg(-1.0, (0, 0, 0.1)), (16, 16), 1.0, 64, 2, "big"),
