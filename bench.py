@@ -143,6 +143,44 @@ def fp32_error(csmc, W, lat, temps, ctx, R=8, steps=(1, 10)):
     return out
 
 
+def checkpoint_cost(csmc, b, temps):
+    """Cost of saving the whole PT state of the batch: a synchronous save,
+    and an asynchronous one queued ahead of the next chained launch (1024
+    exchange periods); the host time it exposes is the start call plus the
+    wait after that launch, and the launch's own time shows the copy's share
+    of the stream."""
+    # t_thermalization = 1: with 0, sweep 0 would be a checkpoint sweep
+    # (sweep >= t_thermalization and sweep % checkpoint_rate == 0) and the
+    # first advance would stop there
+    params = dict(t_thermalization=1, t_measurement=3 * 10240, probe_rate=10, swap_rate=10,
+                  overrelaxation_rate=10, report_interval=10 ** 9, checkpoint_rate=10 ** 9)
+    b.pt_init(temps, params, csmc.METROPOLIS, seed=3)
+    assert b.pt_advance(10240)[0] == 10240
+    t0 = time.perf_counter()
+    state = b.pt_save()
+    t_sync = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.pt_save_async()
+    t_start = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.pt_advance(2 * 10240)
+    b.sync()
+    t_with = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    same = b.pt_save_wait()
+    t_wait = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.pt_advance(3 * 10240)
+    b.sync()
+    t_without = time.perf_counter() - t0
+    assert len(same) == len(state)
+    return {"state_bytes": len(state), "sync_save_ms": round(t_sync * 1e3, 3),
+            "async_exposed_ms": round((t_start + t_wait) * 1e3, 3),
+            "launch_ms_with_copy": round(t_with * 1e3, 3), "launch_ms_without": round(t_without * 1e3, 3),
+            "note": "csmc_pt_save vs csmc_pt_save_async + csmc_pt_save_wait around one chained launch of "
+                    "1024 exchange periods"}
+
+
 def run_pt(args, W, csmc, uc, L, S, R, temps, precision, name):
     lat = csmc.Lattice((L, L) if uc.D == 2 else (L, L, L), uc, S, seed=2)
     N = lat.size
@@ -268,6 +306,8 @@ def main():
                    "storage_mode": r["mode"], "workgroups_per_replica": r["parts"]},
         "roofline": roofline,
     }
+    out["checkpoint"] = checkpoint_cost(csmc, r["b"], temps)
+    log(f"[bench] checkpoint: {out['checkpoint']}")
     r["b"].close()
     out["precision"] = fp32_error(csmc, W, r["lat"], temps, ctx)
 

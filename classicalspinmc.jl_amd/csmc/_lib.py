@@ -110,6 +110,10 @@ SIGNATURES = {
     "csmc_pt_observables": [_P, _I32, _P],
     "csmc_spin_correlations": [_P, _I32, _P, _I32, _P],
     "csmc_pt_set_correlations": [_P, _I32, _P, _I32],
+    "csmc_pt_save": [_P, _I32, _P, _P],
+    "csmc_pt_save_async": [_P, _I32],
+    "csmc_pt_save_wait": [_P, _I32, _P, _P],
+    "csmc_pt_load": [_P, _I32, _P, C.c_int64],
     "csmc_pt_correlations": [_P, _I32, _P, _P],
     "csmc_energy_grad_spins": [_P, _I32, _P],
     "csmc_energy_grad_couplings": [_P, _I32, _P],

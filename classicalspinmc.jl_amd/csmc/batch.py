@@ -184,6 +184,31 @@ class SpinBatch:
         _lib.check(_lib.lib().csmc_pt_correlations(self._h, self.id, _lib.ptr(out), C.byref(cnt)))
         return np.transpose(out, (0, 2, 1)).copy(), cnt.value
 
+    def pt_save(self):
+        """the whole PT state as bytes (csmc_pt_save)"""
+        n = C.c_int64()
+        _lib.check(_lib.lib().csmc_pt_save(self._h, self.id, None, C.byref(n)))
+        buf = np.zeros(n.value, dtype=np.uint8)
+        _lib.check(_lib.lib().csmc_pt_save(self._h, self.id, _lib.ptr(buf), C.byref(n)))
+        return buf[:n.value].tobytes()
+
+    def pt_save_async(self):
+        """start a save that overlaps the following pt_advance calls"""
+        _lib.check(_lib.lib().csmc_pt_save_async(self._h, self.id))
+
+    def pt_save_wait(self):
+        """the state captured by pt_save_async, as bytes"""
+        n = C.c_int64()
+        _lib.check(_lib.lib().csmc_pt_save_wait(self._h, self.id, None, C.byref(n)))
+        buf = np.zeros(n.value, dtype=np.uint8)
+        _lib.check(_lib.lib().csmc_pt_save_wait(self._h, self.id, _lib.ptr(buf), C.byref(n)))
+        return buf[:n.value].tobytes()
+
+    def pt_load(self, state):
+        """continue a saved run (after pt_init with the saved run's settings)"""
+        buf = np.frombuffer(bytes(state), dtype=np.uint8).copy()
+        _lib.check(_lib.lib().csmc_pt_load(self._h, self.id, _lib.ptr(buf), len(buf)))
+
     def pt_split_parts(self):
         """workgroups per system of the chained PT kernel (1: whole system
         per workgroup; G > 1: strips with halos)"""

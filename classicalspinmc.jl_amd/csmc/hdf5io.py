@@ -44,6 +44,8 @@ def lib():
             "h5c_gclose": ([I64], C.c_int), "h5c_delete": ([I64, S], C.c_int),
             "h5c_write_f64": ([I64, S, C.c_int, P, P], C.c_int),
             "h5c_write_i64": ([I64, S, C.c_int, P, P], C.c_int),
+            "h5c_write_u8": ([I64, S, C.c_int, P, P], C.c_int),
+            "h5c_read_u8": ([I64, S, P], C.c_int),
             "h5c_write_str": ([I64, S, S], C.c_int),
             "h5c_attr_f64": ([I64, S, C.c_double], C.c_int),
             "h5c_attr_i64": ([I64, S, I64], C.c_int),
@@ -120,7 +122,10 @@ class H5Node:
         a = np.asarray(value)
         shape = a.shape
         dims = np.array(list(shape) + [0], dtype=np.int64)  # never empty
-        if np.issubdtype(a.dtype, np.integer) or a.dtype == bool:
+        if a.dtype == np.uint8:
+            a = np.ascontiguousarray(a)
+            fn = lib().h5c_write_u8
+        elif np.issubdtype(a.dtype, np.integer) or a.dtype == bool:
             a = np.ascontiguousarray(a, dtype=np.int64)  # (ascontiguousarray makes 0-d 1-d)
             fn = lib().h5c_write_i64
         else:
@@ -139,7 +144,10 @@ class H5Node:
             _chk(lib().h5c_read_str(self.hid, _b(name), buf, n + 1), name)
             return buf.value.decode()
         shape = tuple(int(d) for d in dims[: rank.value])
-        if kind.value == 1:
+        if kind.value == 3:
+            out = np.zeros(shape, dtype=np.uint8)
+            _chk(lib().h5c_read_u8(self.hid, _b(name), out.ctypes.data), name)
+        elif kind.value == 1:
             out = np.zeros(shape, dtype=np.int64)
             _chk(lib().h5c_read_i64(self.hid, _b(name), out.ctypes.data), name)
         else:
@@ -284,6 +292,27 @@ def write_MC_checkpoint(mc):
     """write_MC_checkpoint (hdf5.jl:176-180): overwrite the spins dataset"""
     with h5open(mc.outpath, "r+") as f:
         f.write("spins", mc.lattice._spins)
+
+
+def write_resume_state(mc, state, sweep):
+    """the saved parallel-tempering state of the whole run (csmc_pt_save) in
+    the group `resume` of mc's file: `state` (bytes) and `sweep`; no
+    counterpart in the reference, whose checkpoints hold the spins only"""
+    with h5open(mc.outpath, "r+") as f:
+        with f.group("resume", create=True) as g:
+            g.write("state", np.frombuffer(state, dtype=np.uint8))
+            g.write("sweep", int(sweep))
+
+
+def read_resume_state(mc):
+    """(state bytes, sweep) from mc's file, or None"""
+    if not mc.outpath or not os.path.isfile(mc.outpath):
+        return None
+    with h5open(mc.outpath, "r") as f:
+        if not f.has("resume"):
+            return None
+        with f.group("resume") as g:
+            return g.read("state").tobytes(), int(g.read("sweep"))
 
 
 def write_initial_configuration(filename, mc):

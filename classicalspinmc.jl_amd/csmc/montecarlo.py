@@ -226,13 +226,19 @@ def _stats_line(mcs, t, total, stats, prev):
     prev["exch"] = stats["accepted_exchange"].copy()
 
 
-def parallel_tempering(mcs, saveIC=(), alg=None, verbose=True):
+def parallel_tempering(mcs, saveIC=(), alg=None, verbose=True, resume=False):
     """parallel_tempering! (monte_carlo.jl:235-398) for the replicas of one
     process.  `mcs` is one MonteCarlo per temperature (the reference's one
     per MPI rank; a single MonteCarlo runs without exchanges, as the
     reference does with commSize 1).  Replicas run as one device batch;
     after the call mcs[r] holds the configuration at temperature slot r and
-    the slot's observables."""
+    the slot's observables.
+
+    With output files, every checkpoint also saves the whole run (group
+    `resume` of the first replica's file; the copy to the host overlaps the
+    sweeps that follow), and resume=True continues from that save: the
+    result is the one an uninterrupted run gives.  The reference has no
+    resume."""
     single = isinstance(mcs, MonteCarlo)
     mcs = [mcs] if single else list(mcs)
     alg = alg or Metropolis()
@@ -267,6 +273,21 @@ def parallel_tempering(mcs, saveIC=(), alg=None, verbose=True):
         b.pt_set_correlations(ks)
     total = p.t_thermalization + p.t_measurement
     out = any(m.outpath for m in mcs)
+    from . import hdf5io
+    if resume:
+        saved = hdf5io.read_resume_state(mcs[0])
+        if saved is not None:
+            b.pt_load(saved[0])
+            if verbose:
+                print(f"Resuming at sweep {saved[1]} from {mcs[0].outpath}")
+    saving = [False]
+
+    def finish_save():
+        # the save started at the last checkpoint, written once it is on the host
+        if saving[0]:
+            state = b.pt_save_wait()
+            hdf5io.write_resume_state(mcs[0], state, saving[1])
+            saving[0] = False
     if verbose:
         print(f"Running sweeps on {datetime.datetime.now():%d %b %Y %H:%M:%S}.")
     prev = {"t": 0, "local": np.zeros(R), "exch": np.zeros(R)}
@@ -277,16 +298,19 @@ def parallel_tempering(mcs, saveIC=(), alg=None, verbose=True):
             for r, m in enumerate(mcs):
                 m.lattice._spins[:] = b.get_spins(int(st["system_of_slot"][r]))
                 if m.outpath:
-                    from . import hdf5io
                     hdf5io.write_MC_checkpoint(m)
                 if r in saveIC:
-                    from . import hdf5io
                     k = (stopped - p.t_thermalization) // p.checkpoint_rate
                     hdf5io.write_initial_configuration(hdf5io.ic_path(m, r, k), m)
+            if mcs[0].outpath:
+                finish_save()
+                b.pt_save_async()
+                saving[:] = [True, stopped]
         elif event == 2 and verbose:
             _stats_line(mcs, stopped, total, b.pt_state(), prev)
         if event == 3:
             break
+    finish_save()
     st = b.pt_state()
     obs = b.pt_observables()
     if corr:
@@ -301,7 +325,6 @@ def parallel_tempering(mcs, saveIC=(), alg=None, verbose=True):
             m.correlations = cmean[r] if corr else np.zeros((9, 0))
             m.correlation_count = ccount if corr else 0
     if out:
-        from . import hdf5io
         for m in mcs:
             if m.outpath:
                 hdf5io.write_final_observables(m)

@@ -64,6 +64,24 @@ struct DevBuf {
     ~DevBuf() { free_(); }
 };
 
+// page-locked host memory (hipHostMalloc), kept until the owner goes
+struct Pinned {
+    unsigned char *p = nullptr;
+    size_t n = 0;
+    hipError_t reserve(size_t count) {
+        if (p && n >= count) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipHostMalloc((void **)&p, count ? count : 1, hipHostMallocDefault);
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    ~Pinned() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 // the inputs of a correlation call on the device: positions (in the order
 // the kernels visit the sites), the momenta and the device site of each row
 struct SsfBufs {
@@ -104,6 +122,16 @@ struct PTHost {
     SsfPlan corr_plan{};
     SsfBufs corr_in;
     DevBuf<double> corr_acc;
+    // asynchronous save: the state copied to page-locked host memory in
+    // stream order; the host scalars of that moment in snap_head
+    Pinned snap_host;
+    std::vector<unsigned char> snap_head;
+    size_t snap_bytes = 0;
+    hipEvent_t snap_done = nullptr;
+    bool snap_pending = false;
+    ~PTHost() {
+        if (snap_done) (void)hipEventDestroy(snap_done);
+    }
 };
 
 constexpr int PROBE_CAP = 4096;  // probes per slot held before a fold
@@ -1121,6 +1149,7 @@ int32_t csmc_pt_init(csmc_ctx *c, int32_t id, const double *temps, const csmc_pt
     int rc = ensure_pools(c);
     if (rc) return rc;
     PTHost &pt = b->pt;
+    if (pt.snap_pending) return fail(CSMC_ESTATE, "a save is in flight (csmc_pt_save_wait first)");
     pt.p = *p;
     pt.seed = seed;
     pt.sweep = 0;
@@ -1407,6 +1436,195 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
     }
     if (stopped) *stopped = pt.sweep;
     if (event) *event = (pt.sweep >= pt.total && !pt.mid) ? 3 : ev;
+    return CSMC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Saved parallel-tempering state: a header, the temperatures, a table of
+// section sizes and the sections, which are the device buffers that carry
+// the run from one csmc_pt_advance to the next, in a fixed order.
+constexpr char PT_MAGIC[8] = {'C', 'S', 'M', 'C', 'P', 'T', 'S', '1'};
+constexpr int PT_NSEC = 17;
+
+struct PtSaveHead {
+    char magic[8];
+    int64_t version, nsys, prec, nsec;
+    int64_t sweep, mid, probe_pos, corr_nk, corr_count, split_G;
+    uint64_t seed;
+    csmc_pt_params params;
+};
+
+struct SecRef {
+    void *p;
+    size_t bytes;
+};
+
+template <class T>
+SecRef sec(DevBuf<T> &d) {
+    return {d.p, d.n * sizeof(T)};
+}
+
+std::vector<SecRef> pt_sections(Batch *b) {
+    PTHost &pt = b->pt;
+    SecRef corr = pt.corr_nk > 0 ? SecRef{pt.corr_acc.p, (size_t)b->nsys * pt.corr_nk * 9 * 8} : SecRef{nullptr, 0};
+    return {sec(b->spins),     sec(b->mcount),     sec(pt.E),         sec(pt.sigma),      sec(pt.acc_local),
+            sec(pt.acc_exch),  sec(pt.bins),       sec(pt.slot_of_sys), sec(pt.sys_of_slot), sec(pt.done),
+            sec(pt.rec),       sec(pt.err),        sec(pt.series),    sec(pt.split_hx),   sec(pt.split_px),
+            sec(pt.split_epoch), corr};
+}
+
+std::vector<unsigned char> pt_head(Batch *b) {
+    PTHost &pt = b->pt;
+    PtSaveHead h{};
+    std::memcpy(h.magic, PT_MAGIC, 8);
+    h.version = 1;
+    h.nsys = b->nsys;
+    h.prec = b->prec;
+    h.nsec = PT_NSEC;
+    h.sweep = pt.sweep;
+    h.mid = pt.mid;
+    h.probe_pos = pt.probe_pos;
+    h.corr_nk = pt.corr_nk;
+    h.corr_count = pt.corr_count;
+    h.split_G = pt.split_G;
+    h.seed = pt.seed;
+    h.params = pt.p;
+    std::vector<SecRef> secs = pt_sections(b);
+    std::vector<unsigned char> out(sizeof h + (size_t)b->nsys * 8 + PT_NSEC * 8);
+    std::memcpy(out.data(), &h, sizeof h);
+    std::memcpy(out.data() + sizeof h, pt.temps.data(), (size_t)b->nsys * 8);
+    for (int k = 0; k < PT_NSEC; ++k) {
+        const int64_t n = (int64_t)secs[k].bytes;
+        std::memcpy(out.data() + sizeof h + (size_t)b->nsys * 8 + 8 * k, &n, 8);
+    }
+    return out;
+}
+
+size_t sections_bytes(const std::vector<SecRef> &secs) {
+    size_t n = 0;
+    for (const SecRef &r : secs) n += r.bytes;
+    return n;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t csmc_pt_save(csmc_ctx *c, int32_t id, void *buf, int64_t *bytes) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    PTHost &pt = b->pt;
+    if (!pt.init) return fail(CSMC_ESTATE, "csmc_pt_init was not called for this batch");
+    if (!bytes) return fail(CSMC_EINVAL, "null size");
+    std::vector<unsigned char> head = pt_head(b);
+    std::vector<SecRef> secs = pt_sections(b);
+    const size_t total = head.size() + sections_bytes(secs);
+    if (!buf) {
+        *bytes = (int64_t)total;
+        return CSMC_OK;
+    }
+    if (*bytes < (int64_t)total) return fail(CSMC_EINVAL, "save buffer too small");
+    unsigned char *o = (unsigned char *)buf;
+    std::memcpy(o, head.data(), head.size());
+    size_t off = head.size();
+    for (const SecRef &r : secs) {
+        if (r.bytes) HIPCHK(hipMemcpyAsync(o + off, r.p, r.bytes, hipMemcpyDeviceToHost, c->stream));
+        off += r.bytes;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *bytes = (int64_t)total;
+    return CSMC_OK;
+}
+
+int32_t csmc_pt_save_async(csmc_ctx *c, int32_t id) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    PTHost &pt = b->pt;
+    if (!pt.init) return fail(CSMC_ESTATE, "csmc_pt_init was not called for this batch");
+    if (pt.snap_pending) return fail(CSMC_ESTATE, "a save is in flight (csmc_pt_save_wait first)");
+    if (!pt.snap_done) HIPCHK(hipEventCreateWithFlags(&pt.snap_done, hipEventDisableTiming));
+    std::vector<SecRef> secs = pt_sections(b);
+    const size_t total = sections_bytes(secs);
+    HIPCHK(pt.snap_host.reserve(total));
+    // DMA into page-locked memory, queued in order with the sweeps: the
+    // launches enqueued next run after it, and the host does not wait (a
+    // second stream for the copy made the following cooperative launch of
+    // the chained kernel take twice as long)
+    size_t off = 0;
+    for (const SecRef &r : secs) {
+        if (r.bytes) HIPCHK(hipMemcpyAsync(pt.snap_host.p + off, r.p, r.bytes, hipMemcpyDeviceToHost, c->stream));
+        off += r.bytes;
+    }
+    HIPCHK(hipEventRecord(pt.snap_done, c->stream));
+    pt.snap_head = pt_head(b);
+    pt.snap_bytes = total;
+    pt.snap_pending = true;
+    return CSMC_OK;
+}
+
+int32_t csmc_pt_save_wait(csmc_ctx *c, int32_t id, void *buf, int64_t *bytes) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    PTHost &pt = b->pt;
+    if (!pt.snap_pending) return fail(CSMC_ESTATE, "no save in flight (csmc_pt_save_async)");
+    if (!bytes) return fail(CSMC_EINVAL, "null size");
+    const size_t total = pt.snap_head.size() + pt.snap_bytes;
+    if (!buf) {
+        *bytes = (int64_t)total;
+        return CSMC_OK;
+    }
+    if (*bytes < (int64_t)total) return fail(CSMC_EINVAL, "save buffer too small");
+    HIPCHK(hipEventSynchronize(pt.snap_done));
+    std::memcpy(buf, pt.snap_head.data(), pt.snap_head.size());
+    std::memcpy((unsigned char *)buf + pt.snap_head.size(), pt.snap_host.p, pt.snap_bytes);
+    pt.snap_pending = false;
+    *bytes = (int64_t)total;
+    return CSMC_OK;
+}
+
+int32_t csmc_pt_load(csmc_ctx *c, int32_t id, const void *buf, int64_t bytes) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    PTHost &pt = b->pt;
+    if (!pt.init) return fail(CSMC_ESTATE, "csmc_pt_init was not called for this batch");
+    if (pt.snap_pending) return fail(CSMC_ESTATE, "a save is in flight (csmc_pt_save_wait first)");
+    if (!buf || bytes < (int64_t)sizeof(PtSaveHead)) return fail(CSMC_EINVAL, "saved state too short");
+    const unsigned char *in = (const unsigned char *)buf;
+    PtSaveHead h;
+    std::memcpy(&h, in, sizeof h);
+    if (std::memcmp(h.magic, PT_MAGIC, 8) != 0 || h.version != 1 || h.nsec != PT_NSEC)
+        return fail(CSMC_EINVAL, "not a saved parallel-tempering state of this library version");
+    // the run must be set up as the saved one was
+    std::vector<unsigned char> mine = pt_head(b);
+    PtSaveHead m;
+    std::memcpy(&m, mine.data(), sizeof m);
+    if (h.nsys != m.nsys || h.prec != m.prec)
+        return fail(CSMC_EINVAL, "saved state: another number of systems or precision");
+    if (h.seed != m.seed || std::memcmp(&h.params, &m.params, sizeof h.params) != 0)
+        return fail(CSMC_EINVAL, "saved state: another seed or other parameters at csmc_pt_init");
+    if (h.corr_nk != m.corr_nk) return fail(CSMC_EINVAL, "saved state: other correlation momenta");
+    if (h.split_G != m.split_G) return fail(CSMC_EINVAL, "saved state: another split of the replicas");
+    const size_t fixed = sizeof h + (size_t)h.nsys * 8 + PT_NSEC * 8;
+    if ((size_t)bytes < fixed || std::memcmp(in + sizeof h, mine.data() + sizeof m, (size_t)h.nsys * 8) != 0)
+        return fail(CSMC_EINVAL, "saved state: other temperatures");
+    if (std::memcmp(in + sizeof h + (size_t)h.nsys * 8, mine.data() + sizeof m + (size_t)m.nsys * 8,
+                    PT_NSEC * 8) != 0)
+        return fail(CSMC_EINVAL, "saved state: buffers of other sizes (another lattice or batch)");
+    std::vector<SecRef> secs = pt_sections(b);
+    if ((size_t)bytes != fixed + sections_bytes(secs)) return fail(CSMC_EINVAL, "saved state: wrong length");
+    size_t off = fixed;
+    for (const SecRef &r : secs) {
+        if (r.bytes) HIPCHK(hipMemcpyAsync(r.p, in + off, r.bytes, hipMemcpyHostToDevice, c->stream));
+        off += r.bytes;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    pt.sweep = h.sweep;
+    pt.mid = h.mid != 0;
+    pt.probe_pos = h.probe_pos;
+    pt.corr_count = h.corr_count;
     return CSMC_OK;
 }
 

@@ -63,16 +63,17 @@ static hid_t str_type(size_t len)
     return t;
 }
 
-/* write a dataset (kind 0 float64, 1 int64); an existing dataset of the same
- * shape is overwritten in place (write_MC_checkpoint), otherwise replaced */
+/* write a dataset (kind 0 float64, 1 int64, 2 uint8); an existing dataset of
+ * the same shape is overwritten in place (write_MC_checkpoint), otherwise
+ * replaced */
 static int write_ds(int64_t loc, const char *name, int kind, int rank, const int64_t *dims,
                     const void *data)
 {
     init();
     hsize_t hd[8];
     for (int r = 0; r < rank; ++r) hd[r] = (hsize_t)dims[r];
-    hid_t mt = kind ? H5T_NATIVE_INT64 : H5T_NATIVE_DOUBLE;
-    hid_t ft = kind ? H5T_STD_I64LE : H5T_IEEE_F64LE;
+    hid_t mt = kind == 2 ? H5T_NATIVE_UINT8 : kind ? H5T_NATIVE_INT64 : H5T_NATIVE_DOUBLE;
+    hid_t ft = kind == 2 ? H5T_STD_U8LE : kind ? H5T_STD_I64LE : H5T_IEEE_F64LE;
     if (H5Lexists(loc, name, H5P_DEFAULT) > 0) {
         hid_t ds = H5Dopen2(loc, name, H5P_DEFAULT);
         if (ds >= 0) {
@@ -109,6 +110,11 @@ int h5c_write_f64(int64_t loc, const char *name, int rank, const int64_t *dims, 
 int h5c_write_i64(int64_t loc, const char *name, int rank, const int64_t *dims, const int64_t *d)
 {
     return write_ds(loc, name, 1, rank, dims, d);
+}
+
+int h5c_write_u8(int64_t loc, const char *name, int rank, const int64_t *dims, const uint8_t *d)
+{
+    return write_ds(loc, name, 2, rank, dims, d);
 }
 
 int h5c_write_str(int64_t loc, const char *name, const char *s)
@@ -156,8 +162,9 @@ int h5c_attr_str(int64_t loc, const char *name, const char *s)
     return r;
 }
 
-/* class of a dataset: 0 float, 1 integer, 2 string, -1 missing; rank and
- * dims (up to 8) on disk; for strings dims[0] = the string size */
+/* class of a dataset: 0 float, 1 integer, 2 string, 3 unsigned 8-bit
+ * integer, -1 missing; rank and dims (up to 8) on disk; for strings dims[0] =
+ * the string size */
 int h5c_info(int64_t loc, const char *name, int *kind, int *rank, int64_t *dims)
 {
     init();
@@ -175,6 +182,8 @@ int h5c_info(int64_t loc, const char *name, int *kind, int *rank, int64_t *dims)
     if (c == H5T_STRING) {
         *kind = 2;
         dims[0] = H5Tis_variable_str(t) ? -1 : (int64_t)H5Tget_size(t);
+    } else if (c == H5T_INTEGER && H5Tget_size(t) == 1 && H5Tget_sign(t) == H5T_SGN_NONE) {
+        *kind = 3;
     } else {
         *kind = c == H5T_INTEGER ? 1 : 0;
     }
@@ -202,6 +211,11 @@ int h5c_read_f64(int64_t loc, const char *name, double *out)
 int h5c_read_i64(int64_t loc, const char *name, int64_t *out)
 {
     return read_ds(loc, name, H5T_NATIVE_INT64, out);
+}
+
+int h5c_read_u8(int64_t loc, const char *name, uint8_t *out)
+{
+    return read_ds(loc, name, H5T_NATIVE_UINT8, out);
 }
 
 /* string dataset (fixed or variable length) into buf (NUL-terminated) */

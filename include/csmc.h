@@ -230,6 +230,30 @@ int32_t csmc_pt_state(csmc_ctx *ctx, int32_t batch_id, int32_t *system_of_slot, 
  * planner decides) or 2..8 (forces G).  No counterpart in the reference (its
  * replicas are MPI ranks). */
 int32_t csmc_pt_split_parts(csmc_ctx *ctx, int32_t batch_id, int32_t *parts);
+/* ---- resume (no reference counterpart: the reference's checkpoints,
+ * write_MC_checkpoint hdf5.jl:176-180, hold the spins only and cannot
+ * restart a run) ---------------------------------------------------------- */
+/* The whole state of a parallel-tempering run between two csmc_pt_advance
+ * calls (spins, Metropolis counters, slot maps, energies, cone widths,
+ * acceptance sums, binners, probe series, exchange records, correlation sums
+ * and the sweep position) as one opaque buffer.  buf = NULL returns the size
+ * in *bytes; otherwise *bytes is the buffer's capacity on entry and the
+ * length written on return. */
+int32_t csmc_pt_save(csmc_ctx *ctx, int32_t batch_id, void *buf, int64_t *bytes);
+/* The same state taken without waiting: its copy to page-locked host memory
+ * is queued on the context's stream after the launches so far (later
+ * csmc_pt_advance launches queue behind it) and the call returns at once.
+ * csmc_pt_save_wait waits for that copy and returns the buffer (buf = NULL:
+ * its size); one save at a time per batch. */
+int32_t csmc_pt_save_async(csmc_ctx *ctx, int32_t batch_id);
+int32_t csmc_pt_save_wait(csmc_ctx *ctx, int32_t batch_id, void *buf, int64_t *bytes);
+/* Continue a saved run: after csmc_pt_init (and csmc_pt_set_correlations)
+ * with the temperatures, parameters, seed and momenta of the saved run on a
+ * batch of the same lattice, size and precision, restore the state; the
+ * following csmc_pt_advance calls continue the run as if it had never
+ * stopped.  A state that does not match fails with CSMC_EINVAL. */
+int32_t csmc_pt_load(csmc_ctx *ctx, int32_t batch_id, const void *buf, int64_t bytes);
+
 /* Final observables per slot (hdf5.jl:204-239 / observables.jl:36-63):
  * out[n_sys][8] = C, dC, chi, dchi, <E>, dE, <|M|>, d|M| (absolute errors). */
 int32_t csmc_pt_observables(csmc_ctx *ctx, int32_t batch_id, double *out);
