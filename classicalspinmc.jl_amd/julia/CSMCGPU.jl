@@ -298,6 +298,24 @@ function compute_equal_time_correlations(b::Batch, ks::Matrix{Float64})
     return out
 end
 
+# ---- resume (no reference counterpart) ---------------------------------------------
+
+"""The whole state of a PT run on batch `b` between two csmc_pt_advance calls."""
+function pt_save(b::Batch)
+    n = Ref{Int64}(0)
+    check(ccall((:csmc_pt_save, libcsmc), Int32, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Ref{Int64}),
+                b.ctx.h, b.id, C_NULL, n))
+    buf = Vector{UInt8}(undef, n[])
+    check(ccall((:csmc_pt_save, libcsmc), Int32, (Ptr{Cvoid}, Int32, Ptr{UInt8}, Ref{Int64}),
+                b.ctx.h, b.id, buf, n))
+    return buf
+end
+
+"""Continue a saved run, after csmc_pt_init with the saved run's settings."""
+pt_load!(b::Batch, state::Vector{UInt8}) =
+    check(ccall((:csmc_pt_load, libcsmc), Int32, (Ptr{Cvoid}, Int32, Ptr{UInt8}, Int64),
+                b.ctx.h, b.id, state, length(state)))
+
 # ---- differentiation ------------------------------------------------------------
 
 """dE/dS for every site of every system of `b` (3 x N per system)."""
