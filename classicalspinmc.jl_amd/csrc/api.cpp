@@ -86,7 +86,7 @@ struct Pinned {
 // the kernels visit the sites), the momenta and the device site of each row
 struct SsfBufs {
     DevBuf<double> pos, ks;
-    DevBuf<int32_t> perm;
+    DevBuf<int32_t> perm, tiles;
 };
 
 struct PTHost {
@@ -1228,8 +1228,9 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
     };
     // S(k) of every system, added to the sum of the temperature slot it holds
     auto corr_probe = [&]() -> int {
-        HIPCHK(launch_ssf_mfma(pt.corr_plan, b->prec, b->spins.p, b->sys_d.p, pt.corr_in.perm.p, pt.corr_in.pos.p,
-                               pt.corr_in.ks.p, b->ssf_B.p, b->ssf_P.p, b->ssf_out.p, c->stream, nullptr, nullptr));
+        HIPCHK(launch_ssf_mfma(pt.corr_plan, b->prec, b->spins.p, b->sys_d.p, pt.corr_in.perm.p,
+                               pt.corr_in.tiles.p, pt.corr_in.pos.p, pt.corr_in.ks.p, b->ssf_B.p, b->ssf_P.p,
+                               b->ssf_out.p, c->stream, nullptr, nullptr));
         HIPCHK(launch_ssf_accumulate(b->ssf_out.p, pt.sys_of_slot.p, R, pt.corr_nk, pt.corr_acc.p, c->stream));
         ++pt.corr_count;
         return CSMC_OK;
@@ -1780,18 +1781,23 @@ int ssf_prepare(csmc_ctx *c, Batch *b, const double *ks, int n_k, bool for_blas,
     }
     for (int k = 0; k < n_k; ++k)
         for (int q = 0; q < D; ++q) kv[(size_t)k * 3 + q] = ks[(size_t)k * D + q];
+    std::vector<int32_t> tiles(plan->wfirst);
+    tiles.insert(tiles.end(), plan->nparts.begin(), plan->nparts.end());
     HIPCHK(in.pos.reserve(pv.size()));
     HIPCHK(in.ks.reserve(kv.size()));
     HIPCHK(in.perm.reserve(perm.size()));
+    HIPCHK(in.tiles.reserve(tiles.size()));
     HIPCHK(b->ssf_out.reserve((size_t)R * n_k * 9));
     HIPCHK(hipMemcpyAsync(in.pos.p, pv.data(), pv.size() * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(in.ks.p, kv.data(), kv.size() * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(in.perm.p, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(in.tiles.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice, c->stream));
     // the host vectors go out of scope: wait for the copies
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!for_blas) {
         HIPCHK(b->ssf_B.reserve((size_t)plan->npad * plan->ldb));
-        HIPCHK(b->ssf_P.reserve((size_t)plan->ksplit * 2 * plan->mpad * plan->ldb));
+        HIPCHK(b->ssf_P.reserve((size_t)plan->mblocks * plan->colblocks * plan->maxparts * 2 * 64 * 16 *
+                                plan->nct));
     }
     return CSMC_OK;
 }
@@ -1834,8 +1840,9 @@ int32_t csmc_spin_correlations(csmc_ctx *c, int32_t id, const double *ks, int32_
                                b->ssf_out.p, c->stream));
         if (e1) HIPCHK(hipEventRecord(e1, c->stream));
     } else {
-        HIPCHK(launch_ssf_mfma(plan, b->prec, b->spins.p, b->sys_d.p, b->ssf_in.perm.p, b->ssf_in.pos.p,
-                               b->ssf_in.ks.p, b->ssf_B.p, b->ssf_P.p, b->ssf_out.p, c->stream, e0, e1));
+        HIPCHK(launch_ssf_mfma(plan, b->prec, b->spins.p, b->sys_d.p, b->ssf_in.perm.p, b->ssf_in.tiles.p,
+                               b->ssf_in.pos.p, b->ssf_in.ks.p, b->ssf_B.p, b->ssf_P.p, b->ssf_out.p, c->stream,
+                               e0, e1));
     }
     if (c->timing) c->events.push_back({e0, e1});
     HIPCHK(hipMemcpyAsync(out, b->ssf_out.p, (size_t)R * n_k * 9 * 8, hipMemcpyDeviceToHost, c->stream));

@@ -1,6 +1,7 @@
 // kernels.h -- launch interface between the libcsmc host runtime and its
 // gfx950 kernels.
 #pragma once
+#include <vector>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -157,16 +158,20 @@ struct SsfPlan {
     int colblocks, ldb;       // column blocks of 16 nct; ldb = colblocks 16 nct
     int mblocks, mpad;        // blocks of 64 momenta; mpad = 64 mblocks
     int nchunks, npad;        // chunks of SSF_KC sites; npad = SSF_KC nchunks
-    int ksplit, cps;          // splits of the site sum, chunks per split
+    int64_t units;            // (tile, chunk) pairs, tiles = mblocks colblocks
+    int grid;                 // workgroups: one round of resident ones
+    int maxparts;             // most workgroups sharing one tile
+    std::vector<int32_t> wfirst, nparts;  // per tile: first workgroup, count
 };
 void ssf_plan(int N, int nk, int nsys, int ncu, int ph, SsfPlan *p);
-// perm: N device site indices in the order of pos; pos: npad x 3, ks:
-// mpad x 3 (zero-padded); B: npad x ldb and P: ksplit x 2 x mpad x ldb
+// perm: N device site indices in the order of pos; tiles: wfirst then
+// nparts (2 x mblocks colblocks); pos: npad x 3, ks: mpad x 3
+// (zero-padded); B: npad x ldb and P: tiles x maxparts x 2 x 64 x 16 nct
 // scratch; out: nsys x nk x 9; ev0 / ev1 (may be null) are recorded around
 // the MFMA kernel alone
 hipError_t launch_ssf_mfma(const SsfPlan &p, int prec, const void *spins, const SysDev *sys, const int32_t *perm,
-                           const double *pos, const double *ks, double *B, double *P, double *out, hipStream_t st,
-                           hipEvent_t ev0, hipEvent_t ev1);
+                           const int32_t *tiles, const double *pos, const double *ks, double *B, double *P,
+                           double *out, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
 // acc[T][k][j] += out[sys_of_slot[T]][k][j] for the R temperature slots
 hipError_t launch_ssf_accumulate(const double *out, const int32_t *sys_of_slot, int R, int nk, double *acc,
                                  hipStream_t st);

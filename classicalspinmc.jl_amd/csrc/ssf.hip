@@ -188,21 +188,32 @@ __device__ __forceinline__ void phase_finish(double n, double sv, double cv, dou
 //      8 and 12 of a chunk are the previous ones rotated by exp(i k.d) (two
 //      multiplies and two multiply-adds), and the branch-free sincos of the
 //      next chunk's first 4 sites runs between the last MFMAs of the chunk.
+//
+// Work is spread as units (tile, chunk), a tile being 64 momenta x 16 NCT
+// columns: workgroup g of the grid takes the units [g U / G, (g + 1) U / G)
+// in tile-major order, so every workgroup has the same number of chunks and
+// the grid is one round of resident workgroups.  Where its range crosses a
+// tile boundary a workgroup writes one partial sum per tile it touched, to
+// part g - wfirst[tile] of that tile (P: [tile][part][2][64][16 NCT]).
 template <int NCT, int PH>
 __global__ __launch_bounds__(256) void k_ssf_mfma(const double *__restrict__ pos, const double *__restrict__ ks,
-                                                  const double *__restrict__ B, int ldb, int mpad, int nchunks,
-                                                  int cps, double dx, double dy, double dz,
-                                                  double *__restrict__ P) {
+                                                  const double *__restrict__ B, int ldb, int nchunks,
+                                                  int colblocks, int maxparts, int64_t units,
+                                                  const int32_t *__restrict__ wfirst, double dx, double dy,
+                                                  double dz, double *__restrict__ P) {
     constexpr int KC = SSF_KC, BN = 16 * NCT, NK = KC / 4, NST = KC * BN / 256;
     static_assert(KC * BN % 256 == 0, "a chunk is a whole number of elements per thread");
     __shared__ double Bs[2][KC][BN];
     __shared__ double Ps[2][KC][3];
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-    const int m = blockIdx.x * 64 + w * 16 + (l & 15);
+    const int64_t u_end = (int64_t)(blockIdx.x + 1) * units / gridDim.x;
+    for (int64_t u = (int64_t)blockIdx.x * units / gridDim.x; u < u_end;) {
+    const int tile = (int)(u / nchunks);
+    const int ch0 = (int)(u % nchunks);
+    const int ch1 = (int)min<int64_t>(nchunks, ch0 + (u_end - u));
+    const int m = (tile / colblocks) * 64 + w * 16 + (l & 15);
     const double kx = ks[3 * m], ky = ks[3 * m + 1], kz = ks[3 * m + 2];
-    const int c0 = blockIdx.y * BN;
-    const int ch0 = blockIdx.z * cps;
-    const int ch1 = min(ch0 + cps, nchunks);  // the host makes every split non-empty
+    const int c0 = (tile % colblocks) * BN;
     dbl4 ac[NCT], as[NCT];
 #pragma unroll
     for (int t = 0; t < NCT; ++t) {
@@ -336,30 +347,35 @@ __global__ __launch_bounds__(256) void k_ssf_mfma(const double *__restrict__ pos
         if (ch + 1 < ch1) store(buf ^ 1);
         __syncthreads();
     }
-    const int mrow = blockIdx.x * 64 + w * 16 + (l >> 4);
-    double *pc = P + ((int64_t)(2 * blockIdx.z) * mpad + mrow) * ldb + c0 + (l & 15);
-    double *ps = pc + (int64_t)mpad * ldb;
+    double *pc = P + ((int64_t)tile * maxparts + ((int)blockIdx.x - wfirst[tile])) * (2 * 64 * BN) +
+                 (w * 16 + (l >> 4)) * BN + (l & 15);
+    double *ps = pc + 64 * BN;
 #pragma unroll
     for (int t = 0; t < NCT; ++t)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            pc[(int64_t)4 * v * ldb + 16 * t] = ac[t][v];
-            ps[(int64_t)4 * v * ldb + 16 * t] = as[t][v];
+            pc[4 * v * BN + 16 * t] = ac[t][v];
+            ps[4 * v * BN + 16 * t] = as[t][v];
         }
+    u += ch1 - ch0;
+    __syncthreads();  // the next tile's staging reuses the LDS buffers
+    }
 }
 
-// out[r][k][3 a + b] = (Re_a Re_b + Im_a Im_b) / N, the split partial sums
-// added in split order; consecutive threads take consecutive systems, so
-// the partial sums are read in order of memory
-__global__ void k_ssf_finish(const double *P, int ksplit, int mpad, int ldb, int nk, int nsys, double inv_n,
-                             double *out) {
+// out[r][k][3 a + b] = (Re_a Re_b + Im_a Im_b) / N, the partial sums of the
+// tile added in part order; consecutive threads take consecutive systems, so
+// the partial sums are read in order of memory.  nparts: parts per tile.
+__global__ void k_ssf_finish(const double *P, int colblocks, int bn, int maxparts, const int32_t *nparts, int nk,
+                             int nsys, double inv_n, double *out) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nk * nsys) return;
     const int r = (int)(t % nsys), k = (int)(t / nsys);
+    const int cb = 3 * r / bn, col = 3 * r - cb * bn;  // a system's 3 columns share a column block
+    const int tile = (k / 64) * colblocks + cb;
     double re[3] = {0.0, 0.0, 0.0}, im[3] = {0.0, 0.0, 0.0};
-    for (int z = 0; z < ksplit; ++z) {
-        const double *pc = P + ((int64_t)(2 * z) * mpad + k) * ldb + 3 * r;
-        const double *ps = pc + (int64_t)mpad * ldb;
+    for (int p = 0; p < nparts[tile]; ++p) {
+        const double *pc = P + ((int64_t)tile * maxparts + p) * (2 * 64 * bn) + (k % 64) * bn + col;
+        const double *ps = pc + 64 * bn;
         for (int a = 0; a < 3; ++a) {
             re[a] += pc[a];
             im[a] += ps[a];
@@ -409,37 +425,28 @@ void ssf_plan(int N, int nk, int nsys, int ncu, int ph, SsfPlan *p) {
     p->mpad = p->mblocks * 64;
     p->nchunks = (N + SSF_KC - 1) / SSF_KC;
     p->npad = p->nchunks * SSF_KC;
-    // split of the site sum: the fewest rounds of workgroups over the
-    // resident slots times the chunks per workgroup, plus the cost of writing
-    // and reading back one more set of partial sums
+    // one round of resident workgroups, each with an equal range of units
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mfma_kernel(nct, ph), 256, 0) != hipSuccess || occ < 1)
         occ = 1;
-    const int64_t slots = (int64_t)(ncu > 0 ? ncu : 256) * occ;
     const int64_t tiles = (int64_t)p->mblocks * p->colblocks;
-    const double t_chunk = (2.84 * nct + 1.0) * 1e-7 * occ;             // s per chunk per workgroup
-    const double t_split = 2.0 * 2.0 * p->mpad * p->ldb * 8.0 / 4e12;  // s per set of partials
-    const int64_t max_split = std::max<int64_t>(1, ((int64_t)1 << 27) / (2 * (int64_t)p->mpad * p->ldb));
-    double best = 1e300;
-    p->ksplit = 1;
-    p->cps = p->nchunks;
-    const int64_t ks_max = std::min<int64_t>(p->nchunks, std::min<int64_t>(256, max_split));
-    for (int ks = 1; ks <= ks_max; ++ks) {
-        const int cps = (p->nchunks + ks - 1) / ks;
-        const int kse = (p->nchunks + cps - 1) / cps;  // no empty split
-        const int64_t rounds = (tiles * kse + slots - 1) / slots;
-        const double t = rounds * cps * t_chunk + kse * t_split;
-        if (t < best * (1.0 - 1e-9)) {
-            best = t;
-            p->ksplit = kse;
-            p->cps = cps;
-        }
+    p->units = tiles * p->nchunks;
+    p->grid = (int)std::min<int64_t>(p->units, (int64_t)(ncu > 0 ? ncu : 256) * occ);
+    // the workgroup holding unit u is the last g with g U / G <= u
+    auto owner = [&](int64_t u) { return (int)(((u + 1) * p->grid - 1) / p->units); };
+    p->wfirst.assign(tiles, 0);
+    p->nparts.assign(tiles, 0);
+    p->maxparts = 1;
+    for (int64_t t = 0; t < tiles; ++t) {
+        p->wfirst[t] = owner(t * p->nchunks);
+        p->nparts[t] = owner((t + 1) * p->nchunks - 1) - p->wfirst[t] + 1;
+        p->maxparts = std::max(p->maxparts, p->nparts[t]);
     }
 }
 
 hipError_t launch_ssf_mfma(const SsfPlan &p, int prec, const void *spins, const SysDev *sys, const int32_t *perm,
-                           const double *pos, const double *ks, double *B, double *P, double *out, hipStream_t st,
-                           hipEvent_t ev0, hipEvent_t ev1) {
+                           const int32_t *tiles, const double *pos, const double *ks, double *B, double *P,
+                           double *out, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     const int64_t nb = (int64_t)p.npad * p.ldb;
     if (prec == 32)
         hipLaunchKernelGGL(k_ssf_spins_rows<float>, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, spins,
@@ -451,19 +458,20 @@ hipError_t launch_ssf_mfma(const SsfPlan &p, int prec, const void *spins, const 
     if (e != hipSuccess) return e;
     if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
     {
-        int ldb = p.ldb, mpad = p.mpad, nchunks = p.nchunks, cps = p.cps;
+        int ldb = p.ldb, nchunks = p.nchunks, colblocks = p.colblocks, maxparts = p.maxparts;
+        int64_t units = p.units;
         double dx = p.d[0], dy = p.d[1], dz = p.d[2];
-        void *args[] = {(void *)&pos, (void *)&ks, (void *)&B, &ldb, &mpad, &nchunks, &cps, &dx, &dy, &dz,
-                        (void *)&P};
-        e = hipLaunchKernel(mfma_kernel(p.nct, p.ph), dim3(p.mblocks, p.colblocks, p.ksplit), dim3(256), args, 0,
-                            st);
+        void *args[] = {(void *)&pos, (void *)&ks, (void *)&B, &ldb, &nchunks, &colblocks, &maxparts, &units,
+                        (void *)&tiles, &dx, &dy, &dz, (void *)&P};
+        e = hipLaunchKernel(mfma_kernel(p.nct, p.ph), dim3(p.grid), dim3(256), args, 0, st);
         if (e != hipSuccess) return e;
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev1 && (e = hipEventRecord(ev1, st)) != hipSuccess) return e;
     const int64_t nc = (int64_t)p.nk * p.nsys;
-    hipLaunchKernelGGL(k_ssf_finish, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, P, p.ksplit, p.mpad,
-                       p.ldb, p.nk, p.nsys, 1.0 / p.N, out);
+    const int32_t *nparts = tiles + p.mblocks * p.colblocks;
+    hipLaunchKernelGGL(k_ssf_finish, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, P, p.colblocks,
+                       16 * p.nct, p.maxparts, nparts, p.nk, p.nsys, 1.0 / p.N, out);
     return hipGetLastError();
 }
 
