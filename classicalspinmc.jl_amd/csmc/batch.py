@@ -184,6 +184,16 @@ class SpinBatch:
         _lib.check(_lib.lib().csmc_pt_correlations(self._h, self.id, _lib.ptr(out), C.byref(cnt)))
         return np.transpose(out, (0, 2, 1)).copy(), cnt.value
 
+    def graph_sweeps(self, kind, T=None, sigma=None, n_sweeps=1, seed=0):
+        """n_sweeps plain-form sweeps captured as one hipGraph
+        (csmc_graph_sweeps); replay with .launch()"""
+        Tv = self._vec(T) if T is not None else None
+        sg = self._vec(sigma) if sigma is not None else None
+        g = C.c_void_p()
+        _lib.check(_lib.lib().csmc_graph_sweeps(self._h, self.id, int(kind), _lib.ptr(Tv), _lib.ptr(sg),
+                                                int(n_sweeps), C.c_uint64(int(seed)), C.byref(g)))
+        return SweepGraph(self, g)
+
     def pt_save(self):
         """the whole PT state as bytes (csmc_pt_save)"""
         n = C.c_int64()
@@ -240,3 +250,27 @@ class SpinBatch:
 
     def sync(self):
         _lib.check(_lib.lib().csmc_ctx_sync(self._h))
+
+
+class SweepGraph:
+    """A captured block of plain-form sweeps (csmc_graph_*)."""
+
+    def __init__(self, batch, handle):
+        self.batch = batch
+        self.h = handle
+
+    def launch(self):
+        """enqueue one replay (returns without waiting)"""
+        _lib.check(_lib.lib().csmc_graph_launch(self.batch._h, self.h))
+
+    def accepted(self):
+        """(accepted proposals per system of the last replay, cone widths)"""
+        acc = np.zeros(self.batch.n)
+        sg = np.zeros(self.batch.n)
+        _lib.check(_lib.lib().csmc_graph_accepted(self.batch._h, self.h, _lib.ptr(acc), _lib.ptr(sg)))
+        return acc, sg
+
+    def close(self):
+        if self.h:
+            _lib.lib().csmc_graph_destroy(self.h)
+            self.h = None

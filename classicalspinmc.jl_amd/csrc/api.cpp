@@ -163,6 +163,14 @@ struct Batch {
 
 }  // namespace
 
+struct csmc_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    csmc_ctx *ctx = nullptr;
+    int32_t batch = -1;
+    int32_t kind = 0;
+};
+
 struct csmc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -1071,6 +1079,104 @@ int32_t csmc_sweep_plain(csmc_ctx *c, int32_t id, int32_t kind, const double *T,
     for (int s = 0; s < b->nsys; ++s) acc[s] += got[s];
     if (accepted) std::copy(acc.begin(), acc.end(), accepted);
     if (sigma) std::copy(sg.begin(), sg.end(), sigma);
+    return CSMC_OK;
+}
+
+int32_t csmc_graph_sweeps(csmc_ctx *c, int32_t id, int32_t kind, const double *T, const double *sigma,
+                          int32_t n_sweeps, uint64_t seed, csmc_graph **out) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!out) return fail(CSMC_EINVAL, "null graph output");
+    *out = nullptr;
+    if (!valid_kind(kind)) return fail(CSMC_EINVAL, "unknown sweep kind");
+    if (n_sweeps < 1) return fail(CSMC_EINVAL, "n_sweeps < 1");
+    const bool metro = kind != CSMC_OVERRELAX && kind != CSMC_DETERMINISTIC;
+    if (metro && !T) return fail(CSMC_EINVAL, "Metropolis sweeps need temperatures");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    // the temperatures and cone widths the replays read
+    std::vector<double> sg(b->nsys, 60.0);
+    if (sigma) sg.assign(sigma, sigma + b->nsys);
+    if (T) HIPCHK(hipMemcpyAsync(b->T.p, T, b->nsys * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b->sigma.p, sg.data(), b->nsys * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    PassArgs a{};
+    a.P = pools(c, b);
+    a.spins = b->spins.p;
+    a.kind = kind;
+    a.k0 = (uint32_t)seed;
+    a.k1 = (uint32_t)(seed >> 32);
+    a.ctr = b->mcount.p;  // read on the device, so every replay draws anew
+    a.T_sys = b->T.p;
+    a.sigma_sys = b->sigma.p;
+    a.acc = b->accU.p;
+    // record the same launches csmc_sweep_plain makes, plus the per-sweep
+    // counter step and a clearing kernel in place of its memset
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return fail(CSMC_EHIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    e = launch_graph_clear(b->accU.p, b->acc.p, b->nsys, c->stream);
+    for (int s = 0; s < b->nsys && e == hipSuccess; ++s) {
+        const HostLattice &L = *c->lats[b->lat[s]];
+        a.sys0 = s;
+        for (int sw = 0; sw < n_sweeps && e == hipSuccess; ++sw) {
+            for (int col = 0; col < L.ncol && e == hipSuccess; ++col) {
+                const int n_col = L.runs[L.colrun[col + 1] - 1][1] - L.runs[L.colrun[col]][0];
+                a.colour = col;
+                e = launch_pass(b->prec, b->self, b->gen, 1, n_col, a, c->stream);
+            }
+            if (metro && e == hipSuccess)
+                e = launch_graph_after_sweep(b->mcount.p, b->accU.p, b->acc.p, b->sigma.p, s,
+                                             kind == CSMC_CONE_ADAPTIVE, (double)L.N, c->stream);
+        }
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e == hipSuccess) e = e2;
+    hipGraphExec_t x = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(CSMC_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    auto *gr = new csmc_graph();
+    gr->graph = g;
+    gr->exec = x;
+    gr->ctx = c;
+    gr->batch = id;
+    gr->kind = kind;
+    *out = gr;
+    return CSMC_OK;
+}
+
+int32_t csmc_graph_launch(csmc_ctx *c, csmc_graph *g) {
+    CHECK_CTX(c);
+    if (!g || g->ctx != c) return fail(CSMC_EINVAL, "graph of another context");
+    GET_BATCH(c, g->batch, b);
+    (void)b;
+    HIPCHK(hipGraphLaunch(g->exec, c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_graph_accepted(csmc_ctx *c, csmc_graph *g, double *accepted, double *sigma) {
+    CHECK_CTX(c);
+    if (!g || g->ctx != c) return fail(CSMC_EINVAL, "graph of another context");
+    GET_BATCH(c, g->batch, b);
+    std::vector<uint32_t> got(b->nsys);
+    std::vector<double> tot(b->nsys);
+    HIPCHK(hipMemcpyAsync(got.data(), b->accU.p, b->nsys * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(tot.data(), b->acc.p, b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
+    if (sigma) HIPCHK(hipMemcpyAsync(sigma, b->sigma.p, b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (accepted)
+        for (int s = 0; s < b->nsys; ++s) accepted[s] = tot[s] + (double)got[s];
+    return CSMC_OK;
+}
+
+int32_t csmc_graph_destroy(csmc_graph *g) {
+    if (!g) return CSMC_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
     return CSMC_OK;
 }
 

@@ -116,6 +116,11 @@ hipError_t launch_wg_split(int k2c, int spt, bool diag, int G, int R, size_t lds
                            const WGArgs &a, hipStream_t st);
 hipError_t launch_pass(int prec, bool self, bool gen, int nsys, int max_colour_sites,
                        const PassArgs &a, hipStream_t st);
+// captured plain-form sweeps: the per-sweep counter (and adaptive cone)
+// step of system s, and the clearing of the accepted counters
+hipError_t launch_graph_after_sweep(uint32_t *ctr, uint32_t *acc, double *total, double *sigma, int s,
+                                    bool adaptive, double n_sites, hipStream_t st);
+hipError_t launch_graph_clear(uint32_t *acc, double *total, int nsys, hipStream_t st);
 hipError_t launch_site_eval(int prec, const DevPools &P, const void *spins, int sys, int N,
                             double *H, double *e, hipStream_t st);
 hipError_t launch_energy(int prec, const DevPools &P, const void *spins, int nsys, double *E,

@@ -407,6 +407,46 @@ static void launch_pass_r(bool self, bool gen, dim3 grid, const PassArgs &a, hip
     }
 }
 
+// after a Metropolis sweep of system s in a captured graph: advance its
+// Philox sweep counter, and (adaptive cone) move its accepted count into the
+// total and update the cone width as metropolis_adaptive! does
+// (metropolis.jl:129-131), in the host path's FP64 arithmetic
+__global__ void k_graph_after_sweep(uint32_t *ctr, uint32_t *acc, double *total, double *sigma, int s,
+                                    int adaptive, double n_sites) {
+    if (threadIdx.x != 0) return;
+    ctr[s] += 1u;
+    if (adaptive) {
+        const uint32_t got = acc[s];
+        acc[s] = 0u;
+        total[s] += (double)got;
+        const double av = (double)got / n_sites;
+        const double f = 0.5 / fmax(1.0 - av, 0.05);
+        sigma[s] = fmin(fmax(sigma[s] * f, 0.0), 100.0);
+    }
+}
+
+// clears the accepted counters and totals at the start of a replay (a fill
+// kernel, not a memset node)
+__global__ void k_graph_clear(uint32_t *acc, double *total, int nsys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsys) {
+        acc[i] = 0u;
+        total[i] = 0.0;
+    }
+}
+
+hipError_t launch_graph_after_sweep(uint32_t *ctr, uint32_t *acc, double *total, double *sigma, int s,
+                                    bool adaptive, double n_sites, hipStream_t st) {
+    hipLaunchKernelGGL(k_graph_after_sweep, dim3(1), dim3(64), 0, st, ctr, acc, total, sigma, s, adaptive ? 1 : 0,
+                       n_sites);
+    return hipGetLastError();
+}
+
+hipError_t launch_graph_clear(uint32_t *acc, double *total, int nsys, hipStream_t st) {
+    hipLaunchKernelGGL(k_graph_clear, dim3((nsys + 255) / 256), dim3(256), 0, st, acc, total, nsys);
+    return hipGetLastError();
+}
+
 hipError_t launch_pass(int prec, bool self, bool gen, int nsys, int max_colour_sites,
                        const PassArgs &a, hipStream_t st) {
     dim3 grid((max_colour_sites + 255) / 256, nsys);

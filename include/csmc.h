@@ -45,6 +45,7 @@ extern "C" {
 #define CSMC_DETERMINISTIC 4   /* deterministic_updates! monte_carlo.jl:201-213 */
 
 typedef struct csmc_ctx csmc_ctx;
+typedef struct csmc_graph csmc_graph;
 
 /* UnitCell{D} (unit_cell.jl:3-18) flattened.  Term arrays are in call order
  * of addZeemanCoupling!/addOnSite!/addBilinear!/addCubic!/addQuartic!. */
@@ -230,6 +231,24 @@ int32_t csmc_pt_state(csmc_ctx *ctx, int32_t batch_id, int32_t *system_of_slot, 
  * planner decides) or 2..8 (forces G).  No counterpart in the reference (its
  * replicas are MPI ranks). */
 int32_t csmc_pt_split_parts(csmc_ctx *ctx, int32_t batch_id, int32_t *parts);
+/* ---- graph capture (no reference counterpart: launch-bound loops replayed
+ * as one hipGraph) ------------------------------------------------------------ */
+/* Capture n_sweeps sweeps of `kind` of the plain form (csmc_sweep_plain: one
+ * launch per colour pass per system) into an executable hipGraph.  T and
+ * sigma (null: 60) are copied to the batch now and read by every replay; the
+ * Philox sweep counters are read and advanced on the device, so each replay
+ * continues the random stream as the eager calls would, and the adaptive
+ * cone width is updated on the device (metropolis.jl:129-131).  The batch
+ * must outlive the graph. */
+int32_t csmc_graph_sweeps(csmc_ctx *ctx, int32_t batch_id, int32_t kind, const double *T, const double *sigma,
+                          int32_t n_sweeps, uint64_t seed, csmc_graph **out);
+/* Enqueue one replay on the context's stream (returns without waiting). */
+int32_t csmc_graph_launch(csmc_ctx *ctx, csmc_graph *graph);
+/* Accepted proposals per system of the last replay, and the cone widths
+ * (either may be null); waits for the stream. */
+int32_t csmc_graph_accepted(csmc_ctx *ctx, csmc_graph *graph, double *accepted, double *sigma);
+int32_t csmc_graph_destroy(csmc_graph *graph);
+
 /* ---- resume (no reference counterpart: the reference's checkpoints,
  * write_MC_checkpoint hdf5.jl:176-180, hold the spins only and cannot
  * restart a run) ---------------------------------------------------------- */
