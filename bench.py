@@ -232,7 +232,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1024,
                     help="untimed steps first (1024 = one whole chained launch)")
     ap.add_argument("--workload", default="pt_honeycomb",
-                    choices=["pt_honeycomb", "anneal_uneven", "pt_pyrochlore", "ssf"])
+                    choices=["pt_honeycomb", "anneal_uneven", "pt_pyrochlore", "ssf", "probe_L512"])
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline")
@@ -258,6 +258,8 @@ def main():
         return bench_pyrochlore(args, csmc, W)
     if args.workload == "ssf":
         return bench_ssf(args, csmc, W)
+    if args.workload == "probe_L512":
+        return bench_probe(args, csmc, W)
 
     L, R = args.L, args.replicas
     uc = W.honeycomb_kitaev(1.0, -2.0)
@@ -421,6 +423,68 @@ def bench_pyrochlore(args, csmc, W):
                                   "parallel tempering, 32 replicas, T=0.02..2",
                       "storage_mode": r["mode"]}}
     r["b"].close()
+    finish(out, args)
+
+
+def bench_probe(args, csmc, W):
+    """Roofline probe: the headline geometry at L=512 (524,288 sites) x 64
+    FP32 replicas, 537 MB of spin records -- more than the 256 MB Infinity
+    Cache, so every sweep streams the state from HBM.  One step = 10
+    overrelaxation sweeps + 1 Metropolis sweep of every replica in the
+    batch-wide form (csmc_sweep_passes: one launch per colour pass over all
+    replicas, one thread per site); the persistent one-workgroup-per-replica
+    kernel on the same state is timed beside it."""
+    L, R = 512, 64
+    uc = W.honeycomb_kitaev(1.0, -2.0)
+    lat = csmc.Lattice((L, L), uc, 0.5, seed=2)
+    N = lat.size
+    temps = W.geometric_temperatures(0.01, 1.0, R)
+    ctx = csmc.default_context()
+    b = csmc.SpinBatch([lat] * R, precision=32, ctx=ctx)
+    b.randomize(2)
+    steps = max(1, min(args.steps, 20))
+    b.sweep_passes(csmc.OVERRELAX, n_sweeps=2)
+    ctx.sync()
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.sweep_passes(csmc.OVERRELAX, n_sweeps=10)
+        b.sweep_passes(csmc.METROPOLIS, T=temps, n_sweeps=1, seed=3)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    kern_ms, launches = ctx.timing_read()
+    ctx.timing(False)
+    updates = steps * 11 * R * N
+    value = updates / dt
+    achieved = BYTES_PER_UPDATE_FP32 * updates / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_pass_probe_honeycomb_L512_R64_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes_per_update"] * R * N
+    # the persistent kernel (one workgroup per replica) on the same state
+    b.sweep(csmc.OVERRELAX, n_sweeps=1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    b.sweep(csmc.OVERRELAX, n_sweeps=4)
+    ctx.sync()
+    persistent = 4 * R * N / (time.perf_counter() - t0)
+    out = {"metric": METRIC, "value": value, "unit": "spin-updates/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+           "ms_per_step": dt / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": f"roofline probe: honeycomb Kitaev-Heisenberg J=1 K=-2, L={L} ({N} spins), "
+                                  f"{R} FP32 replicas ({R * N * 16 / 1e6:.0f} MB of spin records), 10 "
+                                  "overrelaxation + 1 Metropolis sweep per step, batch-wide colour passes",
+                      "sites": N, "replicas": R, "storage_mode": b.storage_mode()},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                        "kernel": "k_pass<float> over all replicas (one launch per colour pass)",
+                        "note": "traffic = measured HBM bytes per sweep of all replicas"},
+           "persistent_kernel": {"value": persistent, "unit": "spin-updates/s",
+                                 "form": "one workgroup per replica (64 CUs), overrelaxation sweeps"}}
+    b.close()
+    log(f"[bench] probe: {value / 1e9:.2f} G updates/s, {achieved:.0f} GB/s algorithmic; persistent "
+        f"{persistent / 1e9:.2f} G updates/s")
     finish(out, args)
 
 

@@ -184,6 +184,17 @@ class SpinBatch:
         _lib.check(_lib.lib().csmc_pt_correlations(self._h, self.id, _lib.ptr(out), C.byref(cnt)))
         return np.transpose(out, (0, 2, 1)).copy(), cnt.value
 
+    def sweep_passes(self, kind, T=None, sigma=None, n_sweeps=1, seed=0):
+        """the plain form over the whole batch (csmc_sweep_passes): one
+        launch per colour pass covering every system; returns accepted
+        counts"""
+        Tv = self._vec(T)
+        sg = self._vec(sigma)
+        acc = np.zeros(self.n)
+        _lib.check(_lib.lib().csmc_sweep_passes(self._h, self.id, int(kind), _lib.ptr(Tv), _lib.ptr(sg),
+                                                int(n_sweeps), C.c_uint64(int(seed)), _lib.ptr(acc)))
+        return acc
+
     def graph_sweeps(self, kind, T=None, sigma=None, n_sweeps=1, seed=0):
         """n_sweeps plain-form sweeps captured as one hipGraph
         (csmc_graph_sweeps); replay with .launch()"""

@@ -1082,6 +1082,69 @@ int32_t csmc_sweep_plain(csmc_ctx *c, int32_t id, int32_t kind, const double *T,
     return CSMC_OK;
 }
 
+int32_t csmc_sweep_passes(csmc_ctx *c, int32_t id, int32_t kind, const double *T, const double *sigma,
+                          int32_t n_sweeps, uint64_t seed, double *accepted) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!valid_kind(kind)) return fail(CSMC_EINVAL, "unknown sweep kind");
+    if (kind == CSMC_CONE_ADAPTIVE)
+        return fail(CSMC_EINVAL, "the adaptive cone needs a host step per system (csmc_sweep_plain)");
+    if (n_sweeps < 0) return fail(CSMC_EINVAL, "n_sweeps < 0");
+    const bool metro = kind != CSMC_OVERRELAX && kind != CSMC_DETERMINISTIC;
+    if (metro && !T) return fail(CSMC_EINVAL, "Metropolis sweeps need temperatures");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    std::vector<double> sg(b->nsys, 60.0);
+    if (sigma) sg.assign(sigma, sigma + b->nsys);
+    if (T) HIPCHK(hipMemcpyAsync(b->T.p, T, b->nsys * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b->sigma.p, sg.data(), b->nsys * 8, hipMemcpyHostToDevice, c->stream));
+    // the most colours and the largest colour class of any system
+    int ncol = 0, maxcol = 0;
+    for (int s = 0; s < b->nsys; ++s) {
+        const HostLattice &L = *c->lats[b->lat[s]];
+        ncol = std::max(ncol, L.ncol);
+        for (int col = 0; col < L.ncol; ++col)
+            maxcol = std::max(maxcol, L.runs[L.colrun[col + 1] - 1][1] - L.runs[L.colrun[col]][0]);
+    }
+    PassArgs a{};
+    a.P = pools(c, b);
+    a.spins = b->spins.p;
+    a.kind = kind;
+    a.k0 = (uint32_t)seed;
+    a.k1 = (uint32_t)(seed >> 32);
+    a.ctr = b->mcount.p;
+    a.T_sys = b->T.p;
+    a.sigma_sys = b->sigma.p;
+    a.acc = b->accU.p;
+    a.sys0 = 0;
+    HIPCHK(launch_graph_clear(b->accU.p, b->acc.p, b->nsys, c->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, c->stream));
+    }
+    for (int sw = 0; sw < n_sweeps; ++sw) {
+        for (int col = 0; col < ncol; ++col) {  // systems with fewer colours skip the extra passes
+            a.colour = col;
+            HIPCHK(launch_pass(b->prec, b->self, b->gen, b->nsys, maxcol, a, c->stream));
+        }
+        if (metro)
+            HIPCHK(launch_graph_after_sweep(b->mcount.p, b->accU.p, b->acc.p, b->sigma.p, 0, b->nsys, false, 1.0,
+                                            c->stream));
+    }
+    if (c->timing) {
+        HIPCHK(hipEventRecord(e1, c->stream));
+        c->events.push_back({e0, e1});
+    }
+    std::vector<uint32_t> got(b->nsys);
+    HIPCHK(hipMemcpyAsync(got.data(), b->accU.p, b->nsys * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (accepted)
+        for (int s = 0; s < b->nsys; ++s) accepted[s] = (double)got[s];
+    return CSMC_OK;
+}
+
 int32_t csmc_graph_sweeps(csmc_ctx *c, int32_t id, int32_t kind, const double *T, const double *sigma,
                           int32_t n_sweeps, uint64_t seed, csmc_graph **out) {
     CHECK_CTX(c);
@@ -1125,7 +1188,7 @@ int32_t csmc_graph_sweeps(csmc_ctx *c, int32_t id, int32_t kind, const double *T
                 e = launch_pass(b->prec, b->self, b->gen, 1, n_col, a, c->stream);
             }
             if (metro && e == hipSuccess)
-                e = launch_graph_after_sweep(b->mcount.p, b->accU.p, b->acc.p, b->sigma.p, s,
+                e = launch_graph_after_sweep(b->mcount.p, b->accU.p, b->acc.p, b->sigma.p, s, 1,
                                              kind == CSMC_CONE_ADAPTIVE, (double)L.N, c->stream);
         }
     }

@@ -117,9 +117,10 @@ hipError_t launch_wg_split(int k2c, int spt, bool diag, int G, int R, size_t lds
 hipError_t launch_pass(int prec, bool self, bool gen, int nsys, int max_colour_sites,
                        const PassArgs &a, hipStream_t st);
 // captured plain-form sweeps: the per-sweep counter (and adaptive cone)
-// step of system s, and the clearing of the accepted counters
-hipError_t launch_graph_after_sweep(uint32_t *ctr, uint32_t *acc, double *total, double *sigma, int s,
-                                    bool adaptive, double n_sites, hipStream_t st);
+// step of systems s0 .. s0 + count - 1 (adaptive: count 1), and the clearing
+// of the accepted counters
+hipError_t launch_graph_after_sweep(uint32_t *ctr, uint32_t *acc, double *total, double *sigma, int s0,
+                                    int count, bool adaptive, double n_sites, hipStream_t st);
 hipError_t launch_graph_clear(uint32_t *acc, double *total, int nsys, hipStream_t st);
 hipError_t launch_site_eval(int prec, const DevPools &P, const void *spins, int sys, int N,
                             double *H, double *e, hipStream_t st);

@@ -407,13 +407,14 @@ static void launch_pass_r(bool self, bool gen, dim3 grid, const PassArgs &a, hip
     }
 }
 
-// after a Metropolis sweep of system s in a captured graph: advance its
-// Philox sweep counter, and (adaptive cone) move its accepted count into the
-// total and update the cone width as metropolis_adaptive! does
-// (metropolis.jl:129-131), in the host path's FP64 arithmetic
-__global__ void k_graph_after_sweep(uint32_t *ctr, uint32_t *acc, double *total, double *sigma, int s,
-                                    int adaptive, double n_sites) {
-    if (threadIdx.x != 0) return;
+// after a Metropolis sweep of systems s0 .. s0 + count - 1: advance their
+// Philox sweep counters, and (adaptive cone, one system) move the accepted
+// count into the total and update the cone width as metropolis_adaptive!
+// does (metropolis.jl:129-131), in the host path's FP64 arithmetic
+__global__ void k_graph_after_sweep(uint32_t *ctr, uint32_t *acc, double *total, double *sigma, int s0,
+                                    int count, int adaptive, double n_sites) {
+    const int s = s0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= s0 + count) return;
     ctr[s] += 1u;
     if (adaptive) {
         const uint32_t got = acc[s];
@@ -435,10 +436,10 @@ __global__ void k_graph_clear(uint32_t *acc, double *total, int nsys) {
     }
 }
 
-hipError_t launch_graph_after_sweep(uint32_t *ctr, uint32_t *acc, double *total, double *sigma, int s,
-                                    bool adaptive, double n_sites, hipStream_t st) {
-    hipLaunchKernelGGL(k_graph_after_sweep, dim3(1), dim3(64), 0, st, ctr, acc, total, sigma, s, adaptive ? 1 : 0,
-                       n_sites);
+hipError_t launch_graph_after_sweep(uint32_t *ctr, uint32_t *acc, double *total, double *sigma, int s0,
+                                    int count, bool adaptive, double n_sites, hipStream_t st) {
+    hipLaunchKernelGGL(k_graph_after_sweep, dim3((count + 255) / 256), dim3(256), 0, st, ctr, acc, total, sigma,
+                       s0, count, adaptive ? 1 : 0, n_sites);
     return hipGetLastError();
 }
 

@@ -231,6 +231,14 @@ int32_t csmc_pt_state(csmc_ctx *ctx, int32_t batch_id, int32_t *system_of_slot, 
  * planner decides) or 2..8 (forces G).  No counterpart in the reference (its
  * replicas are MPI ranks). */
 int32_t csmc_pt_split_parts(csmc_ctx *ctx, int32_t batch_id, int32_t *parts);
+/* The plain form over the whole batch: one launch per colour pass covering
+ * every system (one thread per site), the Philox keys and counters of
+ * csmc_sweep_plain, so the results are the same bit for bit.  For systems
+ * too large for one workgroup to sweep well; not for the adaptive cone.
+ * With csmc_timing on, events bracket the passes. */
+int32_t csmc_sweep_passes(csmc_ctx *ctx, int32_t batch_id, int32_t kind, const double *T, const double *sigma,
+                          int32_t n_sweeps, uint64_t seed, double *accepted);
+
 /* ---- graph capture (no reference counterpart: launch-bound loops replayed
  * as one hipGraph) ------------------------------------------------------------ */
 /* Capture n_sweeps sweeps of `kind` of the plain form (csmc_sweep_plain: one

@@ -67,3 +67,31 @@ def test_graph_errors():
     with pytest.raises(csmc.CsmcError, match="n_sweeps"):
         b.graph_sweeps(csmc.OVERRELAX, n_sweeps=0)
     b.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_batch_wide_passes_equal_plain_sweeps(prec):
+    """csmc_sweep_passes (one launch per colour pass for every system) and
+    csmc_sweep_plain (one system at a time) make the same updates with the
+    same draws: equal bit for bit on an uneven batch whose lattices have
+    different colour counts"""
+    cells = W.uneven_batch(n=5, sizes=(5, 6, 8, 9, 12), seed=7)
+    lats = [csmc.Lattice(shape, uc, 1.0, seed=20 + k) for k, (uc, shape) in enumerate(cells)]
+    assert len({lat.info.n_colours for lat in lats}) > 1
+    T = np.linspace(0.3, 1.2, len(lats))
+    a = csmc.SpinBatch(lats, precision=prec)
+    b = csmc.SpinBatch(lats, precision=prec)
+    for x in (a, b):
+        x.randomize(5)
+    for kind in (csmc.OVERRELAX, csmc.METROPOLIS, csmc.CONE, csmc.DETERMINISTIC):
+        acc_a = a.sweep(kind, T=T, sigma=0.8, n_sweeps=2, seed=13, plain=True)
+        acc_b = b.sweep_passes(kind, T=T, sigma=0.8, n_sweeps=2, seed=13)
+        for r in range(len(lats)):
+            assert np.array_equal(a.get_spins(r), b.get_spins(r)), (kind, r)
+        assert np.array_equal(a.counters(), b.counters())
+        if kind in (csmc.METROPOLIS, csmc.CONE):
+            assert np.array_equal(acc_a, acc_b) and acc_a.sum() > 0
+    with pytest.raises(csmc.CsmcError, match="adaptive"):
+        b.sweep_passes(csmc.CONE_ADAPTIVE, T=T)
+    a.close()
+    b.close()
