@@ -469,6 +469,8 @@ def bench_probe(args, csmc, W):
     b.sweep(csmc.OVERRELAX, n_sweeps=4)
     ctx.sync()
     persistent = 4 * R * N / (time.perf_counter() - t0)
+    # the autotuner on the same state (it restores the state)
+    tune_times, tune_choice = b.autotune(csmc.OVERRELAX, n_sweeps=2)
     out = {"metric": METRIC, "value": value, "unit": "spin-updates/s", "n_gpus": 1, "steps": steps, "warmup": 1,
            "ms_per_step": dt / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "fp32", "data": "synthetic",
@@ -481,7 +483,9 @@ def bench_probe(args, csmc, W):
                         "kernel": "k_pass<float> over all replicas (one launch per colour pass)",
                         "note": "traffic = measured HBM bytes per sweep of all replicas"},
            "persistent_kernel": {"value": persistent, "unit": "spin-updates/s",
-                                 "form": "one workgroup per replica (64 CUs), overrelaxation sweeps"}}
+                                 "form": "one workgroup per replica (64 CUs), overrelaxation sweeps"},
+           "autotune": {"ms_for_2_overrelaxation_sweeps": {k: round(v, 3) for k, v in tune_times.items()},
+                        "chosen": tune_choice}}
     b.close()
     log(f"[bench] probe: {value / 1e9:.2f} G updates/s, {achieved:.0f} GB/s algorithmic; persistent "
         f"{persistent / 1e9:.2f} G updates/s")

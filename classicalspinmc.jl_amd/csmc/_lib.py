@@ -111,6 +111,7 @@ SIGNATURES = {
     "csmc_spin_correlations": [_P, _I32, _P, _I32, _P],
     "csmc_pt_set_correlations": [_P, _I32, _P, _I32],
     "csmc_sweep_passes": [_P, _I32, _I32, _P, _P, _I32, C.c_uint64, _P],
+    "csmc_autotune": [_P, _I32, _I32, _P, _I32, _P, _P],
     "csmc_graph_sweeps": [_P, _I32, _I32, _P, _P, _I32, C.c_uint64, _P],
     "csmc_graph_launch": [_P, _P],
     "csmc_graph_accepted": [_P, _P, _P, _P],

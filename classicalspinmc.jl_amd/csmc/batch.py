@@ -195,6 +195,22 @@ class SpinBatch:
                                                 int(n_sweeps), C.c_uint64(int(seed)), _lib.ptr(acc)))
         return acc
 
+    AUTOTUNE_FORMS = ("persistent, storage mode cap 3", "persistent, cap 2", "persistent, cap 1",
+                      "persistent, cap 0", "batch-wide colour passes")
+
+    def autotune(self, kind=None, T=None, n_sweeps=3):
+        """time each sweep form on the batch's own state and keep the
+        fastest for later sweeps (csmc_autotune); returns {form: ms} and the
+        chosen form"""
+        kind = _lib.OVERRELAX if kind is None else kind
+        Tv = self._vec(T)
+        times = np.zeros(5)
+        choice = C.c_int32()
+        _lib.check(_lib.lib().csmc_autotune(self._h, self.id, int(kind), _lib.ptr(Tv), int(n_sweeps),
+                                            _lib.ptr(times), C.byref(choice)))
+        return ({f: t for f, t in zip(self.AUTOTUNE_FORMS, times) if np.isfinite(t)},
+                self.AUTOTUNE_FORMS[choice.value])
+
     def graph_sweeps(self, kind, T=None, sigma=None, n_sweeps=1, seed=0):
         """n_sweeps plain-form sweeps captured as one hipGraph
         (csmc_graph_sweeps); replay with .launch()"""

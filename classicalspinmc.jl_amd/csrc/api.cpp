@@ -157,6 +157,10 @@ struct Batch {
     // equal-time correlations: device positions, momenta and GEMM workspaces
     SsfBufs ssf_in;
     DevBuf<double> ssf_C, ssf_Q, ssf_B, ssf_Cre, ssf_Cim, ssf_P, ssf_out;
+    // set by csmc_autotune: the highest storage mode the persistent kernel
+    // may use, and whether csmc_sweep runs as batch-wide colour passes
+    int lm_cap = 3;
+    bool passes = false;
     PTHost pt;
     size_t spin_bytes() const { return prec == 32 ? 16 : 32; }
 };
@@ -437,16 +441,17 @@ int choose_lm(csmc_ctx *c, Batch *b, size_t *bytes) {
     const bool k2ok = b->k2c == 3 || b->k2c == 4 || b->k2c == 6;
     bool colbasis = true;
     for (int id : b->lat) colbasis = colbasis && (c->h_lat[id].flags & LAT_COLBASIS);
-    if (c->lds_cap >= 3 && c->specialise && b->prec == 32 && !b->self && !b->gen && k2ok && colbasis &&
+    const int cap = std::min(c->lds_cap, b->lm_cap);
+    if (cap >= 3 && c->specialise && b->prec == 32 && !b->self && !b->gen && k2ok && colbasis &&
         reg_spt(b->nslots, b->k2c) > 0 && 16 * (size_t)(b->maxN + 1) <= avail) {
         *bytes = 16 * (size_t)(b->maxN + 1);
         return 3;
     }
-    if (c->lds_cap >= 2 && b->maxN < 65535 && spins + idx <= avail) {
+    if (cap >= 2 && b->maxN < 65535 && spins + idx <= avail) {
         *bytes = spins + idx;
         return 2;
     }
-    if (c->lds_cap >= 1 && spins <= avail) {
+    if (cap >= 1 && spins <= avail) {
         *bytes = spins;
         return 1;
     }
@@ -628,6 +633,11 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
     return CSMC_OK;
 }
 
+}  // namespace
+
+namespace {
+int sweep_passes(csmc_ctx *c, Batch *b, int32_t kind, const double *T, const double *sigma, int32_t n_sweeps,
+                 uint64_t seed, double *accepted);
 }  // namespace
 
 extern "C" {
@@ -993,6 +1003,8 @@ int32_t csmc_sweep(csmc_ctx *c, int32_t id, int32_t kind, const double *T, doubl
     CHECK_CTX(c);
     GET_BATCH(c, id, b);
     if (!valid_kind(kind)) return fail(CSMC_EINVAL, "unknown sweep kind");
+    if (b->passes && kind != CSMC_CONE_ADAPTIVE)  // chosen by csmc_autotune
+        return sweep_passes(c, b, kind, T, sigma, n_sweeps, seed, accepted);
     if (n_sweeps < 0) return fail(CSMC_EINVAL, "n_sweeps < 0");
     bool metro = kind != CSMC_OVERRELAX && kind != CSMC_DETERMINISTIC;
     if (metro && !T) return fail(CSMC_EINVAL, "Metropolis sweeps need temperatures");
@@ -1086,6 +1098,104 @@ int32_t csmc_sweep_passes(csmc_ctx *c, int32_t id, int32_t kind, const double *T
                           int32_t n_sweeps, uint64_t seed, double *accepted) {
     CHECK_CTX(c);
     GET_BATCH(c, id, b);
+    return sweep_passes(c, b, kind, T, sigma, n_sweeps, seed, accepted);
+}
+
+int32_t csmc_autotune(csmc_ctx *c, int32_t id, int32_t kind, const double *T, int32_t n_sweeps, double *times,
+                      int32_t *choice) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!valid_kind(kind)) return fail(CSMC_EINVAL, "unknown sweep kind");
+    if (n_sweeps < 1) return fail(CSMC_EINVAL, "n_sweeps < 1");
+    const bool metro = kind != CSMC_OVERRELAX && kind != CSMC_DETERMINISTIC;
+    if (metro && !T) return fail(CSMC_EINVAL, "Metropolis sweeps need temperatures");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    // the state the trials start from, restored after each
+    DevBuf<unsigned char> spins0;
+    DevBuf<uint32_t> ctr0;
+    HIPCHK(spins0.alloc(b->spins.n));
+    HIPCHK(ctr0.alloc(b->mcount.n));
+    HIPCHK(hipMemcpyAsync(spins0.p, b->spins.p, b->spins.n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(ctr0.p, b->mcount.p, b->mcount.n * 4, hipMemcpyDeviceToDevice, c->stream));
+    auto restore = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(b->spins.p, spins0.p, b->spins.n, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(b->mcount.p, ctr0.p, b->mcount.n * 4, hipMemcpyDeviceToDevice, c->stream));
+        return CSMC_OK;
+    };
+    const int cap0 = b->lm_cap;
+    const bool passes0 = b->passes;
+    const bool timing0 = c->timing;
+    c->timing = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    // candidates 0..3: the persistent kernel capped at storage mode 3..0
+    // (a cap that selects a mode already tried is skipped); 4: batch-wide
+    // colour passes (not for the adaptive cone)
+    double best = 1e300;
+    int best_k = -1, seen_lm[4] = {-1, -1, -1, -1};
+    for (int k = 0; k < 5; ++k) {
+        if (times) times[k] = std::nan("");
+        if (k < 4) {
+            b->lm_cap = 3 - k;
+            b->passes = false;
+            size_t bytes = 0;
+            const int lm = choose_lm(c, b, &bytes);
+            bool dup = false;
+            for (int j = 0; j < k; ++j) dup = dup || seen_lm[j] == lm;
+            seen_lm[k] = lm;
+            if (dup) continue;
+        } else {
+            if (kind == CSMC_CONE_ADAPTIVE) continue;
+            b->lm_cap = 3;
+            b->passes = true;
+        }
+        double ms_best = 1e300;
+        for (int rep = 0; rep < 2 && rc == CSMC_OK; ++rep) {  // the first run pays one-time costs
+            rc = restore();
+            if (rc) break;
+            HIPCHK(hipEventRecord(e0, c->stream));
+            rc = b->passes ? sweep_passes(c, b, kind, T, nullptr, n_sweeps, 1, nullptr)
+                           : csmc_sweep(c, id, kind, T, nullptr, n_sweeps, 1, nullptr);
+            if (rc) break;
+            HIPCHK(hipEventRecord(e1, c->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            ms_best = std::min(ms_best, (double)ms);
+        }
+        if (rc) break;
+        if (times) times[k] = ms_best;
+        if (ms_best < best) {
+            best = ms_best;
+            best_k = k;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->timing = timing0;
+    const int rc2 = restore();
+    if (rc || rc2 || best_k < 0) {
+        b->lm_cap = cap0;
+        b->passes = passes0;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return rc ? rc : rc2 ? rc2 : fail(CSMC_EINVAL, "no sweep form could run");
+    }
+    b->lm_cap = best_k < 4 ? 3 - best_k : 3;
+    b->passes = best_k == 4;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (choice) *choice = best_k;
+    return CSMC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// the batch-wide colour passes (csmc_sweep_passes)
+int sweep_passes(csmc_ctx *c, Batch *b, int32_t kind, const double *T, const double *sigma, int32_t n_sweeps,
+                 uint64_t seed, double *accepted) {
     if (!valid_kind(kind)) return fail(CSMC_EINVAL, "unknown sweep kind");
     if (kind == CSMC_CONE_ADAPTIVE)
         return fail(CSMC_EINVAL, "the adaptive cone needs a host step per system (csmc_sweep_plain)");
@@ -1144,6 +1254,10 @@ int32_t csmc_sweep_passes(csmc_ctx *c, int32_t id, int32_t kind, const double *T
         for (int s = 0; s < b->nsys; ++s) accepted[s] = (double)got[s];
     return CSMC_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int32_t csmc_graph_sweeps(csmc_ctx *c, int32_t id, int32_t kind, const double *T, const double *sigma,
                           int32_t n_sweeps, uint64_t seed, csmc_graph **out) {

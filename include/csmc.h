@@ -239,6 +239,18 @@ int32_t csmc_pt_split_parts(csmc_ctx *ctx, int32_t batch_id, int32_t *parts);
 int32_t csmc_sweep_passes(csmc_ctx *ctx, int32_t batch_id, int32_t kind, const double *T, const double *sigma,
                           int32_t n_sweeps, uint64_t seed, double *accepted);
 
+/* Autotuner (no reference counterpart): time n_sweeps sweeps of `kind` (T
+ * for the Metropolis kinds) from the batch's current state in each form --
+ * the persistent kernel capped at storage mode 3, 2, 1, 0 (caps that select
+ * a mode already timed are skipped) and the batch-wide colour passes -- and
+ * keep the fastest for this batch's csmc_sweep calls.  The state (spins and
+ * Philox counters) is restored afterwards.  times[5] receives each form's
+ * best of two runs in ms (NaN where skipped), *choice its index.  The forms
+ * compute the same schedule with the same draws: storage modes agree bit for
+ * bit, the passes to rounding. */
+int32_t csmc_autotune(csmc_ctx *ctx, int32_t batch_id, int32_t kind, const double *T, int32_t n_sweeps,
+                      double *times, int32_t *choice);
+
 /* ---- graph capture (no reference counterpart: launch-bound loops replayed
  * as one hipGraph) ------------------------------------------------------------ */
 /* Capture n_sweeps sweeps of `kind` of the plain form (csmc_sweep_plain: one
