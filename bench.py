@@ -115,13 +115,18 @@ def finish(out, args):
 
 
 def fp32_error(csmc, W, lat, temps, ctx, R=8, steps=(1, 10)):
-    """Error of the FP32 state against the FP64 state on the device: the same
-    start (FP32-representable spins), the same schedule and Philox stream, in
-    an FP32 and an FP64 batch; the total energies (both reduced in FP64) after
-    1 and 10 exchange periods, relative, max over R replicas.  Trajectories
-    separate where a Metropolis decision falls differently, so the figure
-    after 10 periods bounds the drift of a run, not an evaluation error."""
-    p = dict(t_thermalization=0, t_measurement=10 ** 6, probe_rate=10, swap_rate=10, overrelaxation_rate=10,
+    """Error of the FP32 path against FP64, max over R replicas, relative:
+    - evaluation_energy_rel: the energy of the FP32 run's own state (after
+      the last period) from the FP32 batch against an FP64 batch holding the
+      same spins -- the error of an FP32 result;
+    - trajectory_*: an FP32 and an FP64 batch from the same start with the
+      same schedule and Philox stream, energies after 1 and 10 exchange
+      periods.  Trajectories separate where a Metropolis decision falls
+      differently, so these bound the drift of a run, not an evaluation
+      error."""
+    # t_thermalization = 1: with 0, sweep 0 is a checkpoint sweep and the
+    # first advance stops there
+    p = dict(t_thermalization=1, t_measurement=10 ** 6, probe_rate=10, swap_rate=10, overrelaxation_rate=10,
              report_interval=10 ** 9, checkpoint_rate=10 ** 9)
     b32 = csmc.SpinBatch([lat] * R, precision=32, ctx=ctx)
     b64 = csmc.SpinBatch([lat] * R, precision=64, ctx=ctx)
@@ -135,9 +140,13 @@ def fp32_error(csmc, W, lat, temps, ctx, R=8, steps=(1, 10)):
         b.pt_init(t, p, csmc.METROPOLIS, seed=4)
     for k in steps:
         for b in (b32, b64):
-            b.pt_advance(k * 10)
+            assert b.pt_advance(k * 10)[0] == k * 10
         e32, e64 = b32.total_energy(), b64.total_energy()
-        out[f"after_{k}_steps_energy_rel"] = float(np.max(np.abs(e32 - e64) / np.abs(e64)))
+        out[f"trajectory_after_{k}_periods_energy_rel"] = float(np.max(np.abs(e32 - e64) / np.abs(e64)))
+    for i in range(R):
+        b64.set_spins(i, b32.get_spins(i))
+    e32, e64 = b32.total_energy(), b64.total_energy()
+    out["evaluation_energy_rel"] = float(np.max(np.abs(e32 - e64) / np.abs(e64)))
     b32.close()
     b64.close()
     return out
@@ -156,6 +165,10 @@ def checkpoint_cost(csmc, b, temps):
                   overrelaxation_rate=10, report_interval=10 ** 9, checkpoint_rate=10 ** 9)
     b.pt_init(temps, params, csmc.METROPOLIS, seed=3)
     assert b.pt_advance(10240)[0] == 10240
+    # first use allocates the page-locked buffer: not timed
+    b.pt_save()
+    b.pt_save_async()
+    b.pt_save_wait()
     t0 = time.perf_counter()
     state = b.pt_save()
     t_sync = time.perf_counter() - t0
