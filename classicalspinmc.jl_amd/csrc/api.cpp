@@ -1237,7 +1237,11 @@ int sweep_passes(csmc_ctx *c, Batch *b, int32_t kind, const double *T, const dou
     for (int sw = 0; sw < n_sweeps; ++sw) {
         for (int col = 0; col < ncol; ++col) {  // systems with fewer colours skip the extra passes
             a.colour = col;
-            HIPCHK(launch_pass(b->prec, b->self, b->gen, b->nsys, maxcol, a, c->stream));
+            // systems go on the grid's y dimension, at most 65535 a launch
+            for (int s0 = 0; s0 < b->nsys; s0 += 65535) {
+                a.sys0 = s0;
+                HIPCHK(launch_pass(b->prec, b->self, b->gen, std::min(65535, b->nsys - s0), maxcol, a, c->stream));
+            }
         }
         if (metro)
             HIPCHK(launch_graph_after_sweep(b->mcount.p, b->accU.p, b->acc.p, b->sigma.p, 0, b->nsys, false, 1.0,
