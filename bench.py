@@ -245,7 +245,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1024,
                     help="untimed steps first (1024 = one whole chained launch)")
     ap.add_argument("--workload", default="pt_honeycomb",
-                    choices=["pt_honeycomb", "anneal_uneven", "pt_pyrochlore", "ssf", "probe_L512"])
+                    choices=["pt_honeycomb", "anneal_uneven", "pt_pyrochlore", "ssf", "probe_L512", "adjoint"])
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline")
@@ -273,6 +273,8 @@ def main():
         return bench_ssf(args, csmc, W)
     if args.workload == "probe_L512":
         return bench_probe(args, csmc, W)
+    if args.workload == "adjoint":
+        return bench_adjoint(args, csmc, W)
 
     L, R = args.L, args.replicas
     uc = W.honeycomb_kitaev(1.0, -2.0)
@@ -502,6 +504,81 @@ def bench_probe(args, csmc, W):
     b.close()
     log(f"[bench] probe: {value / 1e9:.2f} G updates/s, {achieved:.0f} GB/s algorithmic; persistent "
         f"{persistent / 1e9:.2f} G updates/s")
+    finish(out, args)
+
+
+def bench_adjoint(args, csmc, W):
+    """Differentiation workload: triangular J1-J2 lattices, L=96 (9216 sites),
+    S=1, J1=1 and J2 ~ U(0, 0.2) per configuration, a batch of 128 seeded FP64
+    configurations.  One step = one projected gradient-descent step of every
+    configuration on the device (dE/dS by k_grad_spins, then the update on
+    the sphere by k_gd).  Roofline of k_grad_spins: 48 algorithmic bytes per
+    site (read the FP64 spin, write the FP64 gradient)."""
+    import oracle.oracle as orc
+    rng = np.random.default_rng(5)
+    n, L = 128, 96
+    J2 = rng.uniform(0.0, 0.2, size=n)
+    cells = [W.triangular_j1j2(1.0, float(j)) for j in J2]
+    lats = [csmc.Lattice((L, L), uc, 1.0, seed=5 + k) for k, uc in enumerate(cells)]
+    N = lats[0].size
+    ctx = csmc.default_context()
+    b = csmc.SpinBatch(lats, precision=64, ctx=ctx)
+    b.randomize(5)
+    steps = max(10, min(args.steps, 200))
+    b.gradient_descent(0.05, 2)
+    ctx.sync()
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    b.gradient_descent(0.05, steps)
+    dt = time.perf_counter() - t0
+    kern_ms, launches = ctx.timing_read()
+    ctx.timing(False)
+    value = steps * n * N / dt
+    avg_ms = kern_ms / launches
+    achieved = 48.0 * n * N / (avg_ms * 1e-3) / 1e9
+    t0 = time.perf_counter()
+    for _ in range(5):
+        b.grad_couplings()
+    dj_ms = (time.perf_counter() - t0) / 5 * 1e3
+    out = {"metric": METRIC, "value": value, "unit": "spin-updates/s", "n_gpus": 1, "steps": steps, "warmup": 2,
+           "ms_per_step": dt / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp64", "data": "synthetic",
+           "config": {"workload": f"projected gradient descent on the sphere, {n} triangular J1-J2 lattices "
+                                  f"L={L} ({N} sites), J1=1, J2~U(0,0.2), FP64, dE/dS on the device each step",
+                      "systems": n, "sites": N},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                        "kernel": "k_grad_spins<double> (dE/dS of every site of every system)",
+                        "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": 48 * n * N},
+           "dEdJ_ms": round(dj_ms, 3)}
+    if not args.no_cpu:
+        # the oracle's local field (= dE/dS for this model) of every site,
+        # configurations spread over host threads (the C loop runs without
+        # the GIL), bounded in time
+        threads = max(1, args.cpu_threads)
+        tabs = [orc.Tables(cells[k], (L, L)) for k in range(threads)]
+        spins = [b.get_spins(k) for k in range(threads)]
+        calls = [0] * threads
+        stop = time.perf_counter() + min(args.cpu_seconds, 8.0)
+
+        def work(k):
+            while time.perf_counter() < stop:
+                orc.local_fields(tabs[k], spins[k])
+                calls[k] += 1
+
+        t0 = time.perf_counter()
+        pool = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+        for t in pool:
+            t.start()
+        for t in pool:
+            t.join()
+        cpu = sum(calls) * N / (time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": cpu, "unit": "site gradients/s", "cores": threads, "kind": "port",
+                               "sample": f"oracle local_fields (dE/dS) of whole configurations, {threads} host "
+                                         f"threads, {sum(calls)} configurations",
+                               "gpu_over_cpu": (n * N / (avg_ms * 1e-3)) / cpu}
+    b.close()
+    log(f"[bench] adjoint: {value / 1e9:.2f} G spin-updates/s, k_grad_spins {avg_ms:.3f} ms ({achieved:.0f} GB/s)")
     finish(out, args)
 
 

@@ -275,6 +275,42 @@ int coupling_flags(const HostLattice &L) {
     return flags;
 }
 
+// LAT_GRAD_LF: the energy's gradient equals the local field.  With the
+// 1/2 of hamiltonian.jl:131, dE/ds_x = 1/2 sum_k M_k s_j (x's own bonds) +
+// 1/2 sum M^T s_i (bonds i -> x); when every bond i -> j with M has a bond
+// j -> i with M^T (as multisets per pair of sites) the second sum equals
+// the first, and (O + O^T) s = 2 O s for symmetric O.
+bool gradient_is_local_field(const HostLattice &L) {
+    if (L.K3 > 0 || L.K4 > 0 || L.has_self) return false;
+    for (int b = 0; b < L.nb; ++b)
+        for (int u = 0; u < 3; ++u)
+            for (int v = u + 1; v < 3; ++v)
+                if (L.coef[L.coef_onsite + b * 9 + u * 3 + v] != L.coef[L.coef_onsite + b * 9 + v * 3 + u])
+                    return false;
+    const int64_t N = L.N;
+    auto mat = [&](int64_t ref, int k) { return &L.coef[L.coef_c2 + ((size_t)L.basis_of((int)ref) * L.K2 + k) * 9]; };
+    for (int64_t i = 0; i < N; ++i)
+        for (int k = 0; k < L.K2; ++k) {
+            const int64_t j = L.nbr2[(size_t)i * L.K2 + k];
+            if (j < 0) continue;
+            const double *M = mat(i, k);
+            // bonds i -> j with the same matrix, and bonds j -> i with M^T
+            int same = 0, back = 0;
+            for (int q = 0; q < L.K2; ++q) {
+                if (L.nbr2[(size_t)i * L.K2 + q] == j && std::equal(M, M + 9, mat(i, q))) ++same;
+                if (L.nbr2[(size_t)j * L.K2 + q] == i) {
+                    const double *T = mat(j, q);
+                    bool t = true;
+                    for (int u = 0; u < 3 && t; ++u)
+                        for (int v = 0; v < 3 && t; ++v) t = T[u * 3 + v] == M[v * 3 + u];
+                    back += t;
+                }
+            }
+            if (same != back) return false;
+        }
+    return true;
+}
+
 // append one lattice's device tables to the host-side pools
 void append_lattice(csmc_ctx *c, const HostLattice &L) {
     LatDev d{};
@@ -290,6 +326,7 @@ void append_lattice(csmc_ctx *c, const HostLattice &L) {
     d.has_self = L.has_self;
     d.S = L.S;
     d.flags = coupling_flags(L);
+    if (gradient_is_local_field(L)) d.flags |= LAT_GRAD_LF;
     auto &I = c->h_i32;
     auto dev = [&](int ref) { return ref < 0 ? -1 : L.dev_of_ref[ref]; };
     auto devp = [&](int ref) { return ref < 0 ? N : L.dev_of_ref[ref]; };  // absent -> pad site
@@ -2213,7 +2250,18 @@ int32_t csmc_gradient_descent(csmc_ctx *c, int32_t id, double eta, int32_t n_ste
     int rc = ensure_pools(c);
     if (rc) return rc;
     for (int k = 0; k < n_steps; ++k) {
+        // with timing on, events bracket each gradient kernel
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (c->timing) {
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            HIPCHK(hipEventRecord(e0, c->stream));
+        }
         HIPCHK(launch_grad_spins(b->prec, pools(c, b), b->spins.p, b->nsys, b->maxN, b->stage.p, c->stream));
+        if (c->timing) {
+            HIPCHK(hipEventRecord(e1, c->stream));
+            c->events.push_back({e0, e1});
+        }
         HIPCHK(launch_gd(b->prec, pools(c, b), b->spins.p, b->nsys, b->maxN, b->stage.p, eta, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));

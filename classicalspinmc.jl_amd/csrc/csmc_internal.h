@@ -97,6 +97,10 @@ constexpr int32_t LAT_COLBASIS = 8;  // every colour class holds sites of one ba
 // two colour classes, each of one basis and of the same whole number (1, 2
 // or 4) of REG_THREADS-site register-table slots
 constexpr int32_t LAT_TWO_COLOUR = 16;
+// dE/dS is the local field (get_local_field, hamiltonian.jl:3-67): bilinear
+// terms only, symmetric onsite matrices, no self-referencing sites, and every
+// bond i -> j with matrix M matched by a bond j -> i with M^T
+constexpr int32_t LAT_GRAD_LF = 32;
 
 // incidence entry: owner device site and a packed (kind, slot, role)
 //   code = kind(2 bits: 0 bil, 1 cub, 2 qua) | role << 2 (2 bits) | slot << 4

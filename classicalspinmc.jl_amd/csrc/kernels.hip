@@ -634,6 +634,19 @@ __global__ __launch_bounds__(256) void k_grad_spins(DevPools P, const void *spin
     const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
     const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
     auto sp = [&](int j) { return to_d<R>(S[j]); };
+    if (ld.flags & LAT_GRAD_LF) {
+        // the gradient is the local field (the host checked the pairing of
+        // every bond with its transpose): one gather over x's own bonds
+        const int b = L.sub[x];
+        double F[3], H[3];
+        interaction_field<double, double>(L, NbrG{L.nbr2, L.N}, b, x, sp, F);
+        local_field<double, double>(L, b, sp(x), F, H);
+        double *o = dEdS + 3 * (sd.site_off + L.ref[x]);
+        o[0] = H[0];
+        o[1] = H[1];
+        o[2] = H[2];
+        return;
+    }
     const int32_t *ioff = P.i32 + ld.inc_off;
     const Inc *inc = reinterpret_cast<const Inc *>(ioff + ld.N + 1);
     const int bx = L.sub[x];

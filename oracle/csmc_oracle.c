@@ -441,6 +441,12 @@ void orc_local_field(const orc_lat *L, const double *spins, long point, double *
 }
 
 /* energy(lattice, point) (hamiltonian.jl:139-196): no 1/2, 1/3, 1/4 */
+/* get_local_field for every site (points 1..N): H[N][3] */
+void orc_local_fields(const orc_lat *L, const double *spins, double *H)
+{
+    for (long p = 1; p <= L->N; ++p) orc_local_field(L, spins, p, H + 3 * (p - 1));
+}
+
 double orc_site_energy(const orc_lat *L, const double *spins, long point)
 {
     long i = point - 1;

@@ -143,7 +143,11 @@ def local_field(T, spins, point):
 
 
 def local_fields(T, spins):
-    return np.array([local_field(T, spins, p) for p in range(1, T.N + 1)])
+    """get_local_field of every site, (N, 3), in one C loop (no GIL held)"""
+    s = _spins(spins)
+    H = np.zeros((T.N, 3))
+    lib().orc_local_fields(T.ref, _p(s, _D), _p(H, _D))
+    return H
 
 
 def site_energy(T, spins, point):
