@@ -11,7 +11,8 @@ import pytest
 import csmc
 import oracle.oracle as orc
 from csmc import workloads as W
-from csmc.unitcell import Honeycomb, addBilinear, addCubic, addOnSite, addQuartic, addZeemanCoupling
+from csmc.unitcell import (Honeycomb, Square, addBasisSite, addBilinear, addCubic, addOnSite, addQuartic,
+                           addZeemanCoupling)
 from conftest import random_spins
 
 pytestmark = pytest.mark.gpu
@@ -32,10 +33,39 @@ def _random_cell(seed=21):
     return uc
 
 
+def _one_way_bond():
+    # bilinear only, but a bond without its reverse: dE/dS is not the local
+    # field, so the incidence gather must run
+    rng = np.random.default_rng(2)
+    uc = Square()
+    addBasisSite(uc, [0.0, 0.0])
+    M = rng.normal(size=(3, 3))
+    addBilinear(uc, 1, 1, M, (1, 0))
+    addBilinear(uc, 1, 1, M.T, (-1, 0))
+    addBilinear(uc, 1, 1, rng.normal(size=(3, 3)), (0, 1))  # one direction only
+    return uc
+
+
+def _asymmetric_onsite():
+    # every bond paired with its transpose, but a non-symmetric onsite matrix
+    rng = np.random.default_rng(4)
+    uc = Square()
+    addBasisSite(uc, [0.0, 0.0])
+    for d in ((1, 0), (0, 1)):
+        M = rng.normal(size=(3, 3))
+        addBilinear(uc, 1, 1, M, d)
+        addBilinear(uc, 1, 1, M.T, (-d[0], -d[1]))
+    addOnSite(uc, 1, rng.normal(size=(3, 3)))
+    return uc
+
+
 CASES = [
     ("random_terms", _random_cell, (3, 4)),
     ("pyro_mixed", W.pyrochlore_mixed, (2, 2, 2)),
     ("tri_j1j2", lambda: W.triangular_j1j2(1.0, 0.15), (6, 6)),
+    ("tri_dm_paired", lambda: W.triangular_dm(-0.9, 0.3, [0.2, -0.1, 0.4]), (6, 6)),
+    ("one_way_bond", _one_way_bond, (5, 4)),
+    ("asymmetric_onsite", _asymmetric_onsite, (4, 4)),
 ]
 
 
