@@ -579,9 +579,9 @@ def bench_adjoint(args, csmc, W):
     """Differentiation workload: triangular J1-J2 lattices, L=96 (9216 sites),
     S=1, J1=1 and J2 ~ U(0, 0.2) per configuration, a batch of 128 seeded FP64
     configurations.  One step = one projected gradient-descent step of every
-    configuration on the device (dE/dS by k_grad_spins, then the update on
-    the sphere by k_gd).  Roofline of k_grad_spins: 48 algorithmic bytes per
-    site (read the FP64 spin, write the FP64 gradient)."""
+    configuration on the device: k_gd_step computes dE/dS from the old spins
+    and writes the updated spins to a second buffer.  Roofline: 48
+    algorithmic bytes per site (read the FP64 spin, write the FP64 spin)."""
     import oracle.oracle as orc
     rng = np.random.default_rng(5)
     n, L = 128, 96
@@ -616,7 +616,7 @@ def bench_adjoint(args, csmc, W):
                       "systems": n, "sites": N},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                        "kernel": "k_grad_spins<double> (dE/dS of every site of every system)",
+                        "kernel": "k_gd_step<double> (dE/dS and the projected update of every site)",
                         "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": 48 * n * N},
            "dEdJ_ms": round(dj_ms, 3)}
     if not args.no_cpu:
@@ -646,7 +646,7 @@ def bench_adjoint(args, csmc, W):
                                          f"threads, {sum(calls)} configurations",
                                "gpu_over_cpu": (n * N / (avg_ms * 1e-3)) / cpu}
     b.close()
-    log(f"[bench] adjoint: {value / 1e9:.2f} G spin-updates/s, k_grad_spins {avg_ms:.3f} ms ({achieved:.0f} GB/s)")
+    log(f"[bench] adjoint: {value / 1e9:.2f} G spin-updates/s, k_gd_step {avg_ms:.3f} ms ({achieved:.0f} GB/s)")
     finish(out, args)
 
 

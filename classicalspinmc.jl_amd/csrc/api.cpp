@@ -150,6 +150,7 @@ struct Batch {
     std::vector<SysDev> sys;
     DevBuf<SysDev> sys_d;
     DevBuf<unsigned char> spins;
+    DevBuf<unsigned char> spins_alt;  // the other buffer of csmc_gradient_descent
     DevBuf<uint32_t> mcount;
     DevBuf<double> T, sigma, acc;  // per-system scratch
     DevBuf<uint32_t> accU;
@@ -2249,20 +2250,32 @@ int32_t csmc_gradient_descent(csmc_ctx *c, int32_t id, double eta, int32_t n_ste
     GET_BATCH(c, id, b);
     int rc = ensure_pools(c);
     if (rc) return rc;
+    if (n_steps < 1) return CSMC_OK;
+    // the steps alternate between the spin buffer and a second one (a copy,
+    // so its pad records are the same); every site is rewritten each step
+    // exactly the size of the spin buffer, so the two can trade places
+    if (!b->spins_alt.p || b->spins_alt.n != b->spins.n) HIPCHK(b->spins_alt.alloc(b->spins.n));
+    HIPCHK(hipMemcpyAsync(b->spins_alt.p, b->spins.p, b->spins.n, hipMemcpyDeviceToDevice, c->stream));
     for (int k = 0; k < n_steps; ++k) {
-        // with timing on, events bracket each gradient kernel
+        // with timing on, events bracket each step's kernel
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (c->timing) {
             HIPCHK(hipEventCreate(&e0));
             HIPCHK(hipEventCreate(&e1));
             HIPCHK(hipEventRecord(e0, c->stream));
         }
-        HIPCHK(launch_grad_spins(b->prec, pools(c, b), b->spins.p, b->nsys, b->maxN, b->stage.p, c->stream));
+        HIPCHK(launch_gd_step(b->prec, pools(c, b), b->spins.p, b->spins_alt.p, b->nsys, b->maxN, eta, c->stream));
         if (c->timing) {
             HIPCHK(hipEventRecord(e1, c->stream));
             c->events.push_back({e0, e1});
         }
-        HIPCHK(launch_gd(b->prec, pools(c, b), b->spins.p, b->nsys, b->maxN, b->stage.p, eta, c->stream));
+        std::swap(b->spins.p, b->spins_alt.p);  // the same size: only the roles change
+    }
+    // the spins stay in the batch's own buffer (captured graphs hold its
+    // address): after an odd number of steps, copy them back
+    if (n_steps % 2) {
+        std::swap(b->spins.p, b->spins_alt.p);
+        HIPCHK(hipMemcpyAsync(b->spins.p, b->spins_alt.p, b->spins.n, hipMemcpyDeviceToDevice, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return CSMC_OK;

@@ -143,6 +143,9 @@ hipError_t launch_grad_couplings(int prec, const DevPools &P, const void *spins,
 // push probes [0, n) of every slot's series into its binners, in order
 hipError_t launch_bin_fold(int R, const double *series, int cap, int n, double *binners,
                            hipStream_t st);
+// one fused gradient step from spins_old into spins_new (same layout)
+hipError_t launch_gd_step(int prec, const DevPools &P, const void *spins_old, void *spins_new, int nsys, int maxN,
+                          double eta, hipStream_t st);
 hipError_t launch_gd(int prec, const DevPools &P, void *spins, int nsys, int maxN,
                      const double *dEdS, double eta, hipStream_t st);
 // equal-time correlations of nsys systems of one lattice (N sites, device

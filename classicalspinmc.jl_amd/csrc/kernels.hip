@@ -625,26 +625,16 @@ hipError_t launch_gather_spins(int prec, const DevPools &P, const void *spins, i
 // entry's energy, weighted 1/2, 1/3, 1/4 as in hamiltonian.jl:131; onsite
 // contributes (O + O^T) s and Zeeman -h.
 template <class R>
-__global__ __launch_bounds__(256) void k_grad_spins(DevPools P, const void *spins, double *dEdS) {
-    const int sys = blockIdx.y;
-    const SysDev sd = P.sys[sys];
-    const LatDev ld = P.lats[sd.lat];
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= ld.N) return;
-    const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
-    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
+__device__ __forceinline__ void site_gradient(const DevPools &P, const LatDev &ld, const LatView<double> &L,
+                                              const spin4<R> *S, int x, double g[3]) {
     auto sp = [&](int j) { return to_d<R>(S[j]); };
     if (ld.flags & LAT_GRAD_LF) {
         // the gradient is the local field (the host checked the pairing of
         // every bond with its transpose): one gather over x's own bonds
         const int b = L.sub[x];
-        double F[3], H[3];
+        double F[3];
         interaction_field<double, double>(L, NbrG{L.nbr2, L.N}, b, x, sp, F);
-        local_field<double, double>(L, b, sp(x), F, H);
-        double *o = dEdS + 3 * (sd.site_off + L.ref[x]);
-        o[0] = H[0];
-        o[1] = H[1];
-        o[2] = H[2];
+        local_field<double, double>(L, b, sp(x), F, g);
         return;
     }
     const int32_t *ioff = P.i32 + ld.inc_off;
@@ -653,7 +643,6 @@ __global__ __launch_bounds__(256) void k_grad_spins(DevPools P, const void *spin
     spin4<double> s = sp(x);
     const double *O = L.onsite + bx * 9;
     const double *h = L.field + bx * 3;
-    double g[3];
     for (int a = 0; a < 3; ++a)
         g[a] = (O[a * 3 + 0] + O[0 * 3 + a]) * s.x + (O[a * 3 + 1] + O[1 * 3 + a]) * s.y +
                (O[a * 3 + 2] + O[2 * 3 + a]) * s.z - h[a];
@@ -710,8 +699,20 @@ __global__ __launch_bounds__(256) void k_grad_spins(DevPools P, const void *spin
                         }
         }
     }
-    int r = L.ref[x];
-    double *o = dEdS + 3 * (sd.site_off + r);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_grad_spins(DevPools P, const void *spins, double *dEdS) {
+    const int sys = blockIdx.y;
+    const SysDev sd = P.sys[sys];
+    const LatDev ld = P.lats[sd.lat];
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= ld.N) return;
+    const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
+    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
+    double g[3];
+    site_gradient<R>(P, ld, L, S, x, g);
+    double *o = dEdS + 3 * (sd.site_off + L.ref[x]);
     o[0] = g[0];
     o[1] = g[1];
     o[2] = g[2];
@@ -878,6 +879,45 @@ __global__ __launch_bounds__(256) void k_gd(DevPools P, void *spins, const doubl
     s.y = (R)(ny / nn * ld.S);
     s.z = (R)(nz / nn * ld.S);
     S[i] = s;
+}
+
+// one projected gradient step of every site from the old spins into a
+// second buffer: the gradient of site_gradient and the update of k_gd in one
+// pass (the result equals k_grad_spins followed by k_gd bit for bit)
+template <class R>
+__global__ __launch_bounds__(256) void k_gd_step(DevPools P, const void *spins_old, void *spins_new,
+                                                 double eta) {
+    const int sys = blockIdx.y;
+    const SysDev sd = P.sys[sys];
+    const LatDev ld = P.lats[sd.lat];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ld.N) return;
+    const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
+    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins_old) + sd.spin_off;
+    double g[3];
+    site_gradient<R>(P, ld, L, S, i, g);
+    spin4<R> s = S[i];
+    double sx = s.x, sy = s.y, sz = s.z;
+    double n = sqrt(sx * sx + sy * sy + sz * sz);
+    double ux = sx / n, uy = sy / n, uz = sz / n;
+    double gu = g[0] * ux + g[1] * uy + g[2] * uz;
+    double px = g[0] - gu * ux, py = g[1] - gu * uy, pz = g[2] - gu * uz;
+    double nx = sx - eta * px, ny = sy - eta * py, nz = sz - eta * pz;
+    double nn = sqrt(nx * nx + ny * ny + nz * nz);
+    s.x = (R)(nx / nn * ld.S);
+    s.y = (R)(ny / nn * ld.S);
+    s.z = (R)(nz / nn * ld.S);
+    reinterpret_cast<spin4<R> *>(spins_new)[sd.spin_off + i] = s;
+}
+
+hipError_t launch_gd_step(int prec, const DevPools &P, const void *spins_old, void *spins_new, int nsys, int maxN,
+                          double eta, hipStream_t st) {
+    dim3 grid((maxN + 255) / 256, nsys);
+    if (prec == 32)
+        hipLaunchKernelGGL((k_gd_step<float>), grid, dim3(256), 0, st, P, spins_old, spins_new, eta);
+    else
+        hipLaunchKernelGGL((k_gd_step<double>), grid, dim3(256), 0, st, P, spins_old, spins_new, eta);
+    return hipGetLastError();
 }
 
 hipError_t launch_gd(int prec, const DevPools &P, void *spins, int nsys, int maxN,

@@ -259,3 +259,38 @@ def test_batched_gradients_equal_single_system():
                 ref.set_spins(0, lj._spins)
                 ref.gradient_descent(0.01, 3)
                 assert np.array_equal(b.get_spins(j), ref.get_spins(0)), j
+
+
+@pytest.mark.parametrize("n_steps", [1, 2, 5])
+def test_fused_gradient_steps_equal_gradient_then_update(n_steps):
+    """csmc_gradient_descent (one fused kernel a step, two buffers) against
+    the gradient of grad_spins and the projected update made on the host:
+    equal to rounding for odd and even step counts, the spins back in the
+    batch's own buffer (a graph captured before still sees them)"""
+    uc = W.triangular_j1j2(1.0, 0.15)
+    lat = csmc.Lattice((6, 6), uc, 1.0, seed=8)
+    b = csmc.SpinBatch([lat], precision=64)
+    b.randomize(3)
+    g_or = b.graph_sweeps(csmc.OVERRELAX, n_sweeps=1)
+    eta = 0.05
+    s = b.get_spins(0)
+    ref = csmc.SpinBatch([lat], precision=64)
+    for _ in range(n_steps):
+        ref.set_spins(0, s)
+        g = ref.grad_spins()[0]
+        u = s / np.linalg.norm(s, axis=1, keepdims=True)
+        p = g - np.sum(g * u, axis=1, keepdims=True) * u
+        n = s - eta * p
+        s = n / np.linalg.norm(n, axis=1, keepdims=True) * 1.0
+    b.gradient_descent(eta, n_steps)
+    assert np.max(np.abs(b.get_spins(0) - s)) < 1e-12
+    # the captured graph works on the batch's spins after the steps
+    after = b.get_spins(0)
+    eager = csmc.SpinBatch([lat], precision=64)
+    eager.set_spins(0, after)
+    eager.sweep(csmc.OVERRELAX, n_sweeps=1, plain=True)
+    g_or.launch()
+    assert np.array_equal(b.get_spins(0), eager.get_spins(0))
+    g_or.close()
+    for x in (b, ref, eager):
+        x.close()
