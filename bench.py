@@ -20,13 +20,21 @@ oracle (CPU FP64 restatement of the reference's PT loop: random-site
 Metropolis, sequential overrelaxation, total energy per Metropolis sweep,
 even/odd exchange) on a bounded sample on the host cores.
 
---workload anneal_uneven and pt_pyrochlore measure the other batched
-workloads (independent lattices of uneven size annealed in one launch; the
-pyrochlore model with cubic, quartic and onsite terms under parallel
-tempering) with the same JSON fields.  --workload ssf measures the
-equal-time structure factor (compute_equal_time_correlations) of the 64
-replicas of the default workload at the 65 x 65 allowed momenta of one
-Brillouin-zone copy, against the FP64 MFMA roofline.
+Other workloads (--workload ...), each printing one line with the same
+fields:
+  anneal_square   the reference's CPU-runnable case: square L=16, simulated
+                  annealing to 1e-3 and deterministic updates, FP64
+  anneal_uneven   256 independent lattices of uneven size annealed in one
+                  launch
+  pt_pyrochlore   the pyrochlore model with cubic, quartic and onsite terms
+                  under parallel tempering
+  adjoint         projected gradient descent on 128 triangular J1-J2
+                  lattices, L=96, FP64
+  ssf             the equal-time structure factor of the 64 replicas of the
+                  default workload at the 65 x 65 allowed momenta of one
+                  zone copy, against the FP64 MFMA roofline
+  probe_L512      the default geometry at L=512 (537 MB of spin records,
+                  beyond the Infinity Cache) as batch-wide colour passes
 """
 import argparse
 import json
