@@ -28,47 +28,75 @@ PARAMS = dict(t_thermalization=20, t_measurement=60, probe_rate=5, swap_rate=10,
               report_interval=10 ** 9, checkpoint_rate=10 ** 9)
 
 
-@pytest.mark.parametrize("prec", [64, 32])
-def test_pt_correlations_match_a_stepped_oracle_replay(prec):
+def _ctx(split):
+    old = os.environ.get("CSMC_SPLIT")
+    os.environ["CSMC_SPLIT"] = split
+    try:
+        return csmc.Context(0)
+    finally:
+        if old is None:
+            os.environ.pop("CSMC_SPLIT", None)
+        else:
+            os.environ["CSMC_SPLIT"] = old
+
+
+# probes sparser than exchanges, so that chains of exchange periods (and, with
+# CSMC_SPLIT=4, the split kernel) run between them
+SPARSE = dict(PARAMS, probe_rate=20, swap_rate=5, t_measurement=100)
+
+
+@pytest.mark.parametrize("prec,shape,split,params", [(64, (6, 6), "1", PARAMS), (32, (6, 6), "1", PARAMS),
+                                                     (32, (32, 8), "4", SPARSE)],
+                         ids=["fp64", "fp32", "fp32_split4_chains"])
+def test_pt_correlations_match_a_stepped_oracle_replay(prec, shape, split, params):
     uc = W.honeycomb_kitaev(1.0, -2.0)
-    lat = csmc.Lattice((6, 6), uc, 0.5, seed=3)
+    lat = csmc.Lattice(shape, uc, 0.5, seed=3)
     R = 6
     temps = W.geometric_temperatures(0.2, 2.0, R)
-    ks = get_allowed_wavevectors(uc, (6, 6))
-    total = PARAMS["t_thermalization"] + PARAMS["t_measurement"]
-    print(f"[pt_corr] stepped replay fp{prec}", flush=True)
-    b = csmc.SpinBatch([lat] * R, precision=prec)
+    ks = get_allowed_wavevectors(uc, shape)
+    total = params["t_thermalization"] + params["t_measurement"]
+    print(f"[pt_corr] stepped replay fp{prec} {shape} split {split}", flush=True)
+    ctx = _ctx(split)
+    b = csmc.SpinBatch([lat] * R, precision=prec, ctx=ctx)
     b.randomize(17)
-    b.pt_init(temps, PARAMS, csmc.METROPOLIS, seed=23)
+    b.pt_init(temps, params, csmc.METROPOLIS, seed=23)
     b.pt_set_correlations(ks)
     acc = np.zeros((R, 9, ks.shape[1]))
     n = 0
     for s in range(total):
         stopped, ev = b.pt_advance(s + 1)
         assert stopped == s + 1
-        if s >= PARAMS["t_thermalization"] and s % PARAMS["probe_rate"] == 0:
+        if s >= params["t_thermalization"] and s % params["probe_rate"] == 0:
             sos = b.pt_state()["system_of_slot"]
             for T in range(R):
                 acc[T] += orc.equal_time_correlations(lat.site_positions, b.get_spins(int(sos[T])), ks)
             n += 1
     mean, count = b.pt_correlations()
-    assert count == n == 12
+    assert count == n == (12 if params is PARAMS else 5)
     assert mean.shape == (R, 9, ks.shape[1])
     assert _close(mean, acc / n)
     assert b.pt_state()["accepted_exchange"].sum() > 0
+    if split != "1":
+        assert b.pt_split_parts() > 1
     b.close()
 
     # the same run in one call (chained launches between the probes)
-    b = csmc.SpinBatch([lat] * R, precision=prec)
+    b = csmc.SpinBatch([lat] * R, precision=prec, ctx=ctx)
     b.randomize(17)
-    b.pt_init(temps, PARAMS, csmc.METROPOLIS, seed=23)
+    b.pt_init(temps, params, csmc.METROPOLIS, seed=23)
     b.pt_set_correlations(ks)
     stopped, ev = b.pt_advance(-1)
     assert ev == 3
     mean2, count2 = b.pt_correlations()
     assert count2 == n
-    assert np.array_equal(mean2, mean)
+    if split == "1":
+        assert np.array_equal(mean2, mean)
+    else:
+        # the split kernel sums the energies the exchanges use in another
+        # order: the same trajectory here, its S(k) equal to rounding
+        assert _close(mean2, mean)
     b.close()
+    ctx.close()
 
 
 def test_correlations_off_and_state_errors():
