@@ -316,6 +316,34 @@ pt_load!(b::Batch, state::Vector{UInt8}) =
     check(ccall((:csmc_pt_load, libcsmc), Int32, (Ptr{Cvoid}, Int32, Ptr{UInt8}, Int64),
                 b.ctx.h, b.id, state, length(state)))
 
+# ---- launch forms (no reference counterpart) ----------------------------------------
+
+"""Time each sweep form on the batch's own state and keep the fastest
+(csmc_autotune); returns the per-form times in ms (NaN: skipped) and the
+chosen form's index."""
+function autotune!(b::Batch, kind::Int32, T::Vector{Float64}; n_sweeps::Integer=3)
+    times = zeros(Float64, 5)
+    choice = Ref{Int32}(0)
+    check(ccall((:csmc_autotune, libcsmc), Int32,
+                (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Int32, Ptr{Float64}, Ref{Int32}),
+                b.ctx.h, b.id, kind, T, n_sweeps, times, choice))
+    return times, choice[]
+end
+
+"""n_sweeps plain-form sweeps captured as one hipGraph; replay with
+graph_launch!(b, g), free with graph_destroy!(g)."""
+function graph_sweeps(b::Batch, kind::Int32, T::Vector{Float64}, sigma::Vector{Float64}; n_sweeps::Integer=1,
+                      seed::UInt64=UInt64(0))
+    g = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:csmc_graph_sweeps, libcsmc), Int32,
+                (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Ptr{Float64}, Int32, UInt64, Ref{Ptr{Cvoid}}),
+                b.ctx.h, b.id, kind, T, sigma, n_sweeps, seed, g))
+    return g[]
+end
+graph_launch!(b::Batch, g::Ptr{Cvoid}) =
+    check(ccall((:csmc_graph_launch, libcsmc), Int32, (Ptr{Cvoid}, Ptr{Cvoid}), b.ctx.h, g))
+graph_destroy!(g::Ptr{Cvoid}) = check(ccall((:csmc_graph_destroy, libcsmc), Int32, (Ptr{Cvoid},), g))
+
 # ---- differentiation ------------------------------------------------------------
 
 """dE/dS for every site of every system of `b` (3 x N per system)."""
