@@ -9,6 +9,8 @@ with the public unit-cell API (the same calls a user of the reference makes).
                            transpose on reverse bonds
   pyrochlore_mixed         pyrochlore NN Heisenberg + biquadratic quartic +
                            tetrahedron-face scalar chirality + <111> onsite
+  tensor_pattern_cell      honeycomb with cubic / quartic couplings of every
+                           sparsity pattern the field kernels distinguish
   triangular_j1j2          isotropic J1, J2 on the triangular lattice
 """
 import math
@@ -118,6 +120,35 @@ def pyrochlore_mixed(J1=1.0, B=-0.1, chi=0.05, A=-0.2):
     for b in range(4):
         z = LOCAL_Z[b]
         addOnSite(uc, b + 1, A * np.outer(z, z))
+    return uc
+
+
+def tensor_pattern_cell(seed=5):
+    """honeycomb cell whose cubic and quartic couplings take every sparsity
+    pattern the field kernels distinguish, with random weights:
+    quartic (Si.Sk)(Sj.Sl)-type (role 0 sees p = r, q = t; role 1 p = t,
+    q = r), (Si.Sj)(Sk.Sl)-type with anisotropic weights (p = q, r = t), a
+    dense quartic on basis 1 only (so basis 2 has an all-zero padding slot);
+    a cubic with non-zeros only at distinct indices, and a dense cubic on
+    basis 2 only.  Bilinear Heisenberg bonds keep the spins coupled."""
+    rng = np.random.default_rng(seed)
+    uc = Honeycomb()
+    d = np.eye(3)
+    for o in ((0, 0), (1, -1), (0, -1)):
+        addBilinear(uc, 1, 2, -1.0 * I3, o)
+    W = rng.normal(size=(3, 3))
+    addQuartic(uc, 1, 2, 1, 2, np.einsum("pr,qt,pq->pqrt", d, d, W), (0, 0), (1, 0), (1, -1))
+    V = rng.normal(size=(3, 3))
+    addQuartic(uc, 2, 1, 2, 1, np.einsum("pq,rt,pr->pqrt", d, d, V), (0, 1), (-1, 1), (-1, 2))
+    addQuartic(uc, 1, 1, 1, 1, 0.1 * rng.normal(size=(3, 3, 3, 3)), (1, 0), (0, 1), (1, 1))
+    alt = np.zeros((3, 3, 3))
+    for p in range(3):
+        for q in range(3):
+            for r in range(3):
+                if len({p, q, r}) == 3:
+                    alt[p, q, r] = rng.normal()
+    addCubic(uc, 1, 2, 1, alt, (0, 0), (1, 0))
+    addCubic(uc, 2, 2, 2, 0.1 * rng.normal(size=(3, 3, 3)), (1, 0), (0, 1))
     return uc
 
 

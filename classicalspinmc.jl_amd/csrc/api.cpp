@@ -203,6 +203,9 @@ struct csmc_ctx {
     void *blas = nullptr;    // rocBLAS handle (created on first use)
     bool ssf_blas = false;   // equal-time correlations through rocBLAS
                              // instead of the fused MFMA kernel
+    bool dense_tensors = false;  // contract every cubic / quartic coupling in
+                                 // full (no sparsity patterns)
+    bool verbose = false;        // CSMC_VERBOSE=1: device and lattice notes on stderr
     uint32_t spin_limit = 1u << 26;  // polls of a wait for another workgroup
                                      // (x 1/16 for the exchange waits)
     bool timing = false;
@@ -312,6 +315,40 @@ bool gradient_is_local_field(const HostLattice &L) {
     return true;
 }
 
+// Sparsity pattern (TP_*) of every cubic coupling [nb][K3], then of every
+// quartic one [nb][K4].  Entry e of a cubic tensor is p + 3q + 9r, of a
+// quartic one p + 3q + 9r + 27t (p the site's own component).  The first
+// pattern that holds every non-zero entry is taken, so an all-zero slot is
+// TP_ZERO; dense = true leaves every non-zero slot TP_DENSE.
+std::vector<int32_t> tensor_patterns(const HostLattice &L, bool dense) {
+    std::vector<int32_t> out;
+    for (long m = 0; m < (long)L.nb * L.K3; ++m) {
+        const double *C = &L.coef[L.coef_c3 + m * 27];
+        bool zero = true, alt = true;
+        for (int e = 0; e < 27; ++e) {
+            if (C[e] == 0.0) continue;
+            const int p = e % 3, q = e / 3 % 3, r = e / 9;
+            zero = false;
+            alt = alt && p != q && q != r && p != r;
+        }
+        out.push_back(zero ? TP_ZERO : (alt && !dense) ? TP_C3_ALT : TP_DENSE);
+    }
+    for (long m = 0; m < (long)L.nb * L.K4; ++m) {
+        const double *Q = &L.coef[L.coef_c4 + m * 81];
+        bool zero = true, pq = true, pr = true, pt = true;
+        for (int e = 0; e < 81; ++e) {
+            if (Q[e] == 0.0) continue;
+            const int p = e % 3, q = e / 3 % 3, r = e / 9 % 3, t = e / 27;
+            zero = false;
+            pq = pq && p == q && r == t;
+            pr = pr && p == r && q == t;
+            pt = pt && p == t && q == r;
+        }
+        out.push_back(zero ? TP_ZERO : dense ? TP_DENSE : pq ? TP_Q4_PQ : pr ? TP_Q4_PR : pt ? TP_Q4_PT : TP_DENSE);
+    }
+    return out;
+}
+
 // append one lattice's device tables to the host-side pools
 void append_lattice(csmc_ctx *c, const HostLattice &L) {
     LatDev d{};
@@ -380,6 +417,20 @@ void append_lattice(csmc_ctx *c, const HostLattice &L) {
         const int spc = size[0] / REG_THREADS;
         if (size[0] == size[1] && size[0] % REG_THREADS == 0 && (spc == 1 || spc == 2 || spc == 4))
             d.flags |= LAT_TWO_COLOUR;
+    }
+    d.pat_off = (int64_t)I.size();
+    {
+        const std::vector<int32_t> pat = tensor_patterns(L, c->dense_tensors);
+        I.insert(I.end(), pat.begin(), pat.end());
+        if (c->verbose) {
+            int n3[5] = {0}, n4[5] = {0};
+            for (size_t m = 0; m < pat.size(); ++m) ++(m < (size_t)L.nb * L.K3 ? n3 : n4)[pat[m]];
+            fprintf(stderr,
+                    "csmc: lattice of %d sites, cubic slots dense %d alt %d zero %d, quartic slots dense %d "
+                    "pq %d pr %d pt %d zero %d\n",
+                    N, n3[TP_DENSE], n3[TP_C3_ALT], n3[TP_ZERO], n4[TP_DENSE], n4[TP_Q4_PQ], n4[TP_Q4_PR],
+                    n4[TP_Q4_PT], n4[TP_ZERO]);
+        }
     }
     // incidence lists: participant -> (owner, kind, role, slot)
     std::vector<std::vector<Inc>> inc(N);
@@ -727,8 +778,11 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
     }
     const char *sf = getenv("CSMC_SSF");
     if (sf && std::string(sf) == "blas") c->ssf_blas = true;
+    const char *dt = getenv("CSMC_DENSE_TENSORS");
+    if (dt && dt[0] == '1') c->dense_tensors = true;
     const char *vb = getenv("CSMC_VERBOSE");
-    if (vb && vb[0] == '1') {
+    c->verbose = vb && vb[0] == '1';
+    if (c->verbose) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess)
             fprintf(stderr, "csmc: device %d %s (%s), %d CUs, max LDS per block %d B%s\n", device,
@@ -879,7 +933,10 @@ int32_t csmc_lattice_update_couplings(csmc_ctx *c, int32_t id, const csmc_unit_c
     if (rc) return fail(rc, err);
     std::vector<double> dc = device_coef(L);
     std::copy(dc.begin(), dc.end(), c->h_coef.begin() + c->h_lat[id].coef_off);
-    c->h_lat[id].flags = coupling_flags(L) | (c->h_lat[id].flags & (LAT_COLBASIS | LAT_TWO_COLOUR));
+    c->h_lat[id].flags = coupling_flags(L) | (gradient_is_local_field(L) ? LAT_GRAD_LF : 0) |
+                         (c->h_lat[id].flags & (LAT_COLBASIS | LAT_TWO_COLOUR));
+    const std::vector<int32_t> pat = tensor_patterns(L, c->dense_tensors);
+    std::copy(pat.begin(), pat.end(), c->h_i32.begin() + c->h_lat[id].pat_off);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->dirty = true;
     return CSMC_OK;

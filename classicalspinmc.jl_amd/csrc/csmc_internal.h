@@ -9,7 +9,8 @@
 //              ref[N] (reference index of
 //              each device site), sub[N] (basis of each device site, 0-based),
 //              colrun[ncol+1] (run index boundaries per colour), incidence CSR
-//              for the spin gradient.
+//              for the spin gradient, the sparsity pattern of each cubic and
+//              quartic coupling.
 //   run pool   int4 {start, end, sub, colour}: maximal ranges of device sites
 //              that share colour and basis.  Device sites are ordered by
 //              (colour, basis, reference index), so every colour class is one
@@ -49,6 +50,8 @@ struct LatDev {
     int64_t slot_off;    // run pool: int4 {first site, run end, basis, colour}
     int64_t colslot_off; // i32 pool: first slot of each colour [ncol + 1], then
                          // the basis all slots of a colour share (-1 if mixed) [ncol]
+    int64_t pat_off;     // i32 pool: sparsity pattern (TP_*) of each cubic
+                         // coupling [nb][K3], then of each quartic one [nb][K4]
     double S;
 };
 
@@ -101,6 +104,19 @@ constexpr int32_t LAT_TWO_COLOUR = 16;
 // terms only, symmetric onsite matrices, no self-referencing sites, and every
 // bond i -> j with matrix M matched by a bond j -> i with M^T
 constexpr int32_t LAT_GRAD_LF = 32;
+
+// Sparsity pattern of one (basis, slot) cubic or quartic coupling tensor,
+// indices p (the site's own component), q, r, t (the partners in slot
+// order).  A pattern names where the tensor may be non-zero; the field
+// kernels then skip the products with the zero entries.  The terms that
+// remain are added in the same order as in the full contraction, and a
+// skipped product is an exact zero, so the sum is the same.
+constexpr int32_t TP_DENSE = 0;
+constexpr int32_t TP_ZERO = 1;    // every entry zero (a padding slot)
+constexpr int32_t TP_C3_ALT = 2;  // cubic: p, q, r pairwise distinct (S.(S x S))
+constexpr int32_t TP_Q4_PQ = 2;   // quartic: p = q and r = t ((Si.Sj)(Sk.Sl))
+constexpr int32_t TP_Q4_PR = 3;   // quartic: p = r and q = t ((Si.Sk)(Sj.Sl))
+constexpr int32_t TP_Q4_PT = 4;   // quartic: p = t and q = r ((Si.Sl)(Sj.Sk))
 
 // incidence entry: owner device site and a packed (kind, slot, role)
 //   code = kind(2 bits: 0 bil, 1 cub, 2 qua) | role << 2 (2 bits) | slot << 4

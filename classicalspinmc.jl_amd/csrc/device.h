@@ -27,7 +27,7 @@ __device__ __forceinline__ T ldc(const T *p) {
 template <class R>
 struct LatView {
     int N, nb, K2, K2P, K3, K4, ncol, has_self, flags;
-    const int32_t *nbr2, *nbr3, *nbr4, *ref, *sub, *colrun;
+    const int32_t *nbr2, *nbr3, *nbr4, *ref, *sub, *colrun, *pat3, *pat4;
     const int4 *runs;
     const R *onsite, *field, *c2, *c3, *c4;
     R S;
@@ -52,6 +52,8 @@ __device__ __forceinline__ LatView<R> lat_view(const LatDev &d, const int32_t *i
     v.ref = i32 + d.ref_off;
     v.sub = i32 + d.sub_off;
     v.colrun = i32 + d.colrun_off;
+    v.pat3 = i32 + d.pat_off;
+    v.pat4 = v.pat3 + d.nb * d.K3;
     v.runs = runs;
     const R *c = coef + d.coef_off;
     v.onsite = c;
@@ -237,6 +239,67 @@ __device__ __forceinline__ void bilinear_field(const LatView<R> &L, int b, const
     }
 }
 
+// Entries a sparsity pattern keeps (TP_* of csmc_internal.h); p is the
+// site's own component, q, r, t index the partners' spins.
+__device__ constexpr bool c3_kept(int pat, int p, int q, int r) {
+    return pat == TP_DENSE || (p != q && q != r && p != r);
+}
+__device__ constexpr bool q4_kept(int pat, int p, int q, int r, int t) {
+    return pat == TP_DENSE || (pat == TP_Q4_PQ && p == q && r == t) ||
+           (pat == TP_Q4_PR && p == r && q == t) || (pat == TP_Q4_PT && p == t && q == r);
+}
+
+// F_p += sum_q (sum_r C[p,q,r] sk_r) sj_q for one cubic slot
+// (hamiltonian.jl:36-49, at the site's position).  Every loop is
+// unrolled, so the pattern tests are resolved at compile time and the
+// products kept are added in the order of the full contraction.
+template <int PAT, class R, class A>
+__device__ __forceinline__ void cubic_term(const R *C, const A sj[3], const A sk[3], A F[3]) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        A acc = 0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (!(c3_kept(PAT, p, q, 0) || c3_kept(PAT, p, q, 1) || c3_kept(PAT, p, q, 2))) continue;
+            A t = 0;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                if (c3_kept(PAT, p, q, r)) t += (A)ldc(C + p + 3 * q + 9 * r) * sk[r];
+            acc += t * sj[q];
+        }
+        F[p] += acc;
+    }
+}
+
+// F_p += sum_q (sum_{r,t} Q[p,q,r,t] sk_r sl_t) sj_q for one quartic slot
+// (hamiltonian.jl:51-65), with the pattern's entries only
+template <int PAT, class R, class A>
+__device__ __forceinline__ void quartic_term(const R *Q, const A sj[3], const A sk[3], const A sl[3],
+                                             A F[3]) {
+    A w[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int t = 0; t < 3; ++t) w[r + 3 * t] = sk[r] * sl[t];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        A acc = 0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            bool any = false;
+#pragma unroll
+            for (int rt = 0; rt < 9; ++rt) any = any || q4_kept(PAT, p, q, rt % 3, rt / 3);
+            if (!any) continue;
+            A t = 0;
+#pragma unroll
+            for (int rt = 0; rt < 9; ++rt)
+                if (q4_kept(PAT, p, q, rt % 3, rt / 3)) t += (A)ldc(Q + p + 3 * q + 9 * rt) * w[rt];
+            acc += t * sj[q];
+        }
+        F[p] += acc;
+    }
+}
+
 // K2C > 0: the lattice has exactly K2C bilinear slots (compile-time), so
 // the slot loop is fully unrolled with no padding work.
 template <class R, class A, bool GEN = true, int K2C = 0, class NB, class LD>
@@ -269,48 +332,35 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
     }
     if (!GEN) return;
     for (int k = 0; k < L.K3; ++k) {
+        const int pat = ldc(L.pat3 + b * L.K3 + k);
+        if (pat == TP_ZERO) continue;
         int j = L.nbr3[((int64_t)k * 2) * L.N + i];
         int kk = L.nbr3[((int64_t)k * 2 + 1) * L.N + i];
         spin4<R> a = ld(j), c = ld(kk);
         A sj[3] = {a.x, a.y, a.z}, sk[3] = {c.x, c.y, c.z};
         const R *C = L.c3 + (b * L.K3 + k) * 27;
-#pragma unroll
-        for (int p = 0; p < 3; ++p) {
-            A acc = 0;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                A t = 0;
-#pragma unroll
-                for (int r = 0; r < 3; ++r) t += (A)ldc(C + p + 3 * q + 9 * r) * sk[r];
-                acc += t * sj[q];
-            }
-            F[p] += acc;
-        }
+        if (pat == TP_C3_ALT)
+            cubic_term<TP_C3_ALT, R>(C, sj, sk, F);
+        else
+            cubic_term<TP_DENSE, R>(C, sj, sk, F);
     }
     for (int k = 0; k < L.K4; ++k) {
+        const int pat = ldc(L.pat4 + b * L.K4 + k);
+        if (pat == TP_ZERO) continue;
         int j = L.nbr4[((int64_t)k * 3) * L.N + i];
         int kk = L.nbr4[((int64_t)k * 3 + 1) * L.N + i];
         int l = L.nbr4[((int64_t)k * 3 + 2) * L.N + i];
         spin4<R> a = ld(j), c = ld(kk), e = ld(l);
         A sj[3] = {a.x, a.y, a.z}, sk[3] = {c.x, c.y, c.z}, sl[3] = {e.x, e.y, e.z};
-        A w[9];
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int t = 0; t < 3; ++t) w[r + 3 * t] = sk[r] * sl[t];
         const R *Q = L.c4 + (b * L.K4 + k) * 81;
-#pragma unroll
-        for (int p = 0; p < 3; ++p) {
-            A acc = 0;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                A t = 0;
-#pragma unroll
-                for (int rt = 0; rt < 9; ++rt) t += (A)ldc(Q + p + 3 * q + 9 * rt) * w[rt];
-                acc += t * sj[q];
-            }
-            F[p] += acc;
-        }
+        if (pat == TP_Q4_PQ)
+            quartic_term<TP_Q4_PQ, R>(Q, sj, sk, sl, F);
+        else if (pat == TP_Q4_PR)
+            quartic_term<TP_Q4_PR, R>(Q, sj, sk, sl, F);
+        else if (pat == TP_Q4_PT)
+            quartic_term<TP_Q4_PT, R>(Q, sj, sk, sl, F);
+        else
+            quartic_term<TP_DENSE, R>(Q, sj, sk, sl, F);
     }
 }
 

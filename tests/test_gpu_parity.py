@@ -41,6 +41,7 @@ CASES = [
     ("cfg3_tri_dm", lambda: W.triangular_dm(-0.9, -0.05, [0.3, 0.1, 0.0]), (24, 16), 1.0, "periodic"),
     ("cfg4_pyro", lambda: W.pyrochlore_mixed(), (4, 4, 4), 1.0, "periodic"),
     ("cfg5_tri_j1j2", lambda: W.triangular_j1j2(1.0, 0.13), (24, 24), 1.0, "periodic"),
+    ("tensor_patterns", W.tensor_pattern_cell, (4, 4), 1.0, "periodic"),
     ("random_terms", _random_cell, (3, 4), 1.0, "periodic"),
     ("open_kgg", W.kgg_test_cell, (5, 3), 1.0, "open"),
     ("tiny_L1", lambda: W.square_heisenberg(), (1, 1), 1.0, "periodic"),
