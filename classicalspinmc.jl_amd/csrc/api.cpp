@@ -365,6 +365,14 @@ void append_lattice(csmc_ctx *c, const HostLattice &L) {
     d.S = L.S;
     d.flags = coupling_flags(L);
     if (gradient_is_local_field(L)) d.flags |= LAT_GRAD_LF;
+    if (L.has_self) {
+        bool bc = false;  // a bilinear or cubic slot names its own site
+        for (int64_t s = 0; s < L.N && !bc; ++s) {
+            for (int k = 0; k < L.K2; ++k) bc = bc || L.nbr2[(size_t)s * L.K2 + k] == s;
+            for (int k = 0; k < L.K3 * 2; ++k) bc = bc || L.nbr3[(size_t)s * L.K3 * 2 + k] == s;
+        }
+        if (!bc) d.flags |= LAT_SELF_Q4;
+    }
     auto &I = c->h_i32;
     auto dev = [&](int ref) { return ref < 0 ? -1 : L.dev_of_ref[ref]; };
     auto devp = [&](int ref) { return ref < 0 ? N : L.dev_of_ref[ref]; };  // absent -> pad site
@@ -934,7 +942,7 @@ int32_t csmc_lattice_update_couplings(csmc_ctx *c, int32_t id, const csmc_unit_c
     std::vector<double> dc = device_coef(L);
     std::copy(dc.begin(), dc.end(), c->h_coef.begin() + c->h_lat[id].coef_off);
     c->h_lat[id].flags = coupling_flags(L) | (gradient_is_local_field(L) ? LAT_GRAD_LF : 0) |
-                         (c->h_lat[id].flags & (LAT_COLBASIS | LAT_TWO_COLOUR));
+                         (c->h_lat[id].flags & (LAT_COLBASIS | LAT_TWO_COLOUR | LAT_SELF_Q4));
     const std::vector<int32_t> pat = tensor_patterns(L, c->dense_tensors);
     std::copy(pat.begin(), pat.end(), c->h_i32.begin() + c->h_lat[id].pat_off);
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -104,6 +104,9 @@ constexpr int32_t LAT_TWO_COLOUR = 16;
 // terms only, symmetric onsite matrices, no self-referencing sites, and every
 // bond i -> j with matrix M matched by a bond j -> i with M^T
 constexpr int32_t LAT_GRAD_LF = 32;
+// some quartic slot names the site itself, and no bilinear or cubic slot
+// does (a Metropolis step then evaluates only the quartic part twice)
+constexpr int32_t LAT_SELF_Q4 = 64;
 
 // Sparsity pattern of one (basis, slot) cubic or quartic coupling tensor,
 // indices p (the site's own component), q, r, t (the partners in slot

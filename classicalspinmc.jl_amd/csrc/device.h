@@ -302,11 +302,13 @@ __device__ __forceinline__ void quartic_term(const R *Q, const A sj[3], const A 
 
 // K2C > 0: the lattice has exactly K2C bilinear slots (compile-time), so
 // the slot loop is fully unrolled with no padding work.
-template <class R, class A, bool GEN = true, int K2C = 0, class NB, class LD>
+// PART 0: the whole field; 1: its bilinear and cubic part; 2: add the
+// quartic part to F (0 is 1 followed by 2, with the same operations).
+template <class R, class A, bool GEN = true, int K2C = 0, int PART = 0, class NB, class LD>
 __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB &nb, int b, int i,
                                                   LD &&ld, A F[3]) {
-    F[0] = F[1] = F[2] = A(0);
-    if constexpr (K2C > 0) {
+    if (PART != 2) F[0] = F[1] = F[2] = A(0);
+    if constexpr (K2C > 0 && PART != 2) {
         int j[K2C];
 #pragma unroll
         for (int k = 0; k < K2C; ++k) j[k] = nb.at(k, i);
@@ -315,7 +317,7 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
         for (int k = 0; k < K2C; ++k) sj[k] = ld(j[k]);
         bilinear_field<R, A, K2C, false>(L, b, sj, F);
     }
-    for (int k0 = 0; K2C == 0 && k0 < L.K2P; k0 += 4) {
+    for (int k0 = 0; K2C == 0 && PART != 2 && k0 < L.K2P; k0 += 4) {
         int j[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) j[u] = nb(k0 + u, i);
@@ -331,36 +333,69 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
         }
     }
     if (!GEN) return;
-    for (int k = 0; k < L.K3; ++k) {
-        const int pat = ldc(L.pat3 + b * L.K3 + k);
-        if (pat == TP_ZERO) continue;
-        int j = L.nbr3[((int64_t)k * 2) * L.N + i];
-        int kk = L.nbr3[((int64_t)k * 2 + 1) * L.N + i];
-        spin4<R> a = ld(j), c = ld(kk);
-        A sj[3] = {a.x, a.y, a.z}, sk[3] = {c.x, c.y, c.z};
-        const R *C = L.c3 + (b * L.K3 + k) * 27;
-        if (pat == TP_C3_ALT)
-            cubic_term<TP_C3_ALT, R>(C, sj, sk, F);
-        else
-            cubic_term<TP_DENSE, R>(C, sj, sk, F);
+    // GS slots at a time: their patterns, partner indices and partner spins
+    // are loaded before any of them is used, so the loads of a group are in
+    // flight together (a group past the last slot repeats the last slot's
+    // loads and skips its product)
+    constexpr int GS = 3;
+    for (int k0 = 0; PART != 2 && k0 < L.K3; k0 += GS) {
+        int pat[GS], j[GS][2];
+#pragma unroll
+        for (int u = 0; u < GS; ++u) {
+            const int k = min(k0 + u, L.K3 - 1);
+            const int p = ldc(L.pat3 + b * L.K3 + k);
+            pat[u] = k0 + u < L.K3 ? p : TP_ZERO;
+            j[u][0] = L.nbr3[((int64_t)k * 2) * L.N + i];
+            j[u][1] = L.nbr3[((int64_t)k * 2 + 1) * L.N + i];
+        }
+        spin4<R> sp[GS][2];
+#pragma unroll
+        for (int u = 0; u < GS; ++u) {
+            sp[u][0] = ld(j[u][0]);
+            sp[u][1] = ld(j[u][1]);
+        }
+#pragma unroll
+        for (int u = 0; u < GS; ++u) {
+            if (pat[u] == TP_ZERO) continue;
+            const A sj[3] = {sp[u][0].x, sp[u][0].y, sp[u][0].z}, sk[3] = {sp[u][1].x, sp[u][1].y, sp[u][1].z};
+            const R *C = L.c3 + (b * L.K3 + k0 + u) * 27;
+            if (pat[u] == TP_C3_ALT)
+                cubic_term<TP_C3_ALT, R>(C, sj, sk, F);
+            else
+                cubic_term<TP_DENSE, R>(C, sj, sk, F);
+        }
     }
-    for (int k = 0; k < L.K4; ++k) {
-        const int pat = ldc(L.pat4 + b * L.K4 + k);
-        if (pat == TP_ZERO) continue;
-        int j = L.nbr4[((int64_t)k * 3) * L.N + i];
-        int kk = L.nbr4[((int64_t)k * 3 + 1) * L.N + i];
-        int l = L.nbr4[((int64_t)k * 3 + 2) * L.N + i];
-        spin4<R> a = ld(j), c = ld(kk), e = ld(l);
-        A sj[3] = {a.x, a.y, a.z}, sk[3] = {c.x, c.y, c.z}, sl[3] = {e.x, e.y, e.z};
-        const R *Q = L.c4 + (b * L.K4 + k) * 81;
-        if (pat == TP_Q4_PQ)
-            quartic_term<TP_Q4_PQ, R>(Q, sj, sk, sl, F);
-        else if (pat == TP_Q4_PR)
-            quartic_term<TP_Q4_PR, R>(Q, sj, sk, sl, F);
-        else if (pat == TP_Q4_PT)
-            quartic_term<TP_Q4_PT, R>(Q, sj, sk, sl, F);
-        else
-            quartic_term<TP_DENSE, R>(Q, sj, sk, sl, F);
+    if (PART == 1) return;
+    for (int k0 = 0; k0 < L.K4; k0 += GS) {
+        int pat[GS], j[GS][3];
+#pragma unroll
+        for (int u = 0; u < GS; ++u) {
+            const int k = min(k0 + u, L.K4 - 1);
+            const int p = ldc(L.pat4 + b * L.K4 + k);
+            pat[u] = k0 + u < L.K4 ? p : TP_ZERO;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) j[u][q] = L.nbr4[((int64_t)k * 3 + q) * L.N + i];
+        }
+        spin4<R> sp[GS][3];
+#pragma unroll
+        for (int u = 0; u < GS; ++u)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) sp[u][q] = ld(j[u][q]);
+#pragma unroll
+        for (int u = 0; u < GS; ++u) {
+            if (pat[u] == TP_ZERO) continue;
+            const A sj[3] = {sp[u][0].x, sp[u][0].y, sp[u][0].z}, sk[3] = {sp[u][1].x, sp[u][1].y, sp[u][1].z},
+                    sl[3] = {sp[u][2].x, sp[u][2].y, sp[u][2].z};
+            const R *Q = L.c4 + (b * L.K4 + k0 + u) * 81;
+            if (pat[u] == TP_Q4_PQ)
+                quartic_term<TP_Q4_PQ, R>(Q, sj, sk, sl, F);
+            else if (pat[u] == TP_Q4_PR)
+                quartic_term<TP_Q4_PR, R>(Q, sj, sk, sl, F);
+            else if (pat[u] == TP_Q4_PT)
+                quartic_term<TP_Q4_PT, R>(Q, sj, sk, sl, F);
+            else
+                quartic_term<TP_DENSE, R>(Q, sj, sk, sl, F);
+        }
     }
 }
 
@@ -531,8 +566,21 @@ __device__ __forceinline__ int metro_update(const LatView<R> &L, const NB &nb, c
         const R *hp = L.field + b * 3;
         const R h[3] = {ldc(hp), ldc(hp + 1), ldc(hp + 2)};
         R Fo[3], Fn[3];
-        interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, Fo);
-        interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return j == i ? n : X.ld(j); }, Fn);
+        auto ldo = [&](int j) { return X.ld(j); };
+        auto ldn = [&](int j) { return j == i ? n : X.ld(j); };
+        if (GEN && (L.flags & LAT_SELF_Q4)) {
+            // only quartic slots name the site: the bilinear and cubic part
+            // is the same for both spins, the quartic part is added twice
+            interaction_field<R, R, GEN, K2C, 1>(L, nb, b, i, ldo, Fo);
+            Fn[0] = Fo[0];
+            Fn[1] = Fo[1];
+            Fn[2] = Fo[2];
+            interaction_field<R, R, GEN, K2C, 2>(L, nb, b, i, ldo, Fo);
+            interaction_field<R, R, GEN, K2C, 2>(L, nb, b, i, ldn, Fn);
+        } else {
+            interaction_field<R, R, GEN, K2C>(L, nb, b, i, ldo, Fo);
+            interaction_field<R, R, GEN, K2C>(L, nb, b, i, ldn, Fn);
+        }
         R eo = onsite_energy<R, R>(L, b, s.x, s.y, s.z) + s.x * Fo[0] + s.y * Fo[1] + s.z * Fo[2] -
                (s.x * h[0] + s.y * h[1] + s.z * h[2]);
         R en = onsite_energy<R, R>(L, b, n.x, n.y, n.z) + n.x * Fn[0] + n.y * Fn[1] + n.z * Fn[2] -
