@@ -221,7 +221,7 @@ __device__ __forceinline__ size_t align16(size_t v) { return (v + 15) & ~(size_t
 // specialises the bilinear slot count (all lattices of the batch have it).
 // ---------------------------------------------------------------------------
 template <class R, int LM, bool SELF, bool GEN, int MODE, int K2C>
-__global__ __launch_bounds__((GEN || sizeof(R) == 8) ? 512 : 1024) void k_wg(WGArgs a) {
+__global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     double *red = reinterpret_cast<double *>(smem);  // reduction scratch
 
@@ -309,7 +309,7 @@ static hipError_t launch_wg_t(int nsys, size_t lds_bytes, const WGArgs &a, hipSt
         hipError_t e = allow_large_lds(fn);
         if (e != hipSuccess) return e;
     }
-    const int threads = (GENERAL || sizeof(R) == 8) ? 512 : 1024;
+    const int threads = sizeof(R) == 8 ? 512 : 1024;
     if constexpr (MODE == WG_PT_CHAIN) return launch_cooperative(fn, nsys, threads, bytes, a, st);
     hipLaunchKernelGGL((k_wg<R, LM, GENERAL, GENERAL, MODE, K2C>), dim3(nsys), dim3(threads), bytes,
                        st, a);
