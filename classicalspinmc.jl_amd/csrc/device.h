@@ -337,7 +337,7 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
     // are loaded before any of them is used, so the loads of a group are in
     // flight together (a group past the last slot repeats the last slot's
     // loads and skips its product)
-    constexpr int GS = 3;
+    constexpr int GS = 2;
     for (int k0 = 0; PART != 2 && k0 < L.K3; k0 += GS) {
         int pat[GS], j[GS][2];
 #pragma unroll
