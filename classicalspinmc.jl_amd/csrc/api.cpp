@@ -115,6 +115,11 @@ struct PTHost {
     DevBuf<int32_t> split_tab;
     DevBuf<uint64_t> split_hx, split_px;
     DevBuf<uint32_t> split_epoch;
+    // chained kernel in storage mode 0: parts0 workgroups per system (1: one),
+    // their barriers ([R][2]) and workgroup sums ([R][2][parts0][8])
+    int parts0 = 1;
+    DevBuf<uint32_t> bar0;
+    DevBuf<double> psum0;
     // equal-time correlations at every probe (corr = true): the sum over
     // probes per temperature slot, [R][corr_nk][9], and the probe count
     int corr_nk = 0;
@@ -206,6 +211,8 @@ struct csmc_ctx {
     bool dense_tensors = false;  // contract every cubic / quartic coupling in
                                  // full (no sparsity patterns)
     bool verbose = false;        // CSMC_VERBOSE=1: device and lattice notes on stderr
+    int parts0 = 0;              // workgroups per system of the chained PT kernel in
+                                 // storage mode 0: 0 chosen, 1 off, 2..8 forced
     uint32_t spin_limit = 1u << 26;  // polls of a wait for another workgroup
                                      // (x 1/16 for the exchange waits)
     bool timing = false;
@@ -676,6 +683,39 @@ int drain_events(csmc_ctx *c, bool wait) {
     return CSMC_OK;
 }
 
+// Chained PT in storage mode 0 with parts0 workgroups per system: the
+// systems of a batch too large for LDS leave most CUs idle when each has one
+// workgroup.  Parts: CSMC_PARTS0=1..8 forces a count (1: off); by default
+// min(8, CUs / systems, sites per colour class / workgroup size), so every
+// thread keeps a site per phase, when that is at least 2.
+int setup_parts0(csmc_ctx *c, Batch *b) {
+    PTHost &pt = b->pt;
+    pt.parts0 = 1;
+    size_t bytes = 0;
+    const int R = b->nsys;
+    if (!c->pt_chain || R < 2 || choose_lm(c, b, &bytes) != 0) return CSMC_OK;
+    int G = c->parts0;
+    if (G == 0) {
+        int per_colour = 0;  // the largest colour class of the batch's lattices
+        for (int id : b->lat) {
+            const HostLattice &L = *c->lats[id];
+            for (int col = 0; col < L.ncol; ++col) {
+                int n = 0;
+                for (int r = L.colrun[col]; r < L.colrun[col + 1]; ++r) n += L.runs[r][1] - L.runs[r][0];
+                per_colour = std::max(per_colour, n);
+            }
+        }
+        const int threads = b->prec == 32 ? 1024 : 512;
+        G = std::min({8, c->ncu / R, per_colour / threads});
+    }
+    if (G < 2) return CSMC_OK;
+    HIPCHK(pt.bar0.alloc(2 * (size_t)R));
+    HIPCHK(pt.psum0.alloc((size_t)R * 2 * G * 8));
+    HIPCHK(hipMemsetAsync(pt.bar0.p, 0, 2 * (size_t)R * 4, c->stream));
+    pt.parts0 = G;
+    return CSMC_OK;
+}
+
 int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
     size_t lds_bytes = 0;
     int lm = choose_lm(c, b, &lds_bytes);
@@ -719,8 +759,27 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
         }
         HIPCHK(launch_wg_reg(b->k2c, spt, diag, two, b->nsys, lds_bytes, a, c->stream));
     } else {
-        HIPCHK(launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0, b->nsys,
-                         lds_bytes, a, c->stream));
+        PTHost &pt = b->pt;
+        bool parts_done = false;
+        if (lm == 0 && a.mode == WG_PT_CHAIN && pt.parts0 > 1) {
+            // a launch that gave up on a wait may have left arrivals behind
+            HIPCHK(hipMemsetAsync(pt.bar0.p, 0, pt.bar0.n * sizeof(uint32_t), c->stream));
+            WGArgs s = a;
+            s.parts0 = pt.parts0;
+            s.bar0 = pt.bar0.p;
+            s.psum0 = pt.psum0.p;
+            hipError_t e = launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0,
+                                     b->nsys * pt.parts0, lds_bytes, s, c->stream);
+            if (e == hipSuccess) {
+                parts_done = true;
+            } else {
+                (void)hipGetLastError();  // not co-resident: one workgroup per system from now on
+                pt.parts0 = 1;
+            }
+        }
+        if (!parts_done)
+            HIPCHK(launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0, b->nsys,
+                             lds_bytes, a, c->stream));
     }
     if (c->timing) {
         HIPCHK(hipEventRecord(e1, c->stream));
@@ -786,6 +845,8 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
     }
     const char *sf = getenv("CSMC_SSF");
     if (sf && std::string(sf) == "blas") c->ssf_blas = true;
+    const char *p0 = getenv("CSMC_PARTS0");
+    if (p0 && p0[0] >= '1' && p0[0] <= '8' && p0[1] == 0) c->parts0 = p0[0] - '0';
     const char *dt = getenv("CSMC_DENSE_TENSORS");
     if (dt && dt[0] == '1') c->dense_tensors = true;
     const char *vb = getenv("CSMC_VERBOSE");
@@ -1582,6 +1643,8 @@ int32_t csmc_pt_init(csmc_ctx *c, int32_t id, const double *temps, const csmc_pt
     HIPCHK(launch_energy(b->prec, pools(c, b), b->spins.p, R, pt.E.p, nullptr, c->stream));
     rc = setup_split(c, b);
     if (rc) return rc;
+    rc = setup_parts0(c, b);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     pt.init = true;
     return CSMC_OK;
@@ -1591,7 +1654,7 @@ int32_t csmc_pt_split_parts(csmc_ctx *c, int32_t id, int32_t *parts) {
     CHECK_CTX(c);
     GET_BATCH(c, id, b);
     if (!b->pt.init) return fail(CSMC_ESTATE, "csmc_pt_init was not called for this batch");
-    if (parts) *parts = b->pt.split_G;
+    if (parts) *parts = std::max(b->pt.split_G, b->pt.parts0);
     return CSMC_OK;
 }
 

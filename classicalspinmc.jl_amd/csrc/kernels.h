@@ -71,6 +71,13 @@ struct WGArgs {
     uint32_t spin_limit;  // polls of a wait for another workgroup before it gives up
     int64_t probe_base;   // series index of this launch's first probe
     SplitDev split;       // WG_PT_CHAIN with split.G > 1
+    // WG_PT_CHAIN in storage mode 0: parts0 workgroups per system (block b
+    // is part b % parts0 of system b / parts0); bar0 [nsys][2] holds each
+    // system's barrier (arrivals, generation), psum0 [nsys][2][parts0][8]
+    // the parts' workgroup sums
+    int parts0;
+    uint32_t *bar0;
+    double *psum0;
 };
 
 // Arguments of the one-colour-pass kernel (plain / stream form).

@@ -56,6 +56,90 @@ __device__ __forceinline__ void wg_colour_sweep(const LatView<R> &L, OP &&op) {
     }
 }
 
+// Parts of one system (storage mode 0, chained PT): G workgroups share the
+// system, part p taking the sites run start + (p + G k) * blockDim + thread
+// of every run.  A barrier among the G workgroups ends each colour phase,
+// and a sum over the system is the parts' workgroup sums added in part order,
+// so every part holds the same totals.  G = 1 is the whole-system workgroup:
+// a workgroup barrier, and the workgroup sum unchanged.
+struct Parts {
+    int part, G, sys;
+    uint32_t *bar;    // [nsys][2]: arrivals, generation
+    double *psum;     // [nsys][2][G][8]
+    uint32_t *err;    // set when a wait gives up
+    uint32_t limit;   // polls of a wait
+    int buf;          // half of psum the next sum writes
+
+    // every thread; the other parts' global stores before their barrier
+    // call are visible after it
+    __device__ void barrier() {
+        __syncthreads();
+        if (G == 1) return;
+        if (threadIdx.x == 0) {
+            uint32_t *cnt = bar + 2 * sys, *gen = cnt + 1;
+            const uint32_t g0 = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();  // release this part's stores
+            const uint32_t before = atomicAdd(cnt, 1u);
+            if (before == (uint32_t)G - 1) {
+                // the last to arrive opens the next barrier, then this one
+                __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __threadfence();
+                __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                uint32_t n = 0;
+                while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+                    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+                    if (++n > limit) {
+                        __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __threadfence();  // acquire the other parts' stores
+        }
+        __syncthreads();
+    }
+
+    // v: this workgroup's sums (valid in every thread, as block_sum leaves
+    // them); on return the system's sums, in every thread of every part
+    template <int NV>
+    __device__ void sum(double *v) {
+        if (G == 1) return;
+        uint64_t *q = reinterpret_cast<uint64_t *>(psum) + ((int64_t)(sys * 2 + buf) * G) * 8;
+        if (threadIdx.x == 0)
+            for (int k = 0; k < NV; ++k)
+                __hip_atomic_store(q + part * 8 + k, (uint64_t)__double_as_longlong(v[k]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        barrier();
+        for (int k = 0; k < NV; ++k) {
+            double t = 0.0;
+            for (int p = 0; p < G; ++p)
+                t += __longlong_as_double((long long)__hip_atomic_load(q + p * 8 + k, __ATOMIC_RELAXED,
+                                                                       __HIP_MEMORY_SCOPE_AGENT));
+            v[k] = t;
+        }
+        // a part writes this half again two sums later, after a barrier
+        // that every part reaches only once it has read this one
+        buf ^= 1;
+    }
+};
+
+// one colour sweep of a part: op(b, i) for the part's sites of each colour
+// class, then the parts' barrier
+template <class R, class OP>
+__device__ __forceinline__ void wg_colour_sweep_p(const LatView<R> &L, Parts &pp, OP &&op) {
+    const int bd = (int)blockDim.x, first = pp.part * bd + (int)threadIdx.x, step = pp.G * bd;
+    for (int c = 0; c < L.ncol; ++c) {
+        for (int r = L.colrun[c]; r < L.colrun[c + 1]; ++r) {
+            int4 run = L.runs[r];
+            int b = __builtin_amdgcn_readfirstlane(run.z);
+            for (int i = run.x + first; i < run.y; i += step) op(b, i);
+        }
+        pp.barrier();
+    }
+}
+
 // FP64 energy parts (bilinear, cubic, quartic, zeeman, onsite) of a system
 // and its magnetisation, summed over the workgroup in one pass: out[0..4]
 // the parts, out[5..7] the summed spin; valid in every thread on return
@@ -91,11 +175,13 @@ __device__ void wg_energy(const LatView<double> &Ld, const NB &nb, const SV &X, 
 // summed spin of a system; every thread takes the same sites, in the same
 // order, as in wg_energy, so both give the same FP64 sums
 template <class R, class SV>
-__device__ void wg_magnet(const LatView<double> &Ld, const SV &X, double *red, double out[3]) {
+__device__ void wg_magnet(const LatView<double> &Ld, const SV &X, double *red, double out[3],
+                          int part = 0, int G = 1) {
     double m[3] = {0, 0, 0};
+    const int bd = (int)blockDim.x, first = part * bd + (int)threadIdx.x, step = G * bd;
     for (int r = Ld.colrun[0]; r < Ld.colrun[Ld.ncol]; ++r) {
         const int4 run = Ld.runs[r];
-        for (int i = run.x + (int)threadIdx.x; i < run.y; i += (int)blockDim.x) {
+        for (int i = run.x + first; i < run.y; i += step) {
             spin4<R> s = X.ld(i);
             m[0] += s.x;
             m[1] += s.y;
@@ -111,14 +197,15 @@ __device__ void wg_magnet(const LatView<double> &Ld, const SV &X, double *red, d
 // wg_magnet
 template <class R, bool GEN, class NB, class SV>
 __device__ void thread_energy(const LatView<double> &Ld, const NB &nb, const SV &X, double &e,
-                              double m[3]) {
+                              double m[3], int part = 0, int G = 1) {
     double p[5] = {0, 0, 0, 0, 0};
     m[0] = m[1] = m[2] = 0.0;
+    const int bd = (int)blockDim.x, first = part * bd + (int)threadIdx.x, step = G * bd;
     for (int r = Ld.colrun[0]; r < Ld.colrun[Ld.ncol]; ++r) {
         int4 run = Ld.runs[r];
         int b = __builtin_amdgcn_readfirstlane(run.z);
         const double *h = Ld.field + b * 3;
-        for (int i = run.x + (int)threadIdx.x; i < run.y; i += (int)blockDim.x) {
+        for (int i = run.x + first; i < run.y; i += step) {
             spin4<double> s = to_d<R>(X.ld(i));
             double F2[3], F3[3], F4[3];
             field_parts<GEN>(Ld, nb, b, i, [&](int j) { return to_d<R>(X.ld(j)); }, F2, F3, F4);
@@ -138,10 +225,10 @@ __device__ void thread_energy(const LatView<double> &Ld, const NB &nb, const SV 
 // one Metropolis colour sweep; returns this thread's accepted count
 template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
 __device__ int wg_metro_local(const LatView<R> &L, const NB &nb, const SV &X, R T, int kind, R sigma,
-                              uint32_t k0, uint32_t k1, uint32_t stream, uint32_t ctr) {
+                              uint32_t k0, uint32_t k1, uint32_t stream, uint32_t ctr, Parts &pp) {
     int acc = 0;
     const R invT = R(1) / T;
-    wg_colour_sweep<R>(L, [&](int b, int i) {
+    wg_colour_sweep_p<R>(L, pp, [&](int b, int i) {
         acc += metro_update<R, SELF, GEN, K2C>(L, nb, X, b, i, T, invT, kind, sigma, k0, k1, stream,
                                                ctr);
     });
@@ -150,15 +237,18 @@ __device__ int wg_metro_local(const LatView<R> &L, const NB &nb, const SV &X, R 
 
 template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
 __device__ double wg_metro(const LatView<R> &L, const NB &nb, const SV &X, R T, int kind, R sigma,
-                           uint32_t k0, uint32_t k1, uint32_t stream, uint32_t ctr, double *red) {
+                           uint32_t k0, uint32_t k1, uint32_t stream, uint32_t ctr, double *red,
+                           Parts &pp) {
     double v[1] = {(double)wg_metro_local<R, SELF, GEN, K2C>(L, nb, X, T, kind, sigma, k0, k1, stream,
-                                                             ctr)};
+                                                             ctr, pp)};
     block_sum<1>(v, red);
+    pp.sum<1>(v);
     return v[0];
 }
 
 // The sweeper of the global / LDS-array storage modes: sweeps walk the runs
-// of each colour class with one thread per site (wg_colour_sweep).
+// of each colour class with one thread per site (wg_colour_sweep_p), over
+// the sites of this workgroup's part of the system.
 template <class R, bool SELF, bool GEN, int K2C, class SV, class NB>
 struct SweeperG {
     const LatView<R> &L;
@@ -167,27 +257,29 @@ struct SweeperG {
     const NB &nb;
     double *red;
     uint32_t stream, k0, k1;
+    Parts pp;
 
     __device__ int n_sites() const { return L.N; }
-    __device__ bool leader() const { return true; }  // the only workgroup of its system
+    __device__ bool leader() const { return pp.part == 0; }
     __device__ void or_sweep() {
-        wg_colour_sweep<R>(L, [&](int b, int i) { or_update<R, GEN, K2C>(L, nb, X, b, i); });
+        wg_colour_sweep_p<R>(L, pp, [&](int b, int i) { or_update<R, GEN, K2C>(L, nb, X, b, i); });
     }
     __device__ void det_sweep() {
-        wg_colour_sweep<R>(L, [&](int b, int i) { det_update<R, GEN, K2C>(L, nb, X, b, i); });
+        wg_colour_sweep_p<R>(L, pp, [&](int b, int i) { det_update<R, GEN, K2C>(L, nb, X, b, i); });
     }
     __device__ double metro(double T, int kind, double sigma, uint32_t ctr) {
-        return wg_metro<R, SELF, GEN, K2C>(L, nb, X, (R)T, kind, (R)sigma, k0, k1, stream, ctr, red);
+        return wg_metro<R, SELF, GEN, K2C>(L, nb, X, (R)T, kind, (R)sigma, k0, k1, stream, ctr, red, pp);
     }
     __device__ double metro_energy(double T, int kind, double sigma, uint32_t ctr, bool with_m,
                                    double &E, double m[3]) {
         const int acc = wg_metro_local<R, SELF, GEN, K2C>(L, nb, X, (R)T, kind, (R)sigma, k0, k1,
-                                                          stream, ctr);
+                                                          stream, ctr, pp);
         double e, mm[3];
-        thread_energy<R, GEN>(Ld, nb, X, e, mm);
+        thread_energy<R, GEN>(Ld, nb, X, e, mm, pp.part, pp.G);
         if (with_m) {
             double v[5] = {(double)acc, e, mm[0], mm[1], mm[2]};
             block_sum<5>(v, red);
+            pp.sum<5>(v);
             E = v[1];
             m[0] = v[2];
             m[1] = v[3];
@@ -196,18 +288,22 @@ struct SweeperG {
         }
         double v[2] = {(double)acc, e};
         block_sum<2>(v, red);
+        pp.sum<2>(v);
         E = v[1];
         return v[0];
     }
-    __device__ void magnet(double m[3]) { wg_magnet<R>(Ld, X, red, m); }
+    __device__ void magnet(double m[3]) {
+        wg_magnet<R>(Ld, X, red, m, pp.part, pp.G);
+        pp.sum<3>(m);
+    }
 };
 
 template <int MODE, class R, bool SELF, bool GEN, int K2C, class SV, class NB>
 __device__ __forceinline__ void wg_run(const WGArgs &a, int sys, const LatView<R> &L,
                                        const LatView<double> &Ld, const SV &X, const NB &nb,
-                                       double *red) {
+                                       double *red, const Parts &pp) {
     const uint32_t stream = (uint32_t)a.P.sys[sys].stream;
-    SweeperG<R, SELF, GEN, K2C, SV, NB> sw{L, Ld, X, nb, red, stream, a.k0, a.k1};
+    SweeperG<R, SELF, GEN, K2C, SV, NB> sw{L, Ld, X, nb, red, stream, a.k0, a.k1, pp};
     wg_drive<MODE>(a, sys, sw);
 }
 
@@ -225,7 +321,10 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     double *red = reinterpret_cast<double *>(smem);  // reduction scratch
 
-    const int sys = blockIdx.x;
+    // chained PT in storage mode 0 may share a system among parts0 workgroups
+    const int nparts = (LM == 0 && MODE == WG_PT_CHAIN && a.parts0 > 1) ? a.parts0 : 1;
+    const int sys = (int)blockIdx.x / nparts;
+    const Parts pp{(int)blockIdx.x - sys * nparts, nparts, sys, a.bar0, a.psum0, a.pt.err, a.spin_limit, 0};
     const SysDev sd = a.P.sys[sys];
     const LatDev ld = a.P.lats[sd.lat];
     const int4 *runs = a.P.runs + ld.run_off;
@@ -235,7 +334,7 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
     const int N = L.N;
 
     if constexpr (LM == 0) {
-        wg_run<MODE, R, SELF, GEN, K2C>(a, sys, L, Ld, SpinsAoS<R>{G}, NbrG{L.nbr2, N}, red);
+        wg_run<MODE, R, SELF, GEN, K2C>(a, sys, L, Ld, SpinsAoS<R>{G}, NbrG{L.nbr2, N}, red, pp);
     } else {
         // N + 1 entries per component: the last one is the zero pad site
         R *sx = reinterpret_cast<R *>(smem + RED_BYTES);
@@ -243,7 +342,7 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
         wg_stage<8>(N + 1, [&](int i) { return G[i]; }, [&](int i, const spin4<R> &v) { X.st(i, v); });
         if constexpr (LM == 1) {
             __syncthreads();
-            wg_run<MODE, R, SELF, GEN, K2C>(a, sys, L, Ld, X, NbrG{L.nbr2, N}, red);
+            wg_run<MODE, R, SELF, GEN, K2C>(a, sys, L, Ld, X, NbrG{L.nbr2, N}, red, pp);
         } else {
             uint16_t *idx =
                 reinterpret_cast<uint16_t *>(smem + RED_BYTES + align16(3 * sizeof(R) * (N + 1)));
@@ -251,7 +350,7 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
             wg_stage<8>(L.K2 * N, [&](int t) { return nbr2[t]; },
                         [&](int t, int32_t v) { idx[t] = (uint16_t)v; });
             __syncthreads();
-            wg_run<MODE, R, SELF, GEN, K2C>(a, sys, L, Ld, X, NbrL{idx, N, L.K2}, red);
+            wg_run<MODE, R, SELF, GEN, K2C>(a, sys, L, Ld, X, NbrL{idx, N, L.K2}, red, pp);
         }
     }
     // the exchange goes before the LDS modes store the spins back, so that

@@ -226,10 +226,13 @@ int32_t csmc_pt_advance(csmc_ctx *ctx, int32_t batch_id, int64_t until, int64_t 
 int32_t csmc_pt_state(csmc_ctx *ctx, int32_t batch_id, int32_t *system_of_slot, double *E,
                       double *accepted_local, double *accepted_exchange, double *sigma);
 /* Workgroups per system of the chained PT kernel chosen at csmc_pt_init:
- * 1 = one workgroup per system (the default); G > 1 = each system split into
- * G strips with halos, when the environment variable CSMC_SPLIT is "auto" (the
- * planner decides) or 2..8 (forces G).  No counterpart in the reference (its
- * replicas are MPI ranks). */
+ * 1 = one workgroup per system; G > 1 = either each system split into G
+ * strips with halos (lattices held in registers and LDS, when the environment
+ * variable CSMC_SPLIT is "auto" or 2..8), or, for lattices whose spins stay in
+ * device memory, G workgroups sharing each colour class of a system with a
+ * barrier per colour phase (chosen from the CU and system counts, or forced
+ * by CSMC_PARTS0=1..8).  No counterpart in the reference (its replicas are
+ * MPI ranks). */
 int32_t csmc_pt_split_parts(csmc_ctx *ctx, int32_t batch_id, int32_t *parts);
 /* The plain form over the whole batch: one launch per colour pass covering
  * every system (one thread per site), the Philox keys and counters of

@@ -1,0 +1,83 @@
+"""Chained parallel tempering with several workgroups per system when the
+spins stay in device memory (storage mode 0): G workgroups share each
+colour class of a system, a barrier among them ends every colour phase, and
+the system's sums (accepted moves, FP64 energy, summed spin) are the parts'
+workgroup sums added in part order.
+
+The per-site updates and their Philox draws do not depend on the partition,
+so the spins of every replica, the slot map and the exchange counts equal
+those of the one-workgroup run bit for bit; the energies and observables are
+summed in another order and agree to rounding."""
+import os
+
+import numpy as np
+import pytest
+
+import csmc
+from csmc import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(parts):
+    env = {"CSMC_LDS": "0", "CSMC_PARTS0": str(parts)}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return csmc.Context(0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+PARAMS = dict(t_thermalization=20, t_measurement=80, probe_rate=3, swap_rate=5, overrelaxation_rate=2,
+              report_interval=10 ** 9, checkpoint_rate=10 ** 9)
+
+CASES = [
+    # (name, cell, shape, S, precision, replicas, parts, kind)
+    ("pyrochlore_fp32_R6_G4", W.pyrochlore_mixed, (3, 3, 3), 1.0, 32, 6, 4, csmc.METROPOLIS),
+    ("pyrochlore_fp64_R4_G3", W.pyrochlore_mixed, (2, 2, 2), 1.0, 64, 4, 3, csmc.CONE),
+    ("honeycomb_fp32_R8_G2", lambda: W.honeycomb_kitaev(1.0, -2.0), (24, 24), 0.5, 32, 8, 2,
+     csmc.CONE_ADAPTIVE),
+    ("square_fp64_R5_G8", lambda: W.square_heisenberg(-1.0, (0, 0, 0.1)), (40, 40), 1.0, 64, 5, 8,
+     csmc.METROPOLIS),
+]
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
+    return float(np.max(np.abs(a - b)) / max(1.0, float(np.max(np.abs(b)))))
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_parts_equal_one_workgroup_per_system(case):
+    name, mk, shape, S, prec, R, G, kind = case
+    print(f"[parts0] {name}", flush=True)
+    lat = csmc.Lattice(shape, mk(), S, seed=4)
+    temps = W.geometric_temperatures(1.0, 1.5, R)
+    runs = []
+    for parts in (1, G):
+        ctx = _ctx(parts)
+        b = csmc.SpinBatch([lat] * R, precision=prec, ctx=ctx)
+        b.randomize(21)
+        b.pt_init(temps, PARAMS, kind, seed=8)
+        assert b.pt_split_parts() == parts
+        stopped, ev = b.pt_advance(-1)
+        assert ev == 3
+        runs.append(dict(spins=[b.get_spins(r) for r in range(R)], state=b.pt_state(),
+                         obs=b.pt_observables(), E=b.total_energy()))
+        b.close()
+        ctx.close()
+    one, many = runs
+    for r in range(R):
+        assert np.array_equal(one["spins"][r], many["spins"][r]), (name, r)
+    for key in ("system_of_slot", "accepted_exchange"):
+        assert np.array_equal(one["state"][key], many["state"][key]), key
+    assert one["state"]["accepted_exchange"].sum() > 0
+    assert _rel(one["state"]["accepted_local"], many["state"]["accepted_local"]) == 0.0
+    assert np.array_equal(one["E"], many["E"])
+    for key in one["obs"]:
+        assert _rel(many["obs"][key], one["obs"][key]) <= 1e-9, key
