@@ -323,8 +323,16 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
 
     // chained PT in storage mode 0 may share a system among parts0 workgroups
     const int nparts = (LM == 0 && MODE == WG_PT_CHAIN && a.parts0 > 1) ? a.parts0 : 1;
-    const int sys = (int)blockIdx.x / nparts;
-    const Parts pp{(int)blockIdx.x - sys * nparts, nparts, sys, a.bar0, a.psum0, a.pt.err, a.spin_limit, 0};
+    int sys = (int)blockIdx.x / nparts, part = (int)blockIdx.x - sys * nparts;
+    if (nparts > 1 && (gridDim.x / nparts) % 8 == 0) {
+        // blocks go to the 8 XCDs in turn (block b to XCD b % 8): give the
+        // parts of a system blocks of one XCD, so that they share its L2.
+        // A bijection of blocks onto (system, part) whatever the placement.
+        const int x = (int)blockIdx.x % 8, k = (int)blockIdx.x / 8;
+        sys = x + 8 * (k / nparts);
+        part = k % nparts;
+    }
+    const Parts pp{part, nparts, sys, a.bar0, a.psum0, a.pt.err, a.spin_limit, 0};
     const SysDev sd = a.P.sys[sys];
     const LatDev ld = a.P.lats[sd.lat];
     const int4 *runs = a.P.runs + ld.run_off;
