@@ -78,13 +78,18 @@ struct Parts {
         if (threadIdx.x == 0) {
             uint32_t *cnt = bar + 2 * sys, *gen = cnt + 1;
             const uint32_t g0 = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();  // release this part's stores
+            // release this part's stores (and order the load of g0 before
+            // the arrival): a release fence writes L2 back, it does not
+            // invalidate it
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             const uint32_t before = atomicAdd(cnt, 1u);
             if (before == (uint32_t)G - 1) {
-                // the last to arrive opens the next barrier, then this one
+                // the last to arrive read every arrival: acquire what the
+                // others released, then open the next barrier and this one
+                // (the release passes their stores on to the waiters)
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __threadfence();
-                __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             } else {
                 uint32_t n = 0;
                 while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0) {
@@ -95,8 +100,8 @@ struct Parts {
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the other parts' stores
             }
-            __threadfence();  // acquire the other parts' stores
         }
         __syncthreads();
     }
