@@ -268,7 +268,8 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; --gpus N > 1 is not "
-                                   "supported (replica batches are sharded by process, see DESIGN.md)"}))
+                                   "supported: a multi-GPU run would shard the replica batch by process, one "
+                                   "process per GPU, which this build does not do"}))
         sys.exit(2)
 
     import csmc
