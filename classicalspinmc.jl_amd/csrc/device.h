@@ -24,6 +24,18 @@ __device__ __forceinline__ T ldc(const T *p) {
     return *(const __attribute__((address_space(4))) T *)(p);
 }
 
+// f(b) for a site of basis b.  When every active lane of the wave has the
+// same basis, b is passed as a wave-uniform value, so the coupling loads
+// indexed by it are scalar loads; otherwise each lane passes its own.
+template <class F>
+__device__ __forceinline__ void with_wave_basis(int b, F &&f) {
+    const int b0 = __builtin_amdgcn_readfirstlane(b);
+    if (__all(b == b0))
+        f(b0);
+    else
+        f(b);
+}
+
 template <class R>
 struct LatView {
     int N, nb, K2, K2P, K3, K4, ncol, has_self, flags;

@@ -490,17 +490,18 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a) {
     if (i < end) {
         const SpinsAoS<R> X{reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off};
         const NbrG nb{L.nbr2, L.N};
-        const int b = L.sub[i];
-        if (a.kind == CSMC_OVERRELAX) {
-            or_update<R, GEN, 0>(L, nb, X, b, i);
-        } else if (a.kind == CSMC_DETERMINISTIC) {
-            det_update<R, GEN, 0>(L, nb, X, b, i);
-        } else {
-            R sigma = a.sigma_sys ? (R)a.sigma_sys[sys] : R(60);
-            R T = (R)a.T_sys[sys];
-            acc = metro_update<R, SELF, GEN, 0>(L, nb, X, b, i, T, R(1) / T, a.kind, sigma, a.k0,
-                                                a.k1, (uint32_t)sd.stream, a.ctr ? a.ctr[sys] : a.ctr0);
-        }
+        with_wave_basis(L.sub[i], [&](int b) __attribute__((always_inline)) {
+            if (a.kind == CSMC_OVERRELAX) {
+                or_update<R, GEN, 0>(L, nb, X, b, i);
+            } else if (a.kind == CSMC_DETERMINISTIC) {
+                det_update<R, GEN, 0>(L, nb, X, b, i);
+            } else {
+                R sigma = a.sigma_sys ? (R)a.sigma_sys[sys] : R(60);
+                R T = (R)a.T_sys[sys];
+                acc = metro_update<R, SELF, GEN, 0>(L, nb, X, b, i, T, R(1) / T, a.kind, sigma, a.k0, a.k1,
+                                                    (uint32_t)sd.stream, a.ctr ? a.ctr[sys] : a.ctr0);
+            }
+        });
     }
     if (a.kind != CSMC_OVERRELAX && a.kind != CSMC_DETERMINISTIC) {
         unsigned long long m = __ballot(acc);
@@ -743,10 +744,11 @@ __device__ __forceinline__ void site_gradient(const DevPools &P, const LatDev &l
     if (ld.flags & LAT_GRAD_LF) {
         // the gradient is the local field (the host checked the pairing of
         // every bond with its transpose): one gather over x's own bonds
-        const int b = L.sub[x];
-        double F[3];
-        interaction_field<double, double>(L, NbrG{L.nbr2, L.N}, b, x, sp, F);
-        local_field<double, double>(L, b, sp(x), F, g);
+        with_wave_basis(L.sub[x], [&](int b) __attribute__((always_inline)) {
+            double F[3];
+            interaction_field<double, double>(L, NbrG{L.nbr2, L.N}, b, x, sp, F);
+            local_field<double, double>(L, b, sp(x), F, g);
+        });
         return;
     }
     const int32_t *ioff = P.i32 + ld.inc_off;
