@@ -395,6 +395,17 @@ void append_lattice(csmc_ctx *c, const HostLattice &L) {
     for (int k = 0; k < L.K4; ++k)
         for (int q = 0; q < 3; ++q)
             for (int i = 0; i < N; ++i) I.push_back(devp(L.nbr4[((size_t)L.ref_of_dev[i] * L.K4 + k) * 3 + q]));
+    // lattices with the same neighbour tables (the same cell structure and
+    // shape, whatever the couplings) share one copy: a batch of them then
+    // gathers through one table that stays in cache
+    for (const LatDev &o : c->h_lat) {
+        if (o.N != d.N || o.K2P != d.K2P || o.K3 != d.K3 || o.K4 != d.K4) continue;
+        if (std::equal(I.begin() + d.nbr_off, I.end(), I.begin() + o.nbr_off)) {
+            I.resize((size_t)d.nbr_off);
+            d.nbr_off = o.nbr_off;
+            break;
+        }
+    }
     d.ref_off = (int64_t)I.size();
     for (int i = 0; i < N; ++i) I.push_back(L.ref_of_dev[i]);
     d.sub_off = (int64_t)I.size();

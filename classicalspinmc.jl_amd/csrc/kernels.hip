@@ -815,12 +815,31 @@ __device__ __forceinline__ void site_gradient(const DevPools &P, const LatDev &l
     }
 }
 
+// Block of a one-thread-per-site kernel over a batch (a 1-D grid of bps
+// blocks per system): its system and its block within the system.  Blocks go
+// to the 8 XCDs in turn (block b to XCD b % 8); when the system count is a
+// multiple of 8, all blocks of a system go to one XCD, so the spins its
+// sites gather stay in that XCD's L2.  A bijection either way.
+__device__ __forceinline__ void batch_block(int nsys, int bps, int &sys, int &blk) {
+    const int b = (int)blockIdx.x;
+    if (nsys % 8 == 0) {
+        const int k = b / 8;
+        sys = b % 8 + 8 * (k / bps);
+        blk = k % bps;
+    } else {
+        sys = b / bps;
+        blk = b % bps;
+    }
+}
+
 template <class R>
-__global__ __launch_bounds__(256) void k_grad_spins(DevPools P, const void *spins, double *dEdS) {
-    const int sys = blockIdx.y;
+__global__ __launch_bounds__(256) void k_grad_spins(DevPools P, const void *spins, double *dEdS, int nsys,
+                                                    int bps) {
+    int sys, blk;
+    batch_block(nsys, bps, sys, blk);
     const SysDev sd = P.sys[sys];
     const LatDev ld = P.lats[sd.lat];
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int x = blk * blockDim.x + threadIdx.x;
     if (x >= ld.N) return;
     const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
     const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
@@ -834,9 +853,10 @@ __global__ __launch_bounds__(256) void k_grad_spins(DevPools P, const void *spin
 
 hipError_t launch_grad_spins(int prec, const DevPools &P, const void *spins, int nsys, int maxN,
                              double *dEdS, hipStream_t st) {
-    dim3 grid((maxN + 255) / 256, nsys);
-    if (prec == 32) hipLaunchKernelGGL((k_grad_spins<float>), grid, dim3(256), 0, st, P, spins, dEdS);
-    else hipLaunchKernelGGL((k_grad_spins<double>), grid, dim3(256), 0, st, P, spins, dEdS);
+    const int bps = (maxN + 255) / 256;
+    dim3 grid((unsigned)((int64_t)bps * nsys));
+    if (prec == 32) hipLaunchKernelGGL((k_grad_spins<float>), grid, dim3(256), 0, st, P, spins, dEdS, nsys, bps);
+    else hipLaunchKernelGGL((k_grad_spins<double>), grid, dim3(256), 0, st, P, spins, dEdS, nsys, bps);
     return hipGetLastError();
 }
 
@@ -1000,11 +1020,12 @@ __global__ __launch_bounds__(256) void k_gd(DevPools P, void *spins, const doubl
 // pass (the result equals k_grad_spins followed by k_gd bit for bit)
 template <class R>
 __global__ __launch_bounds__(256) void k_gd_step(DevPools P, const void *spins_old, void *spins_new,
-                                                 double eta) {
-    const int sys = blockIdx.y;
+                                                 double eta, int nsys, int bps) {
+    int sys, blk;
+    batch_block(nsys, bps, sys, blk);
     const SysDev sd = P.sys[sys];
     const LatDev ld = P.lats[sd.lat];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= ld.N) return;
     const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
     const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins_old) + sd.spin_off;
@@ -1026,11 +1047,12 @@ __global__ __launch_bounds__(256) void k_gd_step(DevPools P, const void *spins_o
 
 hipError_t launch_gd_step(int prec, const DevPools &P, const void *spins_old, void *spins_new, int nsys, int maxN,
                           double eta, hipStream_t st) {
-    dim3 grid((maxN + 255) / 256, nsys);
+    const int bps = (maxN + 255) / 256;
+    dim3 grid((unsigned)((int64_t)bps * nsys));
     if (prec == 32)
-        hipLaunchKernelGGL((k_gd_step<float>), grid, dim3(256), 0, st, P, spins_old, spins_new, eta);
+        hipLaunchKernelGGL((k_gd_step<float>), grid, dim3(256), 0, st, P, spins_old, spins_new, eta, nsys, bps);
     else
-        hipLaunchKernelGGL((k_gd_step<double>), grid, dim3(256), 0, st, P, spins_old, spins_new, eta);
+        hipLaunchKernelGGL((k_gd_step<double>), grid, dim3(256), 0, st, P, spins_old, spins_new, eta, nsys, bps);
     return hipGetLastError();
 }
 
