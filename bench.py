@@ -613,6 +613,11 @@ def bench_adjoint(args, csmc, W):
     value = steps * n * N / dt
     avg_ms = kern_ms / launches
     achieved = 48.0 * n * N / (avg_ms * 1e-3) / 1e9
+    traffic = None  # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)
+    tpath = os.path.join(ROOT, "profiles", "k_gd_step_adjoint_tri_L96_B128_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes_per_launch"]
     t0 = time.perf_counter()
     for _ in range(5):
         b.grad_couplings()
@@ -624,7 +629,7 @@ def bench_adjoint(args, csmc, W):
                                   f"L={L} ({N} sites), J1=1, J2~U(0,0.2), FP64, dE/dS on the device each step",
                       "systems": n, "sites": N},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                         "kernel": "k_gd_step<double> (dE/dS and the projected update of every site)",
                         "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": 48 * n * N},
            "dEdJ_ms": round(dj_ms, 3)}
