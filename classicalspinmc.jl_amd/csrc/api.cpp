@@ -694,17 +694,18 @@ int drain_events(csmc_ctx *c, bool wait) {
     return CSMC_OK;
 }
 
-// Chained PT in storage mode 0 with parts0 workgroups per system: the
-// systems of a batch too large for LDS leave most CUs idle when each has one
-// workgroup.  Parts: CSMC_PARTS0=1..8 forces a count (1: off); by default
-// min(8, CUs / systems, sites per colour class / workgroup size), so every
-// thread keeps a site per phase, when that is at least 2.
+// PT in storage mode 0 with parts0 workgroups per system (chained and
+// per-segment launches): the systems of a batch too large for LDS leave most
+// CUs idle when each has one workgroup.  CSMC_PARTS0=1..8 forces the count
+// (1: off); by default it is min(8, CUs / systems, sites per colour class /
+// workgroup size), so every thread keeps a site per phase, when that is at
+// least 2.
 int setup_parts0(csmc_ctx *c, Batch *b) {
     PTHost &pt = b->pt;
     pt.parts0 = 1;
     size_t bytes = 0;
     const int R = b->nsys;
-    if (!c->pt_chain || R < 2 || choose_lm(c, b, &bytes) != 0) return CSMC_OK;
+    if (choose_lm(c, b, &bytes) != 0) return CSMC_OK;
     int G = c->parts0;
     if (G == 0) {
         int per_colour = 0;  // the largest colour class of the batch's lattices
@@ -772,7 +773,7 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
     } else {
         PTHost &pt = b->pt;
         bool parts_done = false;
-        if (lm == 0 && a.mode == WG_PT_CHAIN && pt.parts0 > 1) {
+        if (lm == 0 && (a.mode == WG_PT_CHAIN || a.mode == WG_PT) && pt.parts0 > 1) {
             // a launch that gave up on a wait may have left arrivals behind
             HIPCHK(hipMemsetAsync(pt.bar0.p, 0, pt.bar0.n * sizeof(uint32_t), c->stream));
             WGArgs s = a;

@@ -326,8 +326,8 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     double *red = reinterpret_cast<double *>(smem);  // reduction scratch
 
-    // chained PT in storage mode 0 may share a system among parts0 workgroups
-    const int nparts = (LM == 0 && MODE == WG_PT_CHAIN && a.parts0 > 1) ? a.parts0 : 1;
+    // PT in storage mode 0 may share a system among parts0 workgroups
+    const int nparts = (LM == 0 && (MODE == WG_PT_CHAIN || MODE == WG_PT) && a.parts0 > 1) ? a.parts0 : 1;
     int sys = (int)blockIdx.x / nparts, part = (int)blockIdx.x - sys * nparts;
     if (nparts > 1 && (gridDim.x / nparts) % 8 == 0) {
         // blocks go to the 8 XCDs in turn (block b to XCD b % 8): give the
@@ -422,7 +422,10 @@ static hipError_t launch_wg_t(int nsys, size_t lds_bytes, const WGArgs &a, hipSt
         if (e != hipSuccess) return e;
     }
     const int threads = sizeof(R) == 8 ? 512 : 1024;
+    // the waits of a chain and the barriers of parts need every workgroup resident
     if constexpr (MODE == WG_PT_CHAIN) return launch_cooperative(fn, nsys, threads, bytes, a, st);
+    if constexpr (MODE == WG_PT && LM == 0)
+        if (a.parts0 > 1) return launch_cooperative(fn, nsys, threads, bytes, a, st);
     hipLaunchKernelGGL((k_wg<R, LM, GENERAL, GENERAL, MODE, K2C>), dim3(nsys), dim3(threads), bytes,
                        st, a);
     return hipGetLastError();

@@ -19,8 +19,8 @@ from csmc import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def _ctx(parts):
-    env = {"CSMC_LDS": "0", "CSMC_PARTS0": str(parts)}
+def _ctx(parts, extra=None):
+    env = {"CSMC_LDS": "0", "CSMC_PARTS0": str(parts), **(extra or {})}
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     try:
@@ -36,14 +36,23 @@ def _ctx(parts):
 PARAMS = dict(t_thermalization=20, t_measurement=80, probe_rate=3, swap_rate=5, overrelaxation_rate=2,
               report_interval=10 ** 9, checkpoint_rate=10 ** 9)
 
+# checkpoints every 7 sweeps end segments at host events: per-segment
+# launches (with the exchange done by the last workgroup) between the chains
+CHECKPOINTS = dict(PARAMS, checkpoint_rate=7)
+NO_CHAIN = dict(PARAMS, checkpoint_rate=7)
+
 CASES = [
-    # (name, cell, shape, S, precision, replicas, parts, kind)
-    ("pyrochlore_fp32_R6_G4", W.pyrochlore_mixed, (3, 3, 3), 1.0, 32, 6, 4, csmc.METROPOLIS),
-    ("pyrochlore_fp64_R4_G3", W.pyrochlore_mixed, (2, 2, 2), 1.0, 64, 4, 3, csmc.CONE),
+    # (name, cell, shape, S, precision, replicas, parts, kind, parameters, environment)
+    ("pyrochlore_fp32_R6_G4", W.pyrochlore_mixed, (3, 3, 3), 1.0, 32, 6, 4, csmc.METROPOLIS, PARAMS, {}),
+    ("pyrochlore_fp64_R4_G3", W.pyrochlore_mixed, (2, 2, 2), 1.0, 64, 4, 3, csmc.CONE, PARAMS, {}),
     ("honeycomb_fp32_R8_G2", lambda: W.honeycomb_kitaev(1.0, -2.0), (24, 24), 0.5, 32, 8, 2,
-     csmc.CONE_ADAPTIVE),
+     csmc.CONE_ADAPTIVE, PARAMS, {}),
     ("square_fp64_R5_G8", lambda: W.square_heisenberg(-1.0, (0, 0, 0.1)), (40, 40), 1.0, 64, 5, 8,
-     csmc.METROPOLIS),
+     csmc.METROPOLIS, PARAMS, {}),
+    ("pyrochlore_fp32_R4_G4_checkpoints", W.pyrochlore_mixed, (3, 3, 3), 1.0, 32, 4, 4, csmc.METROPOLIS,
+     CHECKPOINTS, {}),
+    ("honeycomb_fp32_R3_G3_no_chain", lambda: W.honeycomb_kitaev(1.0, -2.0), (24, 24), 0.5, 32, 3, 3,
+     csmc.METROPOLIS, NO_CHAIN, {"CSMC_PT_CHAIN": "0"}),
 ]
 
 
@@ -54,19 +63,24 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_parts_equal_one_workgroup_per_system(case):
-    name, mk, shape, S, prec, R, G, kind = case
+    name, mk, shape, S, prec, R, G, kind, params, extra = case
     print(f"[parts0] {name}", flush=True)
     lat = csmc.Lattice(shape, mk(), S, seed=4)
     temps = W.geometric_temperatures(1.0, 1.5, R)
     runs = []
     for parts in (1, G):
-        ctx = _ctx(parts)
+        ctx = _ctx(parts, extra)
         b = csmc.SpinBatch([lat] * R, precision=prec, ctx=ctx)
         b.randomize(21)
-        b.pt_init(temps, PARAMS, kind, seed=8)
+        b.pt_init(temps, params, kind, seed=8)
         assert b.pt_split_parts() == parts
-        stopped, ev = b.pt_advance(-1)
-        assert ev == 3
+        events = []
+        while True:
+            stopped, ev = b.pt_advance(-1)
+            events.append(ev)
+            if ev == 3:
+                break
+        assert (1 in events) == (params is not PARAMS)
         runs.append(dict(spins=[b.get_spins(r) for r in range(R)], state=b.pt_state(),
                          obs=b.pt_observables(), E=b.total_energy()))
         b.close()
