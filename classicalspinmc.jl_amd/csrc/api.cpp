@@ -118,7 +118,9 @@ struct PTHost {
     // chained kernel in storage mode 0: parts0 workgroups per system (1: one),
     // their barriers ([R][2]) and workgroup sums ([R][2][parts0][8])
     int parts0 = 1;
-    DevBuf<uint32_t> bar0;
+    bool parts_ready = false;  // parts0 chosen (at csmc_pt_init or the first
+                               // persistent launch in storage mode 0)
+    DevBuf<uint32_t> bar0, err0;
     DevBuf<double> psum0;
     // equal-time correlations at every probe (corr = true): the sum over
     // probes per temperature slot, [R][corr_nk][9], and the probe count
@@ -694,8 +696,8 @@ int drain_events(csmc_ctx *c, bool wait) {
     return CSMC_OK;
 }
 
-// PT in storage mode 0 with parts0 workgroups per system (chained and
-// per-segment launches): the systems of a batch too large for LDS leave most
+// Persistent launches in storage mode 0 with parts0 workgroups per system
+// (sweeps, annealing, PT): the systems of a batch too large for LDS leave most
 // CUs idle when each has one workgroup.  CSMC_PARTS0=1..8 forces the count
 // (1: off); by default it is min(8, CUs / systems, sites per colour class /
 // workgroup size), so every thread keeps a site per phase, when that is at
@@ -703,9 +705,10 @@ int drain_events(csmc_ctx *c, bool wait) {
 int setup_parts0(csmc_ctx *c, Batch *b) {
     PTHost &pt = b->pt;
     pt.parts0 = 1;
-    size_t bytes = 0;
+    pt.parts_ready = true;
     const int R = b->nsys;
-    if (choose_lm(c, b, &bytes) != 0) return CSMC_OK;
+    size_t bytes = 0;
+    if (choose_lm(c, b, &bytes) != 0) return CSMC_OK;  // spins in LDS: no parts
     int G = c->parts0;
     if (G == 0) {
         int per_colour = 0;  // the largest colour class of the batch's lattices
@@ -722,9 +725,26 @@ int setup_parts0(csmc_ctx *c, Batch *b) {
     }
     if (G < 2) return CSMC_OK;
     HIPCHK(pt.bar0.alloc(2 * (size_t)R));
+    HIPCHK(pt.err0.alloc(1));
     HIPCHK(pt.psum0.alloc((size_t)R * 2 * G * 8));
     HIPCHK(hipMemsetAsync(pt.bar0.p, 0, 2 * (size_t)R * 4, c->stream));
+    HIPCHK(hipMemsetAsync(pt.err0.p, 0, 4, c->stream));
     pt.parts0 = G;
+    return CSMC_OK;
+}
+
+// after a synchronised sweep or annealing call: whether a barrier wait of
+// the parts gave up (the call's results are then not valid)
+int parts_check(Batch *b) {
+    PTHost &pt = b->pt;
+    if (pt.parts0 < 2 || !pt.err0.p) return CSMC_OK;
+    uint32_t err = 0;
+    HIPCHK(hipMemcpy(&err, pt.err0.p, 4, hipMemcpyDeviceToHost));
+    if (err) {
+        const uint32_t zero = 0;
+        HIPCHK(hipMemcpy(pt.err0.p, &zero, 4, hipMemcpyHostToDevice));
+        return fail(CSMC_EHIP, "a wait for another workgroup of the same system timed out");
+    }
     return CSMC_OK;
 }
 
@@ -773,13 +793,21 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
     } else {
         PTHost &pt = b->pt;
         bool parts_done = false;
-        if (lm == 0 && (a.mode == WG_PT_CHAIN || a.mode == WG_PT) && pt.parts0 > 1) {
+        if (lm == 0 && !pt.parts_ready) {
+            const int rc = setup_parts0(c, b);
+            if (rc) return rc;
+        }
+        if (lm == 0 && pt.parts0 > 1) {
             // a launch that gave up on a wait may have left arrivals behind
             HIPCHK(hipMemsetAsync(pt.bar0.p, 0, pt.bar0.n * sizeof(uint32_t), c->stream));
             WGArgs s = a;
             s.parts0 = pt.parts0;
             s.bar0 = pt.bar0.p;
             s.psum0 = pt.psum0.p;
+            // PT waits report through the PT state's flag, checked after
+            // every csmc_pt_advance; sweeps and annealing through err0
+            s.err0 = (a.mode == WG_PT || a.mode == WG_PT_CHAIN) ? pt.err.p : pt.err0.p;
+            s.spin_limit = c->spin_limit;  // only the PT paths set it
             hipError_t e = launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0,
                                      b->nsys * pt.parts0, lds_bytes, s, c->stream);
             if (e == hipSuccess) {
@@ -1201,7 +1229,7 @@ int32_t csmc_sweep(csmc_ctx *c, int32_t id, int32_t kind, const double *T, doubl
     if (accepted) HIPCHK(hipMemcpyAsync(accepted, b->acc.p, b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
     if (sigma) HIPCHK(hipMemcpyAsync(sigma, b->sigma.p, b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    return CSMC_OK;
+    return parts_check(b);
 }
 
 int32_t csmc_sweep_plain(csmc_ctx *c, int32_t id, int32_t kind, const double *T, double *sigma,
@@ -1591,7 +1619,7 @@ int32_t csmc_anneal(csmc_ctx *c, int32_t id, const double *temps, int32_t n_temp
     if (accepted)
         HIPCHK(hipMemcpyAsync(accepted, dacc.p, (size_t)n_temps * b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    return CSMC_OK;
+    return parts_check(b);
 }
 
 int32_t csmc_pt_init(csmc_ctx *c, int32_t id, const double *temps, const csmc_pt_params *p,
@@ -1895,7 +1923,7 @@ int32_t csmc_pt_advance(csmc_ctx *c, int32_t id, int64_t until, int64_t *stopped
     rc = flush(-1);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (chained) {
+    if (chained || pt.parts0 > 1) {
         uint32_t err = 0;
         HIPCHK(hipMemcpy(&err, pt.err.p, 4, hipMemcpyDeviceToHost));
         if (err) return fail(CSMC_EHIP, "parallel tempering: a wait for another workgroup timed out");

@@ -78,6 +78,7 @@ struct WGArgs {
     int parts0;
     uint32_t *bar0;
     double *psum0;
+    uint32_t *err0;  // set when a barrier wait of the parts gives up
 };
 
 // Arguments of the one-colour-pass kernel (plain / stream form).

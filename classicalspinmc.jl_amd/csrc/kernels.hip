@@ -326,8 +326,8 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     double *red = reinterpret_cast<double *>(smem);  // reduction scratch
 
-    // PT in storage mode 0 may share a system among parts0 workgroups
-    const int nparts = (LM == 0 && (MODE == WG_PT_CHAIN || MODE == WG_PT) && a.parts0 > 1) ? a.parts0 : 1;
+    // in storage mode 0 a system may be shared among parts0 workgroups
+    const int nparts = (LM == 0 && a.parts0 > 1) ? a.parts0 : 1;
     int sys = (int)blockIdx.x / nparts, part = (int)blockIdx.x - sys * nparts;
     if (nparts > 1 && (gridDim.x / nparts) % 8 == 0) {
         // blocks go to the 8 XCDs in turn (block b to XCD b % 8): give the
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
         sys = x + 8 * (k / nparts);
         part = k % nparts;
     }
-    const Parts pp{part, nparts, sys, a.bar0, a.psum0, a.pt.err, a.spin_limit, 0};
+    const Parts pp{part, nparts, sys, a.bar0, a.psum0, a.err0, a.spin_limit, 0};
     const SysDev sd = a.P.sys[sys];
     const LatDev ld = a.P.lats[sd.lat];
     const int4 *runs = a.P.runs + ld.run_off;
@@ -424,7 +424,7 @@ static hipError_t launch_wg_t(int nsys, size_t lds_bytes, const WGArgs &a, hipSt
     const int threads = sizeof(R) == 8 ? 512 : 1024;
     // the waits of a chain and the barriers of parts need every workgroup resident
     if constexpr (MODE == WG_PT_CHAIN) return launch_cooperative(fn, nsys, threads, bytes, a, st);
-    if constexpr (MODE == WG_PT && LM == 0)
+    if constexpr (LM == 0)
         if (a.parts0 > 1) return launch_cooperative(fn, nsys, threads, bytes, a, st);
     hipLaunchKernelGGL((k_wg<R, LM, GENERAL, GENERAL, MODE, K2C>), dim3(nsys), dim3(threads), bytes,
                        st, a);

@@ -211,7 +211,7 @@ __device__ __forceinline__ void wg_drive(const WGArgs &a, int sys, SW &sw) {
             else if (a.kind == CSMC_DETERMINISTIC) sw.det_sweep();
             else acc += metro_sweep(T, a.kind, sigma);
         }
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0 && sw.leader()) {
             if (a.acc_sys) a.acc_sys[sys] = acc;
             if (a.sigma_sys) a.sigma_sys[sys] = sigma;
         }
@@ -233,7 +233,7 @@ __device__ __forceinline__ void wg_drive(const WGArgs &a, int sys, SW &sw) {
                     acc += metro_sweep(T, a.kind, sigma);
                 }
             }
-            if (threadIdx.x == 0) a.acc_T[(int64_t)sys * a.n_temps + k] = acc;
+            if (threadIdx.x == 0 && sw.leader()) a.acc_T[(int64_t)sys * a.n_temps + k] = acc;
         }
     } else {
         // parallel_tempering! sweep loop (monte_carlo.jl:295-388).  State of

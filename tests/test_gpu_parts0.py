@@ -95,3 +95,43 @@ def test_parts_equal_one_workgroup_per_system(case):
     assert np.array_equal(one["E"], many["E"])
     for key in one["obs"]:
         assert _rel(many["obs"][key], one["obs"][key]) <= 1e-9, key
+
+
+SWEEP_CASES = [
+    # (name, cells and shapes, precision, parts)
+    ("pyrochlore_honeycomb_fp32_G4", [(W.pyrochlore_mixed, (3, 3, 3), 1.0),
+                                      (lambda: W.honeycomb_kitaev(1.0, -2.0), (20, 20), 0.5)], 32, 4),
+    ("square_fp64_G3", [(lambda: W.square_heisenberg(-1.0, (0, 0, 0.1)), (36, 36), 1.0)], 64, 3),
+]
+
+
+@pytest.mark.parametrize("case", SWEEP_CASES, ids=[c[0] for c in SWEEP_CASES])
+def test_parts_sweeps_and_annealing_equal_one_workgroup(case):
+    """Sweeps of every kind and a whole annealing schedule in one launch, with
+    the same parts: spins and accepted counts equal to one workgroup per
+    system, bit for bit"""
+    name, cells, prec, G = case
+    print(f"[parts0] sweeps {name}", flush=True)
+    lats = [csmc.Lattice(shape, mk(), S, seed=6 + k) for k, (mk, shape, S) in enumerate(cells)]
+    T = np.linspace(0.4, 1.1, len(lats))
+    runs = []
+    for parts in (1, G):
+        ctx = _ctx(parts)
+        b = csmc.SpinBatch(lats, precision=prec, ctx=ctx)
+        b.randomize(13)
+        accs = []
+        for kind in (csmc.OVERRELAX, csmc.METROPOLIS, csmc.CONE, csmc.DETERMINISTIC):
+            accs.append(b.sweep(kind, T=T, sigma=0.7, n_sweeps=3, seed=19))
+        sg = np.full(len(lats), 0.9)
+        accs.append(b.sweep(csmc.CONE_ADAPTIVE, T=T, sigma=sg, n_sweeps=4, seed=19))
+        accs.append(sg.copy())
+        accs.append(b.anneal(W.geometric_temperatures(0.2, 1.0, 4)[::-1], 6, 2, csmc.METROPOLIS, seed=3))
+        runs.append(([b.get_spins(r) for r in range(len(lats))], accs))
+        b.close()
+        ctx.close()
+    (s1, a1), (sg_, ag) = runs
+    for r in range(len(lats)):
+        assert np.array_equal(s1[r], sg_[r]), (name, r)
+    for x, y in zip(a1, ag):
+        assert np.array_equal(x, y)
+    assert a1[1].sum() > 0 and a1[-1].sum() > 0
