@@ -318,6 +318,15 @@ pt_load!(b::Batch, state::Vector{UInt8}) =
 
 # ---- launch forms (no reference counterpart) ----------------------------------------
 
+"""Workgroups per system of the PT kernel chosen at csmc_pt_init (1: one; more:
+the split kernel's strips, or the parts of a system whose spins stay in device
+memory)."""
+function pt_split_parts(b::Batch)
+    g = Ref{Int32}(0)
+    check(ccall((:csmc_pt_split_parts, libcsmc), Int32, (Ptr{Cvoid}, Int32, Ref{Int32}), b.ctx.h, b.id, g))
+    return Int(g[])
+end
+
 """Time each sweep form on the batch's own state and keep the fastest
 (csmc_autotune); returns the per-form times in ms (NaN: skipped) and the
 chosen form's index."""
