@@ -24,11 +24,22 @@
  *   orc_site_energy /  (FM ground state -2S^2|J|-hS, chirality of an
  *   orc_total_energy   orthonormal triad = +-1, biquadratic B(s.s')^2);
  *                      central finite differences of orc_total_energy.
+ *                      Cubic/quartic role permutations (lattice.jl:209-286):
+ *                      on a pyrochlore cell with generic non-symmetric
+ *                      tensors on distinct-basis faces and tetrahedra, the
+ *                      total energy equals an independent term-by-term sum
+ *                      and the local field equals its central finite
+ *                      difference; the first-matching-role rule for terms
+ *                      that repeat a basis is checked against a hand-built
+ *                      field and the 2/3 (cubic) and 1/2 (quartic) energy
+ *                      weights it implies.
  *   orc_overrelax_*    involution, |s| and s.H preserved, E conserved
  *                      (bilinear + Zeeman).
  *   orc_philox4x32_10  Random123 known-answer vectors.
  *   orc_binner_*       direct blocking analysis in numpy.
- *   orc_metropolis_*   single spin in a field: <s.h> = S L(beta h S) (Langevin).
+ *   orc_metropolis_*   single spin in a field: <s.h> = S L(beta h S) (Langevin),
+ *                      with the uniform proposal and with the cone proposal
+ *                      (metropolis.jl:84-87) at two cone widths.
  *   orc_pt_exchange_*  the acceptance formula monte_carlo.jl:326-331 checked
  *                      against detailed balance on a two-level toy.
  *   orc_equal_time_    closed forms: a ferromagnet along z gives S^zz(0) = N S^2

@@ -300,3 +300,192 @@ def test_exchange_two_level_detailed_balance():
         acc += orc.pt_exchange(temps, E, perm, 4242, xc)
     p = min(1.0, math.exp((1 / 1.0 - 1 / 0.5) * (0.0 - -1.0)))
     assert abs(acc / n - p) < 4 * math.sqrt(p * (1 - p) / n)
+
+
+# -- cubic / quartic role permutations (lattice.jl:209-286) ---------------------
+# The reference tests hold no cubic or quartic case, so these pin the oracle's
+# role branches against what the mathematics fixes: a term-by-term energy sum
+# written independently of the per-site tables, and the gradient of that
+# energy.
+
+def _site_index(shape, b, x):
+    """0-based site index of basis b (1-based) in cell x (0-based), the
+    lexicographic (basis, x1, .., xD) order of lattice.jl:29-33"""
+    idx = b - 1
+    for d, L in enumerate(shape):
+        idx = idx * L + (x[d] % L)
+    return idx
+
+
+def _term_energy(uc, shape, s):
+    """Physical energy of the cubic and quartic terms of uc on a periodic
+    lattice: every term instance counted once, sum over cells x of
+    C[a,b,c] s_b1(x)_a s_b2(x+o2)_b s_b3(x+o3)_c (and the quartic alike)."""
+    import itertools
+    E = 0.0
+    for x in itertools.product(*[range(L) for L in shape]):
+        x = np.array(x)
+        for b1, b2, b3, C, o2, o3 in uc.cubic:
+            i, j, k = (_site_index(shape, b1, x), _site_index(shape, b2, x + np.array(o2)),
+                       _site_index(shape, b3, x + np.array(o3)))
+            E += np.einsum("abc,a,b,c->", C, s[i], s[j], s[k])
+        for b1, b2, b3, b4, R, o2, o3, o4 in uc.quartic:
+            i, j = _site_index(shape, b1, x), _site_index(shape, b2, x + np.array(o2))
+            k, m = _site_index(shape, b3, x + np.array(o3)), _site_index(shape, b4, x + np.array(o4))
+            E += np.einsum("abcd,a,b,c,d->", R, s[i], s[j], s[k], s[m])
+    return E
+
+
+def _generic_pyrochlore_cell(seed=21):
+    """pyrochlore with non-symmetric cubic tensors on faces of three distinct
+    bases (each basis in every role position somewhere) and generic 4-site
+    quartic tensors, with offsets that are not all zero, plus a field on every
+    basis (in basis order, the safe case of lattice.jl:128-140)"""
+    rng = np.random.default_rng(seed)
+    uc = Pyrochlore()
+    addCubic(uc, 1, 2, 3, rng.normal(size=(3, 3, 3)), (0, 0, 0), (0, 0, 0))
+    addCubic(uc, 4, 1, 2, rng.normal(size=(3, 3, 3)), (1, 0, 0), (0, 1, -1))
+    addCubic(uc, 3, 4, 1, rng.normal(size=(3, 3, 3)), (-1, 1, 0), (0, 0, 1))
+    addQuartic(uc, 1, 2, 3, 4, rng.normal(size=(3, 3, 3, 3)), (0, 0, 0), (0, 0, 0), (0, 0, 0))
+    addQuartic(uc, 3, 1, 4, 2, rng.normal(size=(3, 3, 3, 3)), (1, 0, 0), (0, -1, 1), (1, 1, 0))
+    addQuartic(uc, 2, 4, 1, 3, rng.normal(size=(3, 3, 3, 3)), (0, 1, 0), (-1, 0, 1), (0, 0, -1))
+    for b in range(1, 5):
+        addZeemanCoupling(uc, b, rng.normal(size=3))
+    return uc
+
+
+def test_pyrochlore_cubic_quartic_roles_pinned():
+    # total_energy (hamiltonian.jl:70-132, weights 1/3 and 1/4 at :131) equals
+    # the term-by-term sum when every term has distinct sites, and
+    # get_local_field (hamiltonian.jl:3-67) is its gradient: both fail if a
+    # role permutation (lattice.jl:215-227,251-272) or a role offset
+    # (lattice.jl:212-224,244-268) were misread.
+    uc = _generic_pyrochlore_cell()
+    shape = (3, 3, 3)
+    T = orc.Tables(uc, shape)
+    rng = np.random.default_rng(4)
+    s = random_spins(T.N, 1.0, rng)
+    zeeman = sum(-np.dot(uc.field[b][1], s[i]) for b in range(4)
+                 for i in range(b * 27, (b + 1) * 27))
+    E = orc.total_energy(T, s)
+    assert abs(E - (_term_energy(uc, shape, s) + zeeman)) < 1e-11 * max(1.0, abs(E))
+    eps = 1e-6
+    worst = 0.0
+    for i in (0, 13, 27, 40, 54, 67, 81, 107):  # sites of every basis
+        H = orc.local_field(T, s, i + 1)
+        g = np.zeros(3)
+        for a in range(3):
+            sp, sm = s.copy(), s.copy()
+            sp[i, a] += eps
+            sm[i, a] -= eps
+            g[a] = (orc.total_energy(T, sp) - orc.total_energy(T, sm)) / (2 * eps)
+        worst = max(worst, float(np.max(np.abs(g - H))))
+    assert worst <= 1e-7, worst
+
+
+def test_planted_role_permutation_mistake_is_caught():
+    # the same check run on a deliberately wrong contraction: using the
+    # unpermuted tensor for a role-2 site gives a field that is not the
+    # gradient, so the check above can fail
+    uc = Pyrochlore()
+    rng = np.random.default_rng(3)
+    C = rng.normal(size=(3, 3, 3))
+    addCubic(uc, 1, 2, 3, C, (0, 0, 0), (1, 0, 0))
+    shape = (2, 2, 2)
+    T = orc.Tables(uc, shape)
+    s = random_spins(T.N, 1.0, rng)
+    i = _site_index(shape, 2, (1, 0, 1))      # a basis-2 site: role 2 of its term
+    x = np.array((1, 0, 1))
+    i1, i3 = _site_index(shape, 1, x), _site_index(shape, 3, x + np.array((1, 0, 0)))
+    right = np.einsum("abc,a,c->b", C, s[i1], s[i3])
+    wrong = np.einsum("abc,b,c->a", C, s[i1], s[i3])
+    H = orc.local_field(T, s, i + 1)
+    assert np.allclose(H, right, atol=1e-13)
+    assert not np.allclose(H, wrong, atol=1e-3)
+
+
+def test_repeated_basis_term_first_matching_role():
+    # Reference quirk: the role chain of lattice.jl:212-224 / :244-268 is an
+    # if/elseif, so a site takes only its FIRST matching role.  In a term that
+    # repeats a basis (distinct sites, e.g. addCubic!(2,1,1,...)) the later
+    # roles of that basis get no entry.  Then total_energy counts the term
+    # (number of distinct roles hit)/3 (or /4) times, and get_local_field is
+    # the derivative with respect to the first matching role only.
+    uc = Honeycomb()
+    rng = np.random.default_rng(8)
+    C = rng.normal(size=(3, 3, 3))
+    addCubic(uc, 2, 1, 1, C, (1, 0), (0, 1))              # basis 1 matches role 2, never 3
+    shape = (3, 3)
+    T = orc.Tables(uc, shape)
+    s = random_spins(T.N, 1.0, rng)
+    Et = _term_energy(uc, shape, s)
+    assert abs(orc.total_energy(T, s) - 2.0 / 3.0 * Et) < 1e-12
+    H = np.zeros((T.N, 3))
+    import itertools
+    for x in itertools.product(range(3), range(3)):
+        x = np.array(x)
+        i = _site_index(shape, 2, x)
+        j = _site_index(shape, 1, x + (1, 0))
+        k = _site_index(shape, 1, x + (0, 1))
+        H[i] += np.einsum("abc,b,c->a", C, s[j], s[k])   # role 1
+        H[j] += np.einsum("abc,a,c->b", C, s[i], s[k])   # role 2 (first match of basis 1)
+    assert np.allclose(orc.local_fields(T, s), H, atol=1e-12)
+    # so the field is not the gradient of either energy
+    eps = 1e-6
+    p = _site_index(shape, 1, (0, 0))
+    g = np.zeros(3)
+    for a in range(3):
+        sp, sm = s.copy(), s.copy()
+        sp[p, a] += eps
+        sm[p, a] -= eps
+        g[a] = (orc.total_energy(T, sp) - orc.total_energy(T, sm)) / (2 * eps)
+    assert np.max(np.abs(g - H[p])) > 1e-2
+
+    uc = Honeycomb()
+    R = rng.normal(size=(3, 3, 3, 3))
+    addQuartic(uc, 2, 1, 2, 1, R, (0, 0), (1, 1), (-1, 1))  # roles 3 and 4 never taken
+    T = orc.Tables(uc, shape)
+    assert abs(orc.total_energy(T, s) - 0.5 * _term_energy(uc, shape, s)) < 1e-12
+    H = np.zeros((T.N, 3))
+    for x in itertools.product(range(3), range(3)):
+        x = np.array(x)
+        i, j = _site_index(shape, 2, x), _site_index(shape, 1, x)
+        k, m = _site_index(shape, 2, x + (1, 1)), _site_index(shape, 1, x + (-1, 1))
+        H[i] += np.einsum("abcd,b,c,d->a", R, s[j], s[k], s[m])
+        H[j] += np.einsum("abcd,a,c,d->b", R, s[i], s[k], s[m])
+    assert np.allclose(orc.local_fields(T, s), H, atol=1e-12)
+
+
+@pytest.mark.parametrize("sigma", [0.3, 1.0])
+def test_cone_move_langevin(sigma):
+    # the cone proposal s + sigma * (uniform * S), renormalised to S
+    # (metropolis.jl:84-87), is symmetric, so the chain samples the Boltzmann
+    # distribution: <s.h^> = S L(beta h S) for one spin in a field
+    uc = Square()
+    uc.basis.append(np.zeros(2))
+    addZeemanCoupling(uc, 1, [0.0, 0.0, 0.7])
+    T = orc.Tables(uc, (1, 1))
+    s = np.array([[0.0, 0.0, 1.0]])
+    temp = 0.5
+    zs = np.empty(60000)
+    for sw in range(len(zs)):
+        orc.metropolis_random(T, s, 1.0, temp, 1, sigma, 77, 0, sw)
+        zs[sw] = s[0, 2]
+    x = 0.7 / temp
+    exact = 1.0 / math.tanh(x) - 1.0 / x
+    zs = zs[2000:]
+    nb = 200
+    blocks = zs[: len(zs) // nb * nb].reshape(nb, -1).mean(axis=1)
+    err = blocks.std(ddof=1) / math.sqrt(nb)
+    assert abs(zs.mean() - exact) < 4 * err, (zs.mean(), exact, err)
+
+
+def test_cone_move_formula():
+    # metropolis.jl:84-87: normalise(s + sigma * random_spin(S)) * S
+    S, sigma = 1.5, 0.4
+    s = np.array([0.3, -1.2, 0.8])
+    s = S * s / np.linalg.norm(s)
+    u1, u2 = 0.2, 0.9
+    r = orc.random_spin(S, u1, u2)
+    v = s + sigma * r
+    assert np.allclose(orc.cone_move(S, s, sigma, u1, u2), S * v / np.linalg.norm(v), atol=1e-14)
