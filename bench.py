@@ -9,10 +9,12 @@ exchange decisions and observable binning.
 
 One step = one exchange period: 10 overrelaxation sweeps + 1 Metropolis
 sweep + FP64 total energy of every replica + the replica exchange (+ the
-probe), i.e. 11 * 64 * 8192 = 5,767,168 spin updates.  The default timed
-run is as long as the configured schedule, 1e4 thermalisation + 1e5
-measurement sweeps (11,000 steps), after 1024 untimed warm-up steps (one
-chained launch of the sweep kernel).  Metric: spin-updates/s.
+probe), i.e. 11 * 64 * 8192 = 5,767,168 spin updates.  The 1e4
+thermalisation sweeps run first, untimed; then 1024 untimed warm-up steps
+(one chained launch of the sweep kernel) and the timed steps, all in the
+measurement phase, so every timed step carries its probe.  The default
+timed run is the configured 1e5 measurement sweeps (10,000 steps).
+Metric: spin-updates/s.
 
 Beside the batched FP32 form it measures the plain FP64 GPU form (one
 replica at a time, one launch per colour pass) on the same lattice, and the
@@ -52,6 +54,39 @@ import numpy as np  # noqa: E402
 METRIC = "spin-updates/s on 1 MI355X (batched FP32 PT vs plain); % of HBM roofline"
 PEAK_HBM_GBS = 8000.0
 BYTES_PER_UPDATE_FP32 = 24  # read + write own spin, 3 x 4 B each
+# MI355X compute figures for the issue-rate fractions: 256 CUs at a 2.4 GHz
+# peak engine clock; a CU issues 64 non-packed FP32 VALU lane-operations per
+# clock (4 SIMDs x 16 lanes; packed FP32, which would double it, is not used
+# by this build) and its LDS serves 128 B per clock
+CHIP_CUS = 256
+CLOCK_HZ = 2.4e9
+VALU_LANES_PER_CU_CLK = 64
+LDS_BYTES_PER_CU_CLK = 128
+LDS_BYTES_PER_LANE_INST = 16  # the sweep kernels' LDS traffic is 16-byte spin records
+
+
+def issue_fractions(sq, updates_per_dispatch, value, cus_busy):
+    """VALU and LDS fractions of the register-table kernel from its counters
+    (SQ_INSTS_VALU, SQ_INSTS_LDS per dispatch, wave instructions, 64 lanes
+    each) at the live throughput `value` (updates/s): against the whole chip
+    and against the CUs the kernel occupies."""
+    valu_per_update = sq["SQ_INSTS_VALU"] * 64 / updates_per_dispatch
+    lds_per_update = sq["SQ_INSTS_LDS"] * 64 * LDS_BYTES_PER_LANE_INST / updates_per_dispatch
+    valu = valu_per_update * value
+    lds = lds_per_update * value
+    vpeak = VALU_LANES_PER_CU_CLK * CLOCK_HZ
+    lpeak = LDS_BYTES_PER_CU_CLK * CLOCK_HZ
+    return {
+        "valu": {"lane_ops_per_update": round(valu_per_update, 2), "achieved": valu / 1e12,
+                 "unit": "T lane-ops/s", "peak": CHIP_CUS * vpeak / 1e12,
+                 "frac": round(valu / (CHIP_CUS * vpeak), 4),
+                 "frac_busy_cus": round(valu / (cus_busy * vpeak), 4), "busy_cus": cus_busy},
+        "lds": {"bytes_per_update": round(lds_per_update, 1), "achieved": lds / 1e12, "unit": "TB/s",
+                "peak": CHIP_CUS * lpeak / 1e12, "frac": round(lds / (CHIP_CUS * lpeak), 4),
+                "frac_busy_cus": round(lds / (cus_busy * lpeak), 4), "busy_cus": cus_busy},
+        "issue_note": "instruction counts per update from the kernel's SQ counters (profiles/), times "
+                      "the live updates/s; the kernel keeps the spins in LDS, so VALU issue and LDS "
+                      "bandwidth on the CUs it occupies bound it, not HBM"}
 
 
 def log(msg):
@@ -75,13 +110,22 @@ def copy_probe(nbytes, reps=5):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
 
 
+def baseline_oracle():
+    """The oracle as the CPU baseline: its speed build (liboracle_bench.so,
+    -O3 for x86-64-v3), never the product path"""
+    import oracle.oracle as orc
+    if orc._lib is None:
+        orc.use_bench_build()
+    return orc
+
+
 def cpu_baseline(uc, L, S, temps, seconds, threads):
     """The oracle's reference-algorithm PT loop (orc_pt_reference_run) on the
     host: the replicas split into `threads` blocks, one host thread per block
     (the oracle's C loop runs without the GIL), each block exchanging within
     itself.  Bounded to about `seconds` of wall time; returns updates/s and a
     description of the sample."""
-    import oracle.oracle as orc
+    orc = baseline_oracle()
     T = orc.Tables(uc, (L, L))
     R = len(temps)
     threads = max(1, min(threads, R))
@@ -112,10 +156,26 @@ def cpu_baseline(uc, L, S, temps, seconds, threads):
     dt = time.perf_counter() - t0
     total = sum(counts)
     sample = (f"oracle PT loop (random-site Metropolis + sequential overrelaxation + total_energy "
-              f"per Metropolis sweep + even/odd exchange), {R} replicas of the same lattice in "
-              f"{threads} blocks of {R // threads}-{-(-R // threads)} replicas, one host thread per "
-              f"block, {min(sweeps)}-{max(sweeps)} sweeps per block in {dt:.1f} s")
+              f"per Metropolis sweep + even/odd exchange; C, gcc -O3 -march=x86-64-v3), {R} replicas "
+              f"of the same lattice in {threads} blocks of {R // threads}-{-(-R // threads)} replicas, "
+              f"one host thread per block, exchanges within a block, no message cost; "
+              f"{min(sweeps)}-{max(sweeps)} sweeps per block in {dt:.1f} s")
     return total / dt, sample
+
+
+def host_info():
+    """CPU model and core counts of the host the baseline ran on"""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "os_cpu_count": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
 
 
 def finish(out, args):
@@ -202,13 +262,15 @@ def checkpoint_cost(csmc, b, temps):
                     "1024 exchange periods"}
 
 
-def run_pt(args, W, csmc, uc, L, S, R, temps, precision, name):
+def run_pt(args, W, csmc, uc, L, S, R, temps, precision, name, t_th=10 ** 4):
+    """The PT schedule of the workload: the t_th thermalisation sweeps run
+    untimed, then `warmup` untimed and `steps` timed exchange periods, all
+    in the measurement phase (every period has its probe)."""
     lat = csmc.Lattice((L, L) if uc.D == 2 else (L, L, L), uc, S, seed=2)
     N = lat.size
     sweeps_per_step = 10
-    total_sweeps = (args.warmup + args.steps) * sweeps_per_step
-    t_th = 10 ** 4
-    params = dict(t_thermalization=t_th, t_measurement=max(10 ** 5, total_sweeps - t_th), probe_rate=10,
+    meas_sweeps = (args.warmup + args.steps) * sweeps_per_step
+    params = dict(t_thermalization=t_th, t_measurement=max(10 ** 5, meas_sweeps), probe_rate=10,
                   swap_rate=10, overrelaxation_rate=10, report_interval=10 ** 9, checkpoint_rate=10 ** 9)
     ctx = csmc.default_context()
     b = csmc.SpinBatch([lat] * R, precision=precision, ctx=ctx)
@@ -218,11 +280,15 @@ def run_pt(args, W, csmc, uc, L, S, R, temps, precision, name):
     parts = b.pt_split_parts()
     log(f"[bench] {name}: lattice N={N} replicas={R} colours={lat.info.n_colours} storage mode {mode}, "
         f"{parts} workgroup(s) per replica")
-    b.pt_advance(args.warmup * sweeps_per_step)
+    t0 = time.perf_counter()
+    assert b.pt_advance(t_th)[0] == t_th
+    ctx.sync()
+    log(f"[bench] {t_th} thermalisation sweeps done (untimed, {time.perf_counter() - t0:.3f} s)")
+    b.pt_advance(t_th + args.warmup * sweeps_per_step)
     ctx.sync()
     log(f"[bench] warmup {args.warmup} steps done")
     ctx.timing(True)
-    start_sweep = args.warmup * sweeps_per_step
+    start_sweep = t_th + args.warmup * sweeps_per_step
     ctx.sync()
     t0 = time.perf_counter()
     stopped, _ = b.pt_advance(start_sweep + args.steps * sweeps_per_step)
@@ -238,6 +304,8 @@ def run_pt(args, W, csmc, uc, L, S, R, temps, precision, name):
         f"{kern_ms / args.steps * 1e3:.2f} us kernel time per step")
     st = b.pt_state()
     assert np.all(np.isfinite(st["E"])) and st["accepted_exchange"].sum() > 0
+    probes = b.pt_observables()
+    assert np.all(np.isfinite(probes["energy"])), "every timed period is in the measurement phase"
     s0 = b.get_spins(0)
     assert np.allclose(np.linalg.norm(s0, axis=1), S, atol=1e-4 * S)
     return dict(b=b, lat=lat, N=N, dt=dt, kern_ms=kern_ms, launches=launches, value=value,
@@ -249,7 +317,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run device-to-device copies of this many bytes (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=11000)
+    ap.add_argument("--steps", type=int, default=10000,
+                    help="timed exchange periods (10000 = the 1e5 measurement sweeps)")
     ap.add_argument("--warmup", type=int, default=1024,
                     help="untimed steps first (1024 = one whole chained launch)")
     ap.add_argument("--workload", default="pt_honeycomb",
@@ -260,8 +329,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU form")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
-                    help="host threads of the CPU baseline (the GPU box's share is 16 cores)")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, len(os.sched_getaffinity(0))),
+                    help="host threads of the CPU baseline (at most 16: the GPU box's host share "
+                         "for one GPU is 16 cores)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -304,12 +374,18 @@ def main():
     avg_ms = kern_ms / launches
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None
+    fractions = None
     split = r["parts"] > 1
     kname = "k_wg_split" if split else "k_wg_reg"
     tpath = os.path.join(ROOT, "profiles", f"{kname}_pt_honeycomb_L64_R64_traffic.json")
     if L == 64 and R == 64 and os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f)["traffic_bytes_per_step"] * steps_per_launch
+            prof = json.load(f)
+        traffic = prof["traffic_bytes_per_step"] * steps_per_launch
+        if "sq_per_dispatch" in prof:
+            fractions = issue_fractions(prof["sq_per_dispatch"],
+                                        updates_per_step * prof["steps_per_dispatch"], value,
+                                        min(CHIP_CUS, R * r["parts"]))
     kdesc = (f"k_wg_split (register-table PT sweep kernel, each replica split over {r['parts']} "
              "workgroups with halos, chained exchange periods)" if split else
              "k_wg_reg (register-table PT sweep kernel, one workgroup per replica, chained "
@@ -321,6 +397,8 @@ def main():
                 "steps_per_launch": round(steps_per_launch, 2),
                 "note": "spins stay in LDS for a whole launch (a chain of up to 1024 exchange "
                         "periods): HBM traffic is far below the algorithmic 24 B per update"}
+    if fractions:
+        roofline.update(fractions)
 
     out = {
         "metric": METRIC,
@@ -386,7 +464,7 @@ def main():
                                    args.cpu_threads)
         out["cpu_baseline"] = {"value": cpu, "unit": "spin-updates/s",
                                "cores": max(1, min(args.cpu_threads, R)), "kind": "port",
-                               "sample": sample, "gpu_over_cpu": value / cpu}
+                               "sample": sample, "gpu_over_cpu": value / cpu, **host_info()}
         log(f"[bench] oracle CPU: {cpu / 1e6:.1f} M updates/s on {out['cpu_baseline']['cores']} threads")
 
     finish(out, args)
@@ -527,7 +605,7 @@ def bench_anneal_square(args, csmc, W):
     updates.  Timed end to end through the MonteCarlo driver; the ground
     state E/N = -2S^2|J| - hS = -2.1 is checked."""
     import contextlib
-    import oracle.oracle as orc
+    orc = baseline_oracle()
     uc = W.square_heisenberg(-1.0, (0.0, 0.0, 0.1))
     lat = csmc.Lattice((16, 16), uc, 1.0, seed=1)
     N = lat.size
@@ -591,7 +669,7 @@ def bench_adjoint(args, csmc, W):
     configuration on the device: k_gd_step computes dE/dS from the old spins
     and writes the updated spins to a second buffer.  Roofline: 48
     algorithmic bytes per site (read the FP64 spin, write the FP64 spin)."""
-    import oracle.oracle as orc
+    orc = baseline_oracle()
     rng = np.random.default_rng(5)
     n, L = 128, 96
     J2 = rng.uniform(0.0, 0.2, size=n)
@@ -673,7 +751,7 @@ def ssf_cpu_baseline(pos, spins, ks, seconds, threads):
     split over `threads` host threads (the C loop runs without the GIL), one
     system after another until about `seconds` have passed.  Returns
     terms/s (momenta x sites per system) and a description of the sample."""
-    import oracle.oracle as orc
+    orc = baseline_oracle()
     threads = max(1, threads)
     parts = np.array_split(np.arange(ks.shape[1]), threads)
     stop = time.perf_counter() + seconds

@@ -5,9 +5,11 @@ and loaded from this directory.  There is no fallback: if the library is
 missing the import raises, and every device entry point fails with the
 library's own error when no HIP device is present.
 """
+import atexit
 import ctypes as C
 import itertools
 import os
+import weakref
 
 import numpy as np
 
@@ -176,6 +178,31 @@ def device_count():
     return n.value
 
 
+# Live contexts and captured graphs.  They are released by an atexit handler,
+# which runs at the start of interpreter shutdown: their device buffers,
+# page-locked host buffers, rocBLAS handle and stream then go back to the HIP
+# runtime while it is fully up, never from a finaliser that runs in whatever
+# order garbage collection reaches it, after other libraries' exit handlers.
+_live_contexts = weakref.WeakSet()
+_live_graphs = weakref.WeakSet()
+
+
+def _shutdown():
+    global _default_ctx
+    for g in list(_live_graphs):
+        g.close()
+    for c in list(_live_contexts):
+        c.close()
+    _default_ctx = None
+
+
+atexit.register(_shutdown)
+
+
+def register_graph(g):
+    _live_graphs.add(g)
+
+
 class Context:
     """One libcsmc context (HIP stream + table pools) on one device."""
 
@@ -188,9 +215,16 @@ class Context:
         # cache key for per-context device objects: a handle value can be
         # reused by a later context once this one is destroyed
         self.uid = next(Context._serial)
+        _live_contexts.add(self)
 
     def close(self):
+        """Wait for the context's stream and release everything it holds
+        (batches, lattice pools, stream).  Batches of the context are
+        unusable afterwards.  Idempotent."""
         if self.h:
+            for g in list(_live_graphs):
+                if g.batch.ctx is self:
+                    g.close()
             lib().csmc_ctx_destroy(self.h)
             self.h = C.c_void_p()
 

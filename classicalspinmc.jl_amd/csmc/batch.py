@@ -285,6 +285,7 @@ class SweepGraph:
     def __init__(self, batch, handle):
         self.batch = batch
         self.h = handle
+        _lib.register_graph(self)
 
     def launch(self):
         """enqueue one replay (returns without waiting)"""

@@ -26,6 +26,17 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def use_bench_build():
+    """Load liboracle_bench.so (the same source at -O3 for x86-64-v3) instead
+    of the checker build; for bench.py's CPU baseline only, before the first
+    oracle call of the process."""
+    global _SO
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    _SO = os.path.join(_HERE, "liboracle_bench.so")
+    return _SO
+
+
 def lib():
     global _lib
     if _lib is None:
