@@ -125,6 +125,7 @@ SIGNATURES = {
     "csmc_pt_correlations": [_P, _I32, _P, _P],
     "csmc_energy_grad_spins": [_P, _I32, _P],
     "csmc_energy_grad_couplings": [_P, _I32, _P],
+    "csmc_energy_grad_couplings_noalloc": [_P, _I32, _P],
     "csmc_gradient_descent": [_P, _I32, _D, _I32],
 }
 

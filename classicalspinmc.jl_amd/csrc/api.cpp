@@ -14,6 +14,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,13 @@
 
 using namespace csmc;
 
+// Device scratch of the no-allocation entry point
+// (csmc_energy_grad_couplings_noalloc): one array per GPU, held until the
+// process ends.  In host code `scratch` names a host-side stand-in, not a
+// device address: host code uses only the address hipGetSymbolAddress gives
+// for the current device, kept per device by scratch_setup().
+static __device__ __align__(256) unsigned char scratch[CSMC_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_err;
@@ -32,6 +40,27 @@ thread_local std::string g_err;
 int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
+}
+
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;  // device -> address of `scratch` there
+
+// once per device, with that device current (csmc_ctx_create)
+int scratch_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.count(device)) return CSMC_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(CSMC_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+    return CSMC_OK;
+}
+
+void *scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    return it == g_scratch_ptr.end() ? nullptr : it->second;
 }
 
 #define HIPCHK(expr)                                                                     \
@@ -857,6 +886,7 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
         return fail(CSMC_EHIP, "no HIP device available: the engine has no CPU fallback");
     if (device < 0 || device >= n) return fail(CSMC_EINVAL, "device index out of range");
     HIPCHK(hipSetDevice(device));
+    if (int rc = scratch_setup(device)) return rc;
     auto *c = new csmc_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -2409,6 +2439,35 @@ int32_t csmc_energy_grad_couplings(csmc_ctx *c, int32_t id, double *dEdJ) {
     HIPCHK(d.alloc((size_t)gl * b->nsys));
     HIPCHK(launch_grad_couplings(b->prec, pools(c, b), b->spins.p, b->nsys, gl, d.p, c->stream));
     HIPCHK(hipMemcpyAsync(dEdJ, d.p, (size_t)gl * b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_energy_grad_couplings_noalloc(csmc_ctx *c, int32_t id, double *dEdJ) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!dEdJ) return fail(CSMC_EINVAL, "null output");
+    int gl = c->lats[b->lat[0]]->grad_len;
+    for (int s = 1; s < b->nsys; ++s)
+        if (c->lats[b->lat[s]]->grad_len != gl)
+            return fail(CSMC_EINVAL, "coupling gradients need systems with one term layout");
+    if (gl == 0) return CSMC_OK;
+    const size_t bytes = (size_t)gl * b->nsys * sizeof(double);
+    if (bytes > CSMC_NOALLOC_SCRATCH_BYTES)
+        return fail(CSMC_EINVAL, "coupling gradients of this batch need " + std::to_string(bytes) +
+                                     " bytes of scratch, more than the " +
+                                     std::to_string((size_t)CSMC_NOALLOC_SCRATCH_BYTES) +
+                                     " the no-allocation entry point holds; use csmc_energy_grad_couplings");
+    // uploading the lattice tables would allocate: a call that uploads them
+    // (csmc_set_spins, csmc_randomize_spins, any sweep) must come first
+    if (c->dirty)
+        return fail(CSMC_ESTATE, "lattice tables not on the device yet (csmc_set_spins or "
+                                 "csmc_randomize_spins uploads them)");
+    void *sp = scratch_ptr(c->device);
+    if (!sp) return fail(CSMC_ESTATE, "scratch of this device not set up");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(launch_grad_couplings(b->prec, pools(c, b), b->spins.p, b->nsys, gl, (double *)sp, c->stream));
+    HIPCHK(hipMemcpyAsync(dEdJ, sp, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return CSMC_OK;
 }

@@ -329,6 +329,20 @@ int32_t csmc_energy_grad_spins(csmc_ctx *ctx, int32_t batch_id, double *dEdS);
  * out[n_sys][n_field*3 + n_onsite*9 + n2*9 + n3*27 + n4*81] in unit-cell term
  * order, each tensor in the csmc_unit_cell layout. */
 int32_t csmc_energy_grad_couplings(csmc_ctx *ctx, int32_t batch_id, double *dEdJ);
+/* No counterpart in the reference.  csmc_energy_grad_couplings for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * fitting loops, harnesses that forbid hipMalloc): the same kernel and
+ * output, with the per-system gradients staged in a fixed device array of
+ * CSMC_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) instead
+ * of a buffer allocated per call.  No hipMalloc / hipFree or allocating
+ * library routine runs.  Limits: n_sys * (gradient length) * 8 bytes must not
+ * exceed CSMC_NOALLOC_SCRATCH_BYTES (CSMC_EINVAL otherwise), and the
+ * lattice tables must already be on the device (any csmc_set_spins,
+ * csmc_randomize_spins or sweep uploads them; CSMC_ESTATE otherwise).  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (from several contexts or threads) race. */
+#define CSMC_NOALLOC_SCRATCH_BYTES (1 << 20)
+int32_t csmc_energy_grad_couplings_noalloc(csmc_ctx *ctx, int32_t batch_id, double *dEdJ);
 /* One projected gradient-descent step on the sphere for every system:
  * s <- S * normalise(s - eta * (g - (g.s^) s^)), g = dE/ds. */
 int32_t csmc_gradient_descent(csmc_ctx *ctx, int32_t batch_id, double eta, int32_t n_steps);
