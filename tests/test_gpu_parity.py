@@ -13,6 +13,7 @@ import oracle.oracle as orc
 from csmc import workloads as W
 from csmc.unitcell import Honeycomb, addBilinear, addCubic, addOnSite, addQuartic, addZeemanCoupling
 from conftest import random_spins
+from pt_replay import pt_replay
 
 pytestmark = pytest.mark.gpu
 
@@ -144,33 +145,7 @@ def test_metropolis_decisions_identical(case, kind, plain):
 
 
 def _pt_replay(T, spins, order, temps, S, p, seed, alg_kind):
-    """the reference's PT loop (monte_carlo.jl:295-349) over coloured sweeps,
-    driven from the oracle: returns (spins per system, slot->system, E,
-    acc_local per slot, acc_exch per slot)"""
-    R = len(temps)
-    perm = np.arange(R, dtype=np.intc)
-    E = np.array([orc.total_energy(T, spins[r]) for r in range(R)])
-    Eslot = E.copy()
-    acc_l = np.zeros(R)
-    acc_x = np.zeros(R, dtype=np.int_)
-    mcount = np.zeros(R, dtype=np.int64)
-    dosweep = p["overrelaxation_rate"] or 1
-    total = p["t_thermalization"] + p["t_measurement"]
-    for sw in range(total):
-        for r in range(R):  # r = system
-            slot = int(np.nonzero(perm == r)[0][0])
-            if p["overrelaxation_rate"]:
-                orc.overrelax(T, spins[r], order)
-            if sw % dosweep == 0:
-                a, _, _ = orc.metropolis_coloured(T, spins[r], order, S, temps[slot], alg_kind, 60.0, seed, r,
-                                                  int(mcount[r]))
-                mcount[r] += 1
-                acc_l[slot] += a
-                E[r] = orc.total_energy(T, spins[r])
-        if R > 1 and sw % dosweep == 0 and sw % p["swap_rate"] == 0:
-            Eslot = np.ascontiguousarray(E[perm])
-            orc.pt_exchange(temps, Eslot, perm, seed, sw // p["swap_rate"], acc_x)
-    return spins, perm, E, acc_l, acc_x
+    return pt_replay(T, spins, order, temps, S, p, seed, alg_kind)[:5]
 
 
 @pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[5]], ids=[IDS[0], IDS[2], IDS[5]])
