@@ -5,7 +5,7 @@ The reference API is mirrored with Julia's `!` dropped from mutating names
 Everything that touches spins runs in libcsmc on the GPU; there is no CPU
 fallback.
 """
-from ._lib import (CONE, CONE_ADAPTIVE, DETERMINISTIC, FP32, FP64, METROPOLIS, OVERRELAX,
+from ._lib import (CONE, CONE_ADAPTIVE, CONSTRAINT_ADAPTIVE, DETERMINISTIC, FP32, FP64, METROPOLIS, OVERRELAX,
                    BoundsError, Context, CsmcError, UndefError, default_context, device_count, lib)
 from .batch import SpinBatch
 from .hdf5io import (create_params_file, dump_metadata, h5open, overwrite_keys, read_lattice,
@@ -17,7 +17,7 @@ from .reciprocal import get_allowed_wavevectors, get_k_path, get_k_plane, recipr
 from .spin_correlations import compute_equal_time_structure_factor, runEqualTimeStructureFactor
 from .montecarlo import (MCParamsBuffer, Metropolis, MetropolisAdaptive, MetropolisConstraint,
                          MetropolisConstraintAdaptive, MetropolisFixedCone, MonteCarlo,
-                         SimulationParameters, deterministic_updates, overrelaxation,
+                         SimulationParameters, SiteSumConstraint, deterministic_updates, overrelaxation,
                          parallel_tempering, simulated_annealing)
 from .unitcell import (FCC, BreathingPyrochlore, Honeycomb, IMToMatrix, InteractionMatrix, Pyrochlore,
                        Square, Triangular, UnitCell, addBasisSite, addBilinear, addCubic, addOnSite,

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libcsmc.so")
 
 OK, EINVAL, EBOUNDS, EUNDEF, EBC, EHIP, ENOMEM, ESTATE = range(8)
 FP32, FP64 = 32, 64
-OVERRELAX, METROPOLIS, CONE, CONE_ADAPTIVE, DETERMINISTIC = range(5)
+OVERRELAX, METROPOLIS, CONE, CONE_ADAPTIVE, DETERMINISTIC, CONSTRAINT_ADAPTIVE = range(6)
 
 
 class CsmcError(RuntimeError):
@@ -127,6 +127,8 @@ SIGNATURES = {
     "csmc_energy_grad_couplings": [_P, _I32, _P],
     "csmc_energy_grad_couplings_noalloc": [_P, _I32, _P],
     "csmc_gradient_descent": [_P, _I32, _D, _I32],
+    "csmc_constraint_set": [_P, _I32, _I32, _P, _P, _D, _D, _D],
+    "csmc_constraint_lambda": [_P, _I32, _P, _P],
 }
 
 _lib = None

@@ -113,6 +113,25 @@ class SpinBatch:
         c = np.ascontiguousarray(np.broadcast_to(np.asarray(c, dtype=np.uint32), (self.n,)).copy())
         _lib.check(_lib.lib().csmc_set_counters(self._h, self.id, _lib.ptr(c)))
 
+    def set_constraint(self, a, Q, c0, weight, sigma0=60.0):
+        """the site-sum constraint c = sum_i (a_b . s_i + s_i . Q_b s_i) - c0
+        of metropolis_constraint_adaptive! (csmc_constraint_set); a:
+        (n_basis, 3), Q: (n_basis, 3, 3)"""
+        a = _lib.as_f64(a).reshape(-1, 3)
+        Q = _lib.as_f64(Q).reshape(-1, 9)
+        if len(a) != len(Q):
+            raise ValueError("one a and one Q per basis")
+        _lib.check(_lib.lib().csmc_constraint_set(self._h, self.id, len(a), _lib.ptr(a), _lib.ptr(Q),
+                                                  float(c0), float(weight), float(sigma0)))
+
+    def constraint_lambda(self, set_to=None):
+        """MonteCarlo.lambda per system (set first when set_to is given)"""
+        s = None if set_to is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(set_to, dtype=np.float64), (self.n,)).copy())
+        out = np.zeros(self.n)
+        _lib.check(_lib.lib().csmc_constraint_lambda(self._h, self.id, _lib.ptr(s), _lib.ptr(out)))
+        return out
+
     def anneal(self, temps, t_thermalization, overrelaxation_rate, alg, sigma0=60.0, seed=0):
         temps = _lib.as_f64(temps).reshape(-1)
         acc = np.zeros((self.n, len(temps)))

@@ -63,10 +63,14 @@ class _Alg:
 
     def __call__(self, mc, T):
         b = mc._batch()
+        if self.kind == _lib.CONSTRAINT_ADAPTIVE:
+            _install_constraint(mc, b)
         sg = np.array([float(mc.sigma)])
         acc = b.sweep(self.kind, T=T, sigma=sg, n_sweeps=1, seed=mc.seed)
-        if self.kind == _lib.CONE_ADAPTIVE:
+        if self.kind in (_lib.CONE_ADAPTIVE, _lib.CONSTRAINT_ADAPTIVE):
             mc.sigma = float(sg[0])
+        if self.kind == _lib.CONSTRAINT_ADAPTIVE:
+            mc.lambda_ = float(b.constraint_lambda()[0])
         mc._device_dirty = True
         return float(acc[0])
 
@@ -86,15 +90,68 @@ def MetropolisFixedCone():
     return _Alg(_lib.CONE, "MetropolisFixedCone")
 
 
+def _kind(alg):
+    """the device sweep kind of an `alg` plugin; a plugin with no device form
+    (a user function of (mc, T)) is refused with the reason"""
+    if isinstance(alg, _Alg):
+        return alg.kind
+    raise TypeError(
+        f"alg={alg!r} has no device form: the drivers run the alg plugin as a coloured sweep kernel, "
+        "so alg must be Metropolis(), MetropolisAdaptive(), MetropolisFixedCone() "
+        "(metropolis.jl:181-199) or MetropolisConstraintAdaptive() with a SiteSumConstraint")
+
+
+class SiteSumConstraint:
+    """The device form of MonteCarlo.constraint for
+    metropolis_constraint_adaptive! (metropolis.jl:155-179): a sum over sites
+    of a per-basis linear and quadratic term,
+        c(lattice) = sum_i (a_b . s_i + s_i . Q_b s_i) - c0,  b = basis of i,
+    e.g. a fixed magnetisation along n (a_b = n, Q_b = 0, c0 = N m0) or a
+    staggered one (a_b = +-n per basis).  Calling it on a Lattice evaluates it
+    on the host, as the reference's closure would be called."""
+
+    def __init__(self, a, Q=None, c0=0.0):
+        self.a = np.asarray(a, dtype=np.float64).reshape(-1, 3)
+        self.Q = (np.zeros((len(self.a), 3, 3)) if Q is None
+                  else np.asarray(Q, dtype=np.float64).reshape(-1, 3, 3))
+        if len(self.Q) != len(self.a):
+            raise ValueError("one a and one Q per basis")
+        self.c0 = float(c0)
+
+    def __call__(self, lattice):
+        s = np.asarray(lattice._spins)
+        b = np.repeat(np.arange(len(lattice.unit_cell.basis)), lattice.size // len(lattice.unit_cell.basis))
+        c = 0.0
+        for k in range(min(len(self.a), b.max() + 1)):
+            sk = s[b == k]
+            c += float(np.sum(sk @ self.a[k]) + np.einsum("ia,ab,ib->", sk, self.Q[k], sk))
+        return c - self.c0
+
+
+def _install_constraint(mc, b):
+    """hand mc.constraint, mc.weight, mc.sigma0 and mc.lambda to the batch"""
+    if not isinstance(mc.constraint, SiteSumConstraint):
+        raise TypeError("MetropolisConstraintAdaptive on the device needs mc.constraint to be a "
+                        "SiteSumConstraint: the reference calls an arbitrary Julia function per proposal "
+                        "(metropolis.jl:164), which a GPU kernel cannot run")
+    k = mc.constraint
+    b.set_constraint(k.a, k.Q, k.c0, mc.weight, mc.sigma0)
+    b.constraint_lambda(mc.lambda_)
+
+
 def MetropolisConstraint():
-    """metropolis_constraint! evaluates a user closure over the whole lattice
-    per proposal and calls an undefined function (metropolis.jl:44-45): it
-    is not part of this engine."""
-    raise NotImplementedError("MetropolisConstraint is out of scope (see metropolis.jl:36-58)")
+    """metropolis_constraint! (metropolis.jl:36-58) calls e_diff, which the
+    reference never defines (:44): calling it raises in the reference, so it
+    has no device form."""
+    raise NotImplementedError("MetropolisConstraint calls the undefined e_diff (metropolis.jl:44); "
+                              "use MetropolisConstraintAdaptive")
 
 
 def MetropolisConstraintAdaptive():
-    raise NotImplementedError("MetropolisConstraintAdaptive is out of scope (see metropolis.jl:155-179)")
+    """metropolis_constraint_adaptive! (metropolis.jl:155-179) for a
+    SiteSumConstraint in mc.constraint: runs in csmc_sweep (the plugin call)
+    and simulated_annealing; parallel tempering refuses it."""
+    return _Alg(_lib.CONSTRAINT_ADAPTIVE, "MetropolisConstraintAdaptive")
 
 
 class MonteCarlo:
@@ -165,14 +222,19 @@ def _schedule_temps(schedule, T0, Tmin):
 def simulated_annealing(mc, schedule, T0=1.0, alg=None):
     """simulated_annealing! (monte_carlo.jl:157-190)."""
     alg = alg or Metropolis()
+    kind = _kind(alg)
     p = mc.parameters
     N = mc.lattice.size
     accept_total = p.t_thermalization * N
     if p.overrelaxation_rate != 0:
         accept_total /= p.overrelaxation_rate
     b = mc._batch()
+    if kind == _lib.CONSTRAINT_ADAPTIVE:
+        _install_constraint(mc, b)
     for T in _schedule_temps(schedule, T0, mc.T):
-        acc = b.anneal([T], p.t_thermalization, p.overrelaxation_rate, alg.kind, mc.sigma0, mc.seed)
+        acc = b.anneal([T], p.t_thermalization, p.overrelaxation_rate, kind, mc.sigma0, mc.seed)
+        if kind == _lib.CONSTRAINT_ADAPTIVE:
+            mc.lambda_ = float(b.constraint_lambda()[0])
         R = float(acc[0, 0])
         print(f"Acceptance rate at T={T}: {round(R / accept_total * 100, 5)} % ")
         mc.sigma = mc.sigma0
@@ -242,6 +304,12 @@ def parallel_tempering(mcs, saveIC=(), alg=None, verbose=True, resume=False):
     single = isinstance(mcs, MonteCarlo)
     mcs = [mcs] if single else list(mcs)
     alg = alg or Metropolis()
+    kind = _kind(alg)
+    if kind == _lib.CONSTRAINT_ADAPTIVE:
+        raise NotImplementedError(
+            "MetropolisConstraintAdaptive in parallel tempering: the device PT loop runs every sweep of a "
+            "replica in one persistent kernel, and the constraint's per-proposal multiplier chain is run "
+            "by its own kernels (csmc_sweep / simulated_annealing)")
     p = mcs[0].parameters
     R = len(mcs)
     if R == 1:
@@ -264,7 +332,7 @@ def parallel_tempering(mcs, saveIC=(), alg=None, verbose=True, resume=False):
     for r, m in enumerate(mcs):
         b.set_spins(r, m.lattice._spins)
     params = p.as_dict()
-    b.pt_init(temps, params, alg.kind, sigma0=mcs[0].sigma0, seed=mcs[0].seed)
+    b.pt_init(temps, params, kind, sigma0=mcs[0].sigma0, seed=mcs[0].seed)
     # corr = true: S(k) at every probe sweep, averaged per temperature slot
     # on the device (monte_carlo.jl:371-375)
     ks = mcs[0].momentum_vectors

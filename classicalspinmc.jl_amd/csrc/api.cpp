@@ -199,6 +199,15 @@ struct Batch {
     int lm_cap = 3;
     bool passes = false;
     PTHost pt;
+    // metropolis_constraint_adaptive! (csmc_constraint_set): the per-basis
+    // coefficients, the multiplier lambda, constraint value and accepted
+    // counts per system, and per spin slot the proposal, energy change,
+    // constraint change, uniform and decision of the current colour pass
+    bool cons = false;
+    int cons_nb = 0;
+    double cons_c0 = 0.0, cons_w = 0.0, cons_sigma0 = 60.0;
+    DevBuf<double> cons_coef, cons_lambda, cons_c, cons_acc, cons_total, cons_dE, cons_dC, cons_u;
+    DevBuf<unsigned char> cons_prop, cons_dec;
     size_t spin_bytes() const { return prec == 32 ? 16 : 32; }
 };
 
@@ -863,6 +872,7 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
 namespace {
 int sweep_passes(csmc_ctx *c, Batch *b, int32_t kind, const double *T, const double *sigma, int32_t n_sweeps,
                  uint64_t seed, double *accepted);
+int constraint_run(csmc_ctx *c, Batch *b, uint64_t seed, int64_t n_sweeps);
 }  // namespace
 
 extern "C" {
@@ -1236,6 +1246,24 @@ int32_t csmc_sweep(csmc_ctx *c, int32_t id, int32_t kind, const double *T, doubl
                    int32_t n_sweeps, uint64_t seed, double *accepted) {
     CHECK_CTX(c);
     GET_BATCH(c, id, b);
+    if (kind == CSMC_CONSTRAINT_ADAPTIVE) {
+        if (!b->cons) return fail(CSMC_ESTATE, "no constraint on this batch (csmc_constraint_set)");
+        if (!T) return fail(CSMC_EINVAL, "Metropolis sweeps need temperatures");
+        if (n_sweeps < 0) return fail(CSMC_EINVAL, "n_sweeps < 0");
+        int rc = ensure_pools(c);
+        if (rc) return rc;
+        std::vector<double> sg(b->nsys, b->cons_sigma0);
+        if (sigma) sg.assign(sigma, sigma + b->nsys);
+        HIPCHK(hipMemcpyAsync(b->T.p, T, b->nsys * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(b->sigma.p, sg.data(), b->nsys * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemsetAsync(b->cons_total.p, 0, b->nsys * 8, c->stream));
+        rc = constraint_run(c, b, seed, n_sweeps);
+        if (rc) return rc;
+        if (accepted) HIPCHK(hipMemcpyAsync(accepted, b->cons_total.p, b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
+        if (sigma) HIPCHK(hipMemcpyAsync(sigma, b->sigma.p, b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return CSMC_OK;
+    }
     if (!valid_kind(kind)) return fail(CSMC_EINVAL, "unknown sweep kind");
     if (b->passes && kind != CSMC_CONE_ADAPTIVE)  // chosen by csmc_autotune
         return sweep_passes(c, b, kind, T, sigma, n_sweeps, seed, accepted);
@@ -1428,6 +1456,49 @@ int32_t csmc_autotune(csmc_ctx *c, int32_t id, int32_t kind, const double *T, in
 namespace {
 
 // the batch-wide colour passes (csmc_sweep_passes)
+// Colour passes of metropolis_constraint_adaptive! (constraint.hip): n
+// sweeps of every system of the batch at the temperatures and cone widths
+// already in b->T / b->sigma; accepted counts add up in cons_total.
+int constraint_run(csmc_ctx *c, Batch *b, uint64_t seed, int64_t n_sweeps) {
+    ConsArgs a{};
+    a.P = pools(c, b);
+    a.spins = b->spins.p;
+    a.k0 = (uint32_t)seed;
+    a.k1 = (uint32_t)(seed >> 32);
+    a.ctr = b->mcount.p;
+    a.T_sys = b->T.p;
+    a.sigma_sys = b->sigma.p;
+    a.coef = b->cons_coef.p;
+    a.cnb = b->cons_nb;
+    a.c0 = b->cons_c0;
+    a.weight = b->cons_w;
+    a.sigma0 = b->cons_sigma0;
+    a.prop = b->cons_prop.p;
+    a.dE = b->cons_dE.p;
+    a.dC = b->cons_dC.p;
+    a.u = b->cons_u.p;
+    a.dec = b->cons_dec.p;
+    a.lambda = b->cons_lambda.p;
+    a.cval = b->cons_c.p;
+    a.acc_sweep = b->cons_acc.p;
+    a.acc_total = b->cons_total.p;
+    int maxcol = 0, maxsites = 0;
+    for (int s = 0; s < b->nsys; ++s) {
+        const HostLattice &L = *c->lats[b->lat[s]];
+        maxcol = std::max(maxcol, L.ncol);
+        for (int col = 0; col < L.ncol; ++col)
+            maxsites = std::max(maxsites, L.runs[L.colrun[col + 1] - 1][1] - L.runs[L.colrun[col]][0]);
+    }
+    for (int64_t sw = 0; sw < n_sweeps; ++sw) {
+        HIPCHK(launch_cons_begin(b->prec, b->nsys, a, c->stream));
+        for (int col = 0; col < maxcol; ++col) {
+            a.colour = col;
+            HIPCHK(launch_cons_pass(b->prec, b->self, b->gen, b->nsys, maxsites, a, c->stream));
+        }
+    }
+    return CSMC_OK;
+}
+
 int sweep_passes(csmc_ctx *c, Batch *b, int32_t kind, const double *T, const double *sigma, int32_t n_sweeps,
                  uint64_t seed, double *accepted) {
     if (!valid_kind(kind)) return fail(CSMC_EINVAL, "unknown sweep kind");
@@ -1626,11 +1697,57 @@ int32_t csmc_anneal(csmc_ctx *c, int32_t id, const double *temps, int32_t n_temp
     CHECK_CTX(c);
     GET_BATCH(c, id, b);
     if (n_temps < 0 || (n_temps && !temps)) return fail(CSMC_EINVAL, "bad temperature list");
-    if (alg < CSMC_METROPOLIS || alg > CSMC_CONE_ADAPTIVE) return fail(CSMC_EINVAL, "alg must be a Metropolis kind");
+    if (alg < CSMC_METROPOLIS || alg > CSMC_CONSTRAINT_ADAPTIVE || alg == CSMC_DETERMINISTIC)
+        return fail(CSMC_EINVAL, "alg must be a Metropolis kind");
     if (or_rate < 0) return fail(CSMC_EINVAL, "overrelaxation_rate < 0");
+    if (alg == CSMC_CONSTRAINT_ADAPTIVE && !b->cons)
+        return fail(CSMC_ESTATE, "no constraint on this batch (csmc_constraint_set)");
     if (n_temps == 0) return CSMC_OK;
     int rc = ensure_pools(c);
     if (rc) return rc;
+    if (alg == CSMC_CONSTRAINT_ADAPTIVE) {
+        // simulated_annealing! (monte_carlo.jl:166-187) driven from the host:
+        // per temperature, sigma reset to sigma0 and t = 1 .. t_th - 1 sweep
+        // steps of overrelaxation (one persistent launch per run of them) and
+        // the constraint sweep every or_rate-th step
+        std::vector<double> accT((size_t)b->nsys * n_temps, 0.0), got(b->nsys);
+        for (int k = 0; k < n_temps; ++k) {
+            std::vector<double> Tk(b->nsys, temps[k]), sg(b->nsys, sigma0);
+            HIPCHK(hipMemcpyAsync(b->T.p, Tk.data(), b->nsys * 8, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(b->sigma.p, sg.data(), b->nsys * 8, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemsetAsync(b->cons_total.p, 0, b->nsys * 8, c->stream));
+            int64_t pending_or = 0;
+            auto flush_or = [&]() -> int {
+                if (pending_or == 0) return CSMC_OK;
+                WGArgs w = wg_base(c, b, seed);
+                w.mode = WG_SWEEPS;
+                w.kind = CSMC_OVERRELAX;
+                w.n_sweeps = (int)pending_or;
+                w.T_sys = b->T.p;
+                w.acc_sys = b->acc.p;
+                pending_or = 0;
+                return run_wg(c, b, w);
+            };
+            for (int64_t t = 1; t < t_th; ++t) {
+                bool metro = true;
+                if (or_rate != 0) {
+                    ++pending_or;
+                    metro = t % or_rate == 0;
+                }
+                if (metro) {
+                    if ((rc = flush_or())) return rc;
+                    if ((rc = constraint_run(c, b, seed, 1))) return rc;
+                }
+            }
+            if ((rc = flush_or())) return rc;
+            HIPCHK(hipMemcpyAsync(got.data(), b->cons_total.p, b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if ((rc = parts_check(b))) return rc;
+            for (int s = 0; s < b->nsys; ++s) accT[(size_t)s * n_temps + k] = got[s];
+        }
+        if (accepted) std::copy(accT.begin(), accT.end(), accepted);
+        return CSMC_OK;
+    }
     DevBuf<double> dt, dacc;
     HIPCHK(dt.alloc(n_temps));
     HIPCHK(dacc.alloc((size_t)n_temps * b->nsys));
@@ -2409,6 +2526,50 @@ int32_t csmc_pt_correlations(csmc_ctx *c, int32_t id, double *mean, int64_t *cou
         for (size_t i = 0; i < n; ++i) mean[i] = pt.corr_count > 0 ? mean[i] * inv : inv;
     }
     if (count) *count = pt.corr_count;
+    return CSMC_OK;
+}
+
+int32_t csmc_constraint_set(csmc_ctx *c, int32_t id, int32_t n_basis, const double *a, const double *Q,
+                            double c0, double weight, double sigma0) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (n_basis < 0 || (n_basis && (!a || !Q))) return fail(CSMC_EINVAL, "bad constraint coefficients");
+    std::vector<double> coef((size_t)std::max(n_basis, 1) * 12, 0.0);
+    for (int k = 0; k < n_basis; ++k) {
+        for (int d = 0; d < 3; ++d) coef[k * 12 + d] = a[k * 3 + d];
+        for (int d = 0; d < 9; ++d) coef[k * 12 + 3 + d] = Q[k * 9 + d];
+    }
+    const size_t slots = b->spins.n / b->spin_bytes();
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(b->cons_coef.alloc(coef.size()));
+    HIPCHK(hipMemcpy(b->cons_coef.p, coef.data(), coef.size() * 8, hipMemcpyHostToDevice));
+    if (!b->cons) {
+        HIPCHK(b->cons_lambda.alloc(b->nsys));
+        HIPCHK(hipMemset(b->cons_lambda.p, 0, b->nsys * 8));  // MonteCarlo: lambda = 0.0
+        HIPCHK(b->cons_c.alloc(b->nsys));
+        HIPCHK(b->cons_acc.alloc(b->nsys));
+        HIPCHK(b->cons_total.alloc(b->nsys));
+        HIPCHK(b->cons_dE.alloc(slots));
+        HIPCHK(b->cons_dC.alloc(slots));
+        HIPCHK(b->cons_u.alloc(slots));
+        HIPCHK(b->cons_dec.alloc(slots));
+        HIPCHK(b->cons_prop.alloc(slots * b->spin_bytes()));
+    }
+    b->cons_nb = n_basis;
+    b->cons_c0 = c0;
+    b->cons_w = weight;
+    b->cons_sigma0 = sigma0;
+    b->cons = true;
+    return CSMC_OK;
+}
+
+int32_t csmc_constraint_lambda(csmc_ctx *c, int32_t id, const double *set, double *get) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!b->cons) return fail(CSMC_ESTATE, "no constraint on this batch (csmc_constraint_set)");
+    if (set) HIPCHK(hipMemcpyAsync(b->cons_lambda.p, set, b->nsys * 8, hipMemcpyHostToDevice, c->stream));
+    if (get) HIPCHK(hipMemcpyAsync(get, b->cons_lambda.p, b->nsys * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return CSMC_OK;
 }
 

@@ -562,13 +562,11 @@ __device__ __forceinline__ void metro_commit(const LatView<R> &L, const SV &X, I
     }
 }
 
+// energy(new) - energy(old) of the proposal n for the spin s at site i
+// (calculate_energy_diff!, metropolis.jl:94-110)
 template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
-__device__ __forceinline__ int metro_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
-                                            int i, R T, R invT, int kind, R sigma, uint32_t k0,
-                                            uint32_t k1, uint32_t stream, uint32_t ctr) {
-    spin4<R> s = X.ld(i);
-    uint4 r = philox4x32_10(make_uint4((uint32_t)L.ref[i], stream, ctr, TAG_METRO), k0, k1);
-    spin4<R> n = metro_propose<R>(L, s, r, kind, sigma);
+__device__ __forceinline__ R metro_eval(const LatView<R> &L, const NB &nb, const SV &X, int b, int i,
+                                        const spin4<R> &s, const spin4<R> &n) {
     R dE;
     if (!SELF) {
         R F[3];
@@ -599,6 +597,17 @@ __device__ __forceinline__ int metro_update(const LatView<R> &L, const NB &nb, c
                (n.x * h[0] + n.y * h[1] + n.z * h[2]);
         dE = en - eo;
     }
+    return dE;
+}
+
+template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
+__device__ __forceinline__ int metro_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
+                                            int i, R T, R invT, int kind, R sigma, uint32_t k0,
+                                            uint32_t k1, uint32_t stream, uint32_t ctr) {
+    spin4<R> s = X.ld(i);
+    uint4 r = philox4x32_10(make_uint4((uint32_t)L.ref[i], stream, ctr, TAG_METRO), k0, k1);
+    spin4<R> n = metro_propose<R>(L, s, r, kind, sigma);
+    const R dE = metro_eval<R, SELF, GEN, K2C>(L, nb, X, b, i, s, n);
     int acc = metro_accept<R>(dE, r, T, invT);
     metro_commit<R>(L, X, i, acc, s, n);
     return acc;

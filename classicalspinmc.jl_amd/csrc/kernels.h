@@ -96,6 +96,34 @@ struct PassArgs {
     uint32_t *acc;           // [nsys] accepted proposals
 };
 
+// metropolis_constraint_adaptive! sweeps (constraint.hip) for the constraint
+// c = sum_i (a_b . s_i + s_i . Q_b s_i) - c0
+struct ConsArgs {
+    DevPools P;
+    void *spins;
+    int colour;
+    uint32_t k0, k1;
+    uint32_t *ctr;          // [nsys] Philox Metropolis sweep counters
+    const double *T_sys;    // [nsys]
+    double *sigma_sys;      // [nsys] cone widths (updated after a sweep)
+    const double *coef;     // [cnb][12]: a_b (3) then Q_b (9, row-major)
+    int cnb;
+    double c0, weight, sigma0;
+    void *prop;             // [spin slots] proposed spin records
+    double *dE, *dC, *u;    // [spin slots]
+    uint8_t *dec;           // [spin slots] decisions
+    double *lambda, *cval;  // [nsys]
+    double *acc_sweep;      // [nsys] accepted proposals of the current sweep
+    double *acc_total;      // [nsys] accepted proposals since the call began
+};
+
+// c of every system and a cleared sweep count, at the start of each sweep
+hipError_t launch_cons_begin(int prec, int nsys, const ConsArgs &a, hipStream_t st);
+// one colour pass (a.colour) of every system: proposals, the per-system
+// chain of decisions, the stores
+hipError_t launch_cons_pass(int prec, bool self, bool gen, int nsys, int max_colour_sites, const ConsArgs &a,
+                            hipStream_t st);
+
 int max_lds_bytes();
 // raise a kernel's dynamic LDS limit to the device maximum (once per kernel
 // and device; later calls return at once)

@@ -43,6 +43,9 @@ extern "C" {
 #define CSMC_CONE 2            /* metropolis_fixed_cone! metropolis.jl:136-153  */
 #define CSMC_CONE_ADAPTIVE 3   /* metropolis_adaptive!   metropolis.jl:113-133  */
 #define CSMC_DETERMINISTIC 4   /* deterministic_updates! monte_carlo.jl:201-213 */
+/* metropolis_constraint_adaptive! (metropolis.jl:155-179) for the constraint
+ * set by csmc_constraint_set: csmc_sweep and csmc_anneal only */
+#define CSMC_CONSTRAINT_ADAPTIVE 5
 
 typedef struct csmc_ctx csmc_ctx;
 typedef struct csmc_graph csmc_graph;
@@ -196,6 +199,28 @@ int32_t csmc_get_counters(csmc_ctx *ctx, int32_t batch_id, uint32_t *counters);
  * neighbour tables, 1 LDS spins, 0 global memory); the environment variable
  * CSMC_LDS=0..3, read at context creation, caps it. */
 int32_t csmc_batch_storage_mode(csmc_ctx *ctx, int32_t batch_id, int32_t *mode);
+
+/* ---- metropolis_constraint_adaptive! (metropolis.jl:155-179) ---------------
+ * The reference evaluates mc.constraint(mc.lattice), any Julia function, per
+ * proposal.  The device form takes a constraint that is a sum over sites of a
+ * linear and a quadratic term per basis:
+ *     c = sum_i (a_b . s_i + s_i . Q_b s_i) - c0,   b = basis of site i,
+ * a[n_basis][3], Q[n_basis][9] row-major (bases past n_basis contribute 0),
+ * with weight = MonteCarlo.weight and sigma0 = MonteCarlo.sigma0 (the cone
+ * width the adaptive law resets to).  Each proposal is a cone move (width
+ * sigma[sys]); c is evaluated with the proposed spin in place; new_lambda =
+ * lambda - weight * c; acceptance on dE - new_lambda * c in FP64; lambda
+ * moves on with every accepted proposal.  Sites are visited colour class by
+ * colour class in the device order (the per-proposal chain of lambda runs in
+ * that order, one thread per system).  After each sweep sigma follows
+ * metropolis.jl:176-177.  Sets or replaces the constraint of every system of
+ * the batch; lambda starts at 0 per system (MonteCarlo(), monte_carlo.jl:76)
+ * the first time.  Then csmc_sweep / csmc_anneal with
+ * kind = CSMC_CONSTRAINT_ADAPTIVE run it (csmc_pt_init refuses it). */
+int32_t csmc_constraint_set(csmc_ctx *ctx, int32_t batch_id, int32_t n_basis, const double *a,
+                            const double *Q, double c0, double weight, double sigma0);
+/* MonteCarlo.lambda per system: set (may be NULL) then get (may be NULL) */
+int32_t csmc_constraint_lambda(csmc_ctx *ctx, int32_t batch_id, const double *set, double *get);
 
 /* ---- simulated_annealing! (monte_carlo.jl:157-190) ------------------------ */
 /* For each temperature temps[k] (k < n_temps, the caller evaluates the

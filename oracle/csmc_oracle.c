@@ -40,6 +40,10 @@
  *   orc_metropolis_*   single spin in a field: <s.h> = S L(beta h S) (Langevin),
  *                      with the uniform proposal and with the cone proposal
  *                      (metropolis.jl:84-87) at two cone widths.
+ *   orc_metropolis_    weight 0 gives the plain cone chain draw for draw; a
+ *   constraint_coloured  free spin with c = s_z - m0 and a strong weight is
+ *                      driven to <s_z> = m0; lambda follows its recurrence
+ *                      on a hand-worked two-proposal case.
  *   orc_pt_exchange_*  the acceptance formula monte_carlo.jl:326-331 checked
  *                      against detailed balance on a two-level toy.
  *   orc_equal_time_    closed forms: a ferromagnet along z gives S^zz(0) = N S^2
@@ -700,6 +704,74 @@ long orc_metropolis_coloured(const orc_lat *L, double *spins, const long *order,
         if (dE_out) dE_out[k] = dE;
         acc += a;
     }
+    return acc;
+}
+
+/* metropolis_constraint_adaptive! (metropolis.jl:155-179) over the sites of
+ * `order` (1-based, class after class), for the site-sum constraint
+ *     c(lattice) = sum_i (a_b . s_i + s_i . Q_b s_i) - c0,  b = (i-1) / ncell,
+ * a[cnb][3], Q[cnb][9] row-major (bases >= cnb contribute 0).  Per proposal,
+ * literally as the reference: the cone move with width *sigma and the Philox
+ * draws of orc_metropolis_coloured, dE = energy(new) - energy(old), c
+ * evaluated over the whole lattice with the new spin in place (:164),
+ * new_lambda = lambda - weight * c (:165), acceptance on
+ * dE - new_lambda * c (:167), restore on rejection, lambda = new_lambda on
+ * acceptance (:172).  After the n proposals the width law (:176-177):
+ * f = 0.5 / (1 - rate); sigma = rate < 0.5 ? sigma * f : sigma0.
+ * decisions[k] (optional) receives the decision for order[k].  Returns the
+ * accepted count. */
+static double orc_constraint_value(const orc_lat *L, const double *spins, long ncell, const double *ca,
+                                   const double *cQ, int cnb, double c0)
+{
+    double c = 0.0;
+    for (long i = 0; i < L->N; ++i) {
+        long b = i / ncell;
+        if (b >= cnb) continue;
+        const double *s = &spins[3 * i], *q = &cQ[9 * b], *a = &ca[3 * b];
+        double lin = a[0] * s[0] + a[1] * s[1] + a[2] * s[2];
+        double quad = 0.0;
+        for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) quad += s[x] * q[3 * x + y] * s[y];
+        c += lin + quad;
+    }
+    return c - c0;
+}
+
+long orc_metropolis_constraint_coloured(const orc_lat *L, double *spins, const long *order, long n,
+                                        double S, double T, double *sigma, double sigma0, uint64_t seed,
+                                        uint32_t stream, uint32_t sweep_ctr, long ncell, const double *ca,
+                                        const double *cQ, int cnb, double c0, double weight, double *lambda,
+                                        unsigned char *decisions)
+{
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    long acc = 0;
+    for (long k = 0; k < n; ++k) {
+        long p = order[k];
+        uint32_t ctr[4] = {(uint32_t)(p - 1), stream, sweep_ctr, ORC_TAG_METRO};
+        uint32_t r[4];
+        orc_philox4x32_10(ctr, key, r);
+        double *s = &spins[3 * (p - 1)];
+        double old[3] = {s[0], s[1], s[2]};
+        double E_old = orc_site_energy(L, spins, p);
+        double nw[3];
+        orc_cone_move(S, old, *sigma, orc_u01(r[0]), orc_u01(r[1]), nw);
+        s[0] = nw[0]; s[1] = nw[1]; s[2] = nw[2];
+        double dE = orc_site_energy(L, spins, p) - E_old;
+        double c = orc_constraint_value(L, spins, ncell, ca, cQ, cnb, c0);
+        double new_lambda = *lambda - weight * c;
+        double x = dE - new_lambda * c;
+        int accept = (x < 0) ? 1 : (orc_u01(r[2]) < exp(-x / T));
+        if (!accept) {
+            s[0] = old[0]; s[1] = old[1]; s[2] = old[2];
+        } else {
+            ++acc;
+            *lambda = new_lambda;
+        }
+        if (decisions) decisions[k] = (unsigned char)accept;
+    }
+    double rate = (double)acc / (double)n;
+    double f = 0.5 / (1.0 - rate);
+    *sigma = rate < 0.5 ? *sigma * f : sigma0;
     return acc;
 }
 

@@ -54,6 +54,7 @@ def lib():
         _lib.orc_binner_std_error.restype = C.c_double
         _lib.orc_binner_size.restype = C.c_size_t
         _lib.orc_pt_reference_run.restype = C.c_long
+        _lib.orc_metropolis_constraint_coloured.restype = C.c_long
     return _lib
 
 
@@ -221,6 +222,24 @@ def metropolis_coloured(T, spins, order, S, temp, kind, sigma, seed, stream, swe
         C.c_int(kind), C.c_double(sigma), C.c_uint64(seed), C.c_uint32(stream), C.c_uint32(sweep_ctr),
         _p(dec, C.POINTER(C.c_ubyte)), _p(dE, _D))
     return a, dec, dE
+
+
+def metropolis_constraint_coloured(T, spins, order, S, temp, sigma, sigma0, seed, stream, sweep_ctr, ncell,
+                                   a, Q, c0, weight, lam):
+    """metropolis_constraint_adaptive! over `order` (orc_metropolis_constraint_coloured):
+    returns (accepted, decisions, sigma after the width law, lambda)"""
+    o = np.ascontiguousarray(np.asarray(order, dtype=np.int_))
+    dec = np.zeros(len(o), dtype=np.uint8)
+    av = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1, 3))
+    qv = np.ascontiguousarray(np.asarray(Q, dtype=np.float64).reshape(-1, 9))
+    sg = C.c_double(sigma)
+    lm = C.c_double(lam)
+    acc = lib().orc_metropolis_constraint_coloured(
+        T.ref, _p(spins, _D), _p(o, _L), C.c_long(len(o)), C.c_double(S), C.c_double(temp), C.byref(sg),
+        C.c_double(sigma0), C.c_uint64(seed), C.c_uint32(stream), C.c_uint32(sweep_ctr), C.c_long(ncell),
+        _p(av, _D), _p(qv, _D), C.c_int(len(av)), C.c_double(c0), C.c_double(weight), C.byref(lm),
+        _p(dec, C.POINTER(C.c_ubyte)))
+    return acc, dec, sg.value, lm.value
 
 
 def metropolis_random(T, spins, S, temp, kind, sigma, seed, stream, sweep_ctr):

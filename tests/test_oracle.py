@@ -489,3 +489,77 @@ def test_cone_move_formula():
     r = orc.random_spin(S, u1, u2)
     v = s + sigma * r
     assert np.allclose(orc.cone_move(S, s, sigma, u1, u2), S * v / np.linalg.norm(v), atol=1e-14)
+
+
+# -- metropolis_constraint_adaptive! (metropolis.jl:155-179) --------------------
+
+def _free_spin():
+    uc = Square()
+    uc.basis.append(np.zeros(2))
+    return orc.Tables(uc, (1, 1))
+
+
+def test_constraint_weight_zero_is_the_cone_chain():
+    # weight 0: new_lambda = lambda = 0, so the acceptance is on dE alone and
+    # the chain is the cone chain (metropolis_fixed_cone!) draw for draw
+    uc = W.honeycomb_kitaev(1.0, -2.0, h=[0.1, 0.0, 0.2])
+    T = orc.Tables(uc, (4, 4))
+    rng = np.random.default_rng(2)
+    s = random_spins(T.N, 0.5, rng)
+    order = np.arange(1, T.N + 1)
+    a, b = s.copy(), s.copy()
+    sigma = 0.7
+    for sw in range(3):
+        na, deca, sg, lam = orc.metropolis_constraint_coloured(
+            T, a, order, 0.5, 0.4, sigma, 60.0, 11, 0, sw, T.N // 2, [[0, 0, 1], [0, 0, 1]],
+            np.zeros((2, 9)), 3.0, 0.0, 0.0)
+        nb, decb, _ = orc.metropolis_coloured(T, b, order, 0.5, 0.4, 1, sigma, 11, 0, sw)
+        assert lam == 0.0 and na == nb and np.array_equal(deca, decb)
+        assert np.array_equal(a, b)
+        # the width law of metropolis.jl:176-177
+        rate = na / T.N
+        assert sg == (sigma * 0.5 / (1 - rate) if rate < 0.5 else 60.0)
+        sigma = sg
+
+
+def test_constraint_multiplier_recurrence():
+    # one free spin (dE = 0), c = s_z - 2 (always negative for S = 1):
+    # new_lambda = lambda - w c > lambda, and x = dE - new_lambda c > 0.
+    # At T = 1e300 every proposal is accepted, so lambda = -w sum_k c_k with
+    # c_k taken with the NEW spin in place (metropolis.jl:164-165); at
+    # T = 1e-300 none is, and lambda stays 0 (it moves only on acceptance,
+    # :172).  The width law gives sigma0 at rate 1 and sigma / 2 at rate 0.
+    T = _free_spin()
+    w, c0 = 0.3, 2.0
+    s = np.array([[0.0, 0.0, 1.0]])
+    lam, expect = 0.0, 0.0
+    for sw in range(6):
+        n, dec, sg, lam = orc.metropolis_constraint_coloured(T, s, [1], 1.0, 1e300, 0.9, 60.0, 5, 0, sw, 1,
+                                                             [[0, 0, 1]], np.zeros((1, 9)), c0, w, lam)
+        assert n == 1 and dec[0] == 1 and sg == 60.0
+        expect -= w * (s[0, 2] - c0)
+        assert abs(lam - expect) < 1e-14
+    s0 = s.copy()
+    n, dec, sg, lam0 = orc.metropolis_constraint_coloured(T, s, [1], 1.0, 1e-300, 0.9, 60.0, 5, 0, 99, 1,
+                                                          [[0, 0, 1]], np.zeros((1, 9)), c0, w, 0.0)
+    assert n == 0 and lam0 == 0.0 and np.array_equal(s, s0) and sg == 0.45
+
+
+def test_constraint_quadratic_term():
+    # the quadratic part s.Q s enters c like the reference's closure would:
+    # with Q = I the term is S^2 per site, a constant, so c is the same for
+    # every proposal and the chain equals the Q = 0 chain with c0 shifted
+    uc = W.honeycomb_kitaev(1.0, -2.0)
+    T = orc.Tables(uc, (3, 3))
+    rng = np.random.default_rng(6)
+    s = random_spins(T.N, 1.0, rng)
+    order = np.arange(1, T.N + 1)
+    a, b = s.copy(), s.copy()
+    eye = np.tile(np.eye(3).reshape(9), (2, 1))
+    ra = orc.metropolis_constraint_coloured(T, a, order, 1.0, 0.3, 0.5, 60.0, 3, 1, 0, 9,
+                                            [[0.2, 0, 1], [0, -0.3, 1]], eye, 5.0 + T.N, 0.2, 0.1)
+    rb = orc.metropolis_constraint_coloured(T, b, order, 1.0, 0.3, 0.5, 60.0, 3, 1, 0, 9,
+                                            [[0.2, 0, 1], [0, -0.3, 1]], np.zeros((2, 9)), 5.0, 0.2, 0.1)
+    assert ra[0] == rb[0] and np.array_equal(ra[1], rb[1])
+    assert abs(ra[3] - rb[3]) < 1e-9 * max(1.0, abs(rb[3]))
+    assert np.allclose(a, b, atol=1e-12)
