@@ -147,6 +147,7 @@ struct PTHost {
     // chained kernel in storage mode 0: parts0 workgroups per system (1: one),
     // their barriers ([R][2]) and workgroup sums ([R][2][parts0][8])
     int parts0 = 1;
+    int parts_lm = -1;         // the storage mode parts0 was chosen for
     bool parts_ready = false;  // parts0 chosen (at csmc_pt_init or the first
                                // persistent launch in storage mode 0)
     DevBuf<uint32_t> bar0, err0;
@@ -746,7 +747,8 @@ int setup_parts0(csmc_ctx *c, Batch *b) {
     pt.parts_ready = true;
     const int R = b->nsys;
     size_t bytes = 0;
-    if (choose_lm(c, b, &bytes) != 0) return CSMC_OK;  // spins in LDS: no parts
+    pt.parts_lm = choose_lm(c, b, &bytes);
+    if (pt.parts_lm != 0) return CSMC_OK;  // spins in LDS: no parts
     int G = c->parts0;
     if (G == 0) {
         int per_colour = 0;  // the largest colour class of the batch's lattices
@@ -831,7 +833,9 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
     } else {
         PTHost &pt = b->pt;
         bool parts_done = false;
-        if (lm == 0 && !pt.parts_ready) {
+        // chosen when the batch first runs in storage mode 0, also when an
+        // earlier choice was made for an LDS mode (the autotuner's caps)
+        if (lm == 0 && (!pt.parts_ready || pt.parts_lm != 0)) {
             const int rc = setup_parts0(c, b);
             if (rc) return rc;
         }
@@ -1841,7 +1845,9 @@ int32_t csmc_pt_split_parts(csmc_ctx *c, int32_t id, int32_t *parts) {
     CHECK_CTX(c);
     GET_BATCH(c, id, b);
     if (!b->pt.init) return fail(CSMC_ESTATE, "csmc_pt_init was not called for this batch");
-    if (parts) *parts = std::max(b->pt.split_G, b->pt.parts0);
+    size_t bytes = 0;
+    const int lm = choose_lm(c, b, &bytes);  // parts0 applies in storage mode 0 only
+    if (parts) *parts = std::max(b->pt.split_G, lm == 0 ? b->pt.parts0 : 1);
     return CSMC_OK;
 }
 
