@@ -150,7 +150,7 @@ struct PTHost {
     int parts_lm = -1;         // the storage mode parts0 was chosen for
     bool parts_ready = false;  // parts0 chosen (at csmc_pt_init or the first
                                // persistent launch in storage mode 0)
-    DevBuf<uint32_t> bar0, err0;
+    DevBuf<uint32_t> bar0, err0, xcc0;
     DevBuf<double> psum0;
     // equal-time correlations at every probe (corr = true): the sum over
     // probes per temperature slot, [R][corr_nk][9], and the probe count
@@ -489,6 +489,8 @@ void append_lattice(csmc_ctx *c, const HostLattice &L) {
     {
         const std::vector<int32_t> pat = tensor_patterns(L, c->dense_tensors);
         I.insert(I.end(), pat.begin(), pat.end());
+        for (int32_t v : pat)
+            if (v == TP_DENSE) d.flags |= LAT_DENSE_T;
         if (c->verbose) {
             int n3[5] = {0}, n4[5] = {0};
             for (size_t m = 0; m < pat.size(); ++m) ++(m < (size_t)L.nb * L.K3 ? n3 : n4)[pat[m]];
@@ -767,6 +769,7 @@ int setup_parts0(csmc_ctx *c, Batch *b) {
     HIPCHK(pt.bar0.alloc(2 * (size_t)R));
     HIPCHK(pt.err0.alloc(1));
     HIPCHK(pt.psum0.alloc((size_t)R * 2 * G * 8));
+    HIPCHK(pt.xcc0.alloc((size_t)R * G));
     HIPCHK(hipMemsetAsync(pt.bar0.p, 0, 2 * (size_t)R * 4, c->stream));
     HIPCHK(hipMemsetAsync(pt.err0.p, 0, 4, c->stream));
     pt.parts0 = G;
@@ -833,6 +836,8 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
     } else {
         PTHost &pt = b->pt;
         bool parts_done = false;
+        bool sparse = true;  // no slot of the batch's lattices is contracted in full
+        for (int id : b->lat) sparse = sparse && !(c->h_lat[id].flags & LAT_DENSE_T);
         // chosen when the batch first runs in storage mode 0, also when an
         // earlier choice was made for an LDS mode (the autotuner's caps)
         if (lm == 0 && (!pt.parts_ready || pt.parts_lm != 0)) {
@@ -846,11 +851,12 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
             s.parts0 = pt.parts0;
             s.bar0 = pt.bar0.p;
             s.psum0 = pt.psum0.p;
+            s.xcc0 = pt.xcc0.p;
             // PT waits report through the PT state's flag, checked after
             // every csmc_pt_advance; sweeps and annealing through err0
             s.err0 = (a.mode == WG_PT || a.mode == WG_PT_CHAIN) ? pt.err.p : pt.err0.p;
             s.spin_limit = c->spin_limit;  // only the PT paths set it
-            hipError_t e = launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0,
+            hipError_t e = launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0, sparse,
                                      b->nsys * pt.parts0, lds_bytes, s, c->stream);
             if (e == hipSuccess) {
                 parts_done = true;
@@ -860,7 +866,7 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
             }
         }
         if (!parts_done)
-            HIPCHK(launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0, b->nsys,
+            HIPCHK(launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0, sparse, b->nsys,
                              lds_bytes, a, c->stream));
     }
     if (c->timing) {
@@ -1089,6 +1095,8 @@ int32_t csmc_lattice_update_couplings(csmc_ctx *c, int32_t id, const csmc_unit_c
     c->h_lat[id].flags = coupling_flags(L) | (gradient_is_local_field(L) ? LAT_GRAD_LF : 0) |
                          (c->h_lat[id].flags & (LAT_COLBASIS | LAT_TWO_COLOUR | LAT_SELF_Q4));
     const std::vector<int32_t> pat = tensor_patterns(L, c->dense_tensors);
+    for (int32_t v : pat)
+        if (v == TP_DENSE) c->h_lat[id].flags |= LAT_DENSE_T;
     std::copy(pat.begin(), pat.end(), c->h_i32.begin() + c->h_lat[id].pat_off);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->dirty = true;

@@ -107,6 +107,8 @@ constexpr int32_t LAT_GRAD_LF = 32;
 // some quartic slot names the site itself, and no bilinear or cubic slot
 // does (a Metropolis step then evaluates only the quartic part twice)
 constexpr int32_t LAT_SELF_Q4 = 64;
+// some cubic or quartic slot is contracted in full (a TP_DENSE pattern)
+constexpr int32_t LAT_DENSE_T = 128;
 
 // Sparsity pattern of one (basis, slot) cubic or quartic coupling tensor,
 // indices p (the site's own component), q, r, t (the partners in slot

@@ -312,14 +312,35 @@ __device__ __forceinline__ void quartic_term(const R *Q, const A sj[3], const A 
     }
 }
 
+// The coefficient base address made opaque to the optimiser at each call:
+// loads through it cannot be hoisted out of the caller's site loop.  With
+// cubic and quartic slots, hoisting every slot's coefficients into scalar
+// registers for a whole run of sites takes hundreds of SGPRs (spilled to
+// vector registers and then to scratch); reloading a slot's few kept
+// coefficients per site with scalar loads is cheaper.  Emits no instruction.
+template <class T>
+__device__ __forceinline__ const T *per_site_coef(const T *p) {
+    const uint64_t v = (uint64_t)(size_t)p;
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    asm volatile("" : "+s"(lo), "+s"(hi));
+    return (const T *)(size_t)(((uint64_t)hi << 32) | lo);
+}
+
 // K2C > 0: the lattice has exactly K2C bilinear slots (compile-time), so
 // the slot loop is fully unrolled with no padding work.
 // PART 0: the whole field; 1: its bilinear and cubic part; 2: add the
 // quartic part to F (0 is 1 followed by 2, with the same operations).
-template <class R, class A, bool GEN = true, int K2C = 0, int PART = 0, class NB, class LD>
+// SP: every cubic and quartic slot of the lattice has a sparse pattern (no
+// TP_DENSE slot), so the full contractions are not compiled in: they would
+// hold 27 or 81 coefficients per slot in registers.
+template <class R, class A, bool GEN = true, int K2C = 0, int PART = 0, bool SP = false, class NB,
+          class LD>
 __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB &nb, int b, int i,
                                                   LD &&ld, A F[3]) {
     if (PART != 2) F[0] = F[1] = F[2] = A(0);
+    // lattices with cubic or quartic terms reload coefficients per site
+    const R *const c2 = GEN ? per_site_coef(L.c2) : L.c2;
     if constexpr (K2C > 0 && PART != 2) {
         int j[K2C];
 #pragma unroll
@@ -338,7 +359,7 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
         for (int u = 0; u < 4; ++u) sj[u] = ld(j[u]);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const R *M = L.c2 + (b * L.K2P + k0 + u) * 9;
+            const R *M = c2 + (b * L.K2P + k0 + u) * 9;
             F[0] += (A)ldc(M + 0) * sj[u].x + (A)ldc(M + 1) * sj[u].y + (A)ldc(M + 2) * sj[u].z;
             F[1] += (A)ldc(M + 3) * sj[u].x + (A)ldc(M + 4) * sj[u].y + (A)ldc(M + 5) * sj[u].z;
             F[2] += (A)ldc(M + 6) * sj[u].x + (A)ldc(M + 7) * sj[u].y + (A)ldc(M + 8) * sj[u].z;
@@ -370,8 +391,8 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
         for (int u = 0; u < GS; ++u) {
             if (pat[u] == TP_ZERO) continue;
             const A sj[3] = {sp[u][0].x, sp[u][0].y, sp[u][0].z}, sk[3] = {sp[u][1].x, sp[u][1].y, sp[u][1].z};
-            const R *C = L.c3 + (b * L.K3 + k0 + u) * 27;
-            if (pat[u] == TP_C3_ALT)
+            const R *C = per_site_coef(L.c3) + (b * L.K3 + k0 + u) * 27;
+            if (SP || pat[u] == TP_C3_ALT)
                 cubic_term<TP_C3_ALT, R>(C, sj, sk, F);
             else
                 cubic_term<TP_DENSE, R>(C, sj, sk, F);
@@ -398,12 +419,12 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
             if (pat[u] == TP_ZERO) continue;
             const A sj[3] = {sp[u][0].x, sp[u][0].y, sp[u][0].z}, sk[3] = {sp[u][1].x, sp[u][1].y, sp[u][1].z},
                     sl[3] = {sp[u][2].x, sp[u][2].y, sp[u][2].z};
-            const R *Q = L.c4 + (b * L.K4 + k0 + u) * 81;
+            const R *Q = per_site_coef(L.c4) + (b * L.K4 + k0 + u) * 81;
             if (pat[u] == TP_Q4_PQ)
                 quartic_term<TP_Q4_PQ, R>(Q, sj, sk, sl, F);
             else if (pat[u] == TP_Q4_PR)
                 quartic_term<TP_Q4_PR, R>(Q, sj, sk, sl, F);
-            else if (pat[u] == TP_Q4_PT)
+            else if (SP || pat[u] == TP_Q4_PT)
                 quartic_term<TP_Q4_PT, R>(Q, sj, sk, sl, F);
             else
                 quartic_term<TP_DENSE, R>(Q, sj, sk, sl, F);
@@ -475,22 +496,22 @@ __device__ __forceinline__ bool det_align(const LatView<R> &L, const R H[3], spi
     return true;
 }
 
-template <class R, bool GEN, int K2C, class NB, class SV>
+template <class R, bool GEN, int K2C, bool SP = false, class NB, class SV>
 __device__ __forceinline__ void or_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
                                           int i) {
     spin4<R> s = X.ld(i), n;
     R F[3], H[3];
-    interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
+    interaction_field<R, R, GEN, K2C, 0, SP>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
     local_field<R, R>(L, b, s, F, H);
     if (or_reflect<R>(s, H, n)) X.st(i, n);
 }
 
-template <class R, bool GEN, int K2C, class NB, class SV>
+template <class R, bool GEN, int K2C, bool SP = false, class NB, class SV>
 __device__ __forceinline__ void det_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
                                            int i) {
     spin4<R> s = X.ld(i), n;
     R F[3], H[3];
-    interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
+    interaction_field<R, R, GEN, K2C, 0, SP>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
     local_field<R, R>(L, b, s, F, H);
     if (det_align<R>(L, H, n)) X.st(i, n);
 }
@@ -564,13 +585,13 @@ __device__ __forceinline__ void metro_commit(const LatView<R> &L, const SV &X, I
 
 // energy(new) - energy(old) of the proposal n for the spin s at site i
 // (calculate_energy_diff!, metropolis.jl:94-110)
-template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
+template <class R, bool SELF, bool GEN, int K2C, bool SP = false, class NB, class SV>
 __device__ __forceinline__ R metro_eval(const LatView<R> &L, const NB &nb, const SV &X, int b, int i,
                                         const spin4<R> &s, const spin4<R> &n) {
     R dE;
     if (!SELF) {
         R F[3];
-        interaction_field<R, R, GEN, K2C>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
+        interaction_field<R, R, GEN, K2C, 0, SP>(L, nb, b, i, [&](int j) { return X.ld(j); }, F);
         dE = metro_dE<R>(L, b, s, n, F);
     } else {
         const R *hp = L.field + b * 3;
@@ -581,15 +602,15 @@ __device__ __forceinline__ R metro_eval(const LatView<R> &L, const NB &nb, const
         if (GEN && (L.flags & LAT_SELF_Q4)) {
             // only quartic slots name the site: the bilinear and cubic part
             // is the same for both spins, the quartic part is added twice
-            interaction_field<R, R, GEN, K2C, 1>(L, nb, b, i, ldo, Fo);
+            interaction_field<R, R, GEN, K2C, 1, SP>(L, nb, b, i, ldo, Fo);
             Fn[0] = Fo[0];
             Fn[1] = Fo[1];
             Fn[2] = Fo[2];
-            interaction_field<R, R, GEN, K2C, 2>(L, nb, b, i, ldo, Fo);
-            interaction_field<R, R, GEN, K2C, 2>(L, nb, b, i, ldn, Fn);
+            interaction_field<R, R, GEN, K2C, 2, SP>(L, nb, b, i, ldo, Fo);
+            interaction_field<R, R, GEN, K2C, 2, SP>(L, nb, b, i, ldn, Fn);
         } else {
-            interaction_field<R, R, GEN, K2C>(L, nb, b, i, ldo, Fo);
-            interaction_field<R, R, GEN, K2C>(L, nb, b, i, ldn, Fn);
+            interaction_field<R, R, GEN, K2C, 0, SP>(L, nb, b, i, ldo, Fo);
+            interaction_field<R, R, GEN, K2C, 0, SP>(L, nb, b, i, ldn, Fn);
         }
         R eo = onsite_energy<R, R>(L, b, s.x, s.y, s.z) + s.x * Fo[0] + s.y * Fo[1] + s.z * Fo[2] -
                (s.x * h[0] + s.y * h[1] + s.z * h[2]);
@@ -600,14 +621,14 @@ __device__ __forceinline__ R metro_eval(const LatView<R> &L, const NB &nb, const
     return dE;
 }
 
-template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
+template <class R, bool SELF, bool GEN, int K2C, bool SP = false, class NB, class SV>
 __device__ __forceinline__ int metro_update(const LatView<R> &L, const NB &nb, const SV &X, int b,
                                             int i, R T, R invT, int kind, R sigma, uint32_t k0,
                                             uint32_t k1, uint32_t stream, uint32_t ctr) {
     spin4<R> s = X.ld(i);
     uint4 r = philox4x32_10(make_uint4((uint32_t)L.ref[i], stream, ctr, TAG_METRO), k0, k1);
     spin4<R> n = metro_propose<R>(L, s, r, kind, sigma);
-    const R dE = metro_eval<R, SELF, GEN, K2C>(L, nb, X, b, i, s, n);
+    const R dE = metro_eval<R, SELF, GEN, K2C, SP>(L, nb, X, b, i, s, n);
     int acc = metro_accept<R>(dE, r, T, invT);
     metro_commit<R>(L, X, i, acc, s, n);
     return acc;

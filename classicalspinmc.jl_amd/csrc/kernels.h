@@ -79,6 +79,7 @@ struct WGArgs {
     uint32_t *bar0;
     double *psum0;
     uint32_t *err0;  // set when a barrier wait of the parts gives up
+    uint32_t *xcc0;  // [nsys][parts0]: the XCD each part runs on, written at launch
 };
 
 // Arguments of the one-colour-pass kernel (plain / stream form).
@@ -136,7 +137,10 @@ hipError_t launch_cooperative(const void *fn, int nblocks, int threads, size_t l
 // tables); self / gen: some lattice of the batch has a self-referencing
 // term / cubic or quartic terms; k2c: the bilinear slot count shared by
 // every lattice of the batch (0 if they differ)
-hipError_t launch_wg(int prec, int lm, bool self, bool gen, int k2c, int nsys, size_t lds_bytes,
+// sparse: no cubic or quartic slot of the batch's lattices is contracted in
+// full (TP_DENSE), so the FP32 general kernels may leave the full
+// contractions out
+hipError_t launch_wg(int prec, int lm, bool self, bool gen, int k2c, bool sparse, int nsys, size_t lds_bytes,
                      const WGArgs &a, hipStream_t st);
 // register-table mode (FP32, bilinear-only): k2c in {3, 4, 6}, spt in {4, 8}
 // register slots per thread (8 only with k2c <= 4), diag: every coupling

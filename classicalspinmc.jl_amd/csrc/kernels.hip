@@ -25,7 +25,7 @@ namespace csmc {
 
 // interaction field split by term kind (bilinear, cubic, quartic), FP64,
 // for energies and gradients
-template <bool GEN, class NB, class LD>
+template <bool GEN, bool SP = false, class NB, class LD>
 __device__ __forceinline__ void field_parts(const LatView<double> &L, const NB &nb, int b, int i,
                                             LD &&ld, double F2[3], double F3[3], double F4[3]) {
     interaction_field<double, double, false>(L, nb, b, i, ld, F2);
@@ -34,11 +34,11 @@ __device__ __forceinline__ void field_parts(const LatView<double> &L, const NB &
     LatView<double> Lc = L;
     Lc.K2 = Lc.K2P = 0;
     Lc.K4 = 0;
-    interaction_field<double, double>(Lc, nb, b, i, ld, F3);
+    interaction_field<double, double, true, 0, 0, SP>(Lc, nb, b, i, ld, F3);
     LatView<double> Lq = L;
     Lq.K2 = Lq.K2P = 0;
     Lq.K3 = 0;
-    interaction_field<double, double>(Lq, nb, b, i, ld, F4);
+    interaction_field<double, double, true, 0, 0, SP>(Lq, nb, b, i, ld, F4);
 }
 
 // ---------------------------------------------------------------------------
@@ -62,6 +62,23 @@ __device__ __forceinline__ void wg_colour_sweep(const LatView<R> &L, OP &&op) {
 // and a sum over the system is the parts' workgroup sums added in part order,
 // so every part holds the same totals.  G = 1 is the whole-system workgroup:
 // a workgroup barrier, and the workgroup sum unchanged.
+// Ordering among the CUs of one XCD.  The CUs of an XCD share its L2, and
+// the vector L1 of a CU is write-through, so a store is visible to every CU
+// of the XCD once it has completed (vmcnt) and the reader's L1 no longer
+// holds the line: the code sequences of a workgroup-scope release and
+// acquire when the waves of a workgroup may sit on different CUs of one XCD.
+// An agent-scope fence would also write back (release) and invalidate
+// (acquire) the L2 for the other XCDs.
+__device__ __forceinline__ void xcd_release() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ __forceinline__ void xcd_acquire() {
+    asm volatile("s_waitcnt vmcnt(0)\n\tbuffer_inv sc0" ::: "memory");
+}
+
+// the XCD (XCC) this workgroup runs on: HW_REG_XCC_ID[3:0]
+__device__ __forceinline__ uint32_t xcc_id() {
+    return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | 20) & 0xFu;
+}
+
 struct Parts {
     int part, G, sys;
     uint32_t *bar;    // [nsys][2]: arrivals, generation
@@ -69,10 +86,13 @@ struct Parts {
     uint32_t *err;    // set when a wait gives up
     uint32_t limit;   // polls of a wait
     int buf;          // half of psum the next sum writes
+    bool xlocal;      // every part of the system runs on one XCD (found at launch)
 
     // every thread; the other parts' global stores before their barrier
-    // call are visible after it
+    // call are visible after it.  With every part on one XCD the fences are
+    // the XCD ones; otherwise agent-scope fences.
     __device__ void barrier() {
+        if (G > 1 && xlocal) xcd_release();  // each wave: its stores are in L2
         __syncthreads();
         if (G == 1) return;
         if (threadIdx.x == 0) {
@@ -81,15 +101,25 @@ struct Parts {
             // release this part's stores (and order the load of g0 before
             // the arrival): a release fence writes L2 back, it does not
             // invalidate it
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (xlocal)
+                xcd_release();
+            else
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             const uint32_t before = atomicAdd(cnt, 1u);
             if (before == (uint32_t)G - 1) {
                 // the last to arrive read every arrival: acquire what the
                 // others released, then open the next barrier and this one
                 // (the release passes their stores on to the waiters)
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                if (xlocal) {
+                    xcd_acquire();
+                    __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    xcd_release();  // the reset reaches L2 before the generation moves
+                    __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
             } else {
                 uint32_t n = 0;
                 while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0) {
@@ -100,10 +130,32 @@ struct Parts {
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the other parts' stores
+                // the other parts' stores
+                if (xlocal)
+                    xcd_acquire();
+                else
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
         }
         __syncthreads();
+    }
+
+    // at launch, every thread of every part: publish this part's XCD, pass
+    // one (agent-scope) barrier, and take the XCD fences from then on if all
+    // parts of the system are on one XCD.  Every part reads the same values,
+    // so all reach the same choice.
+    __device__ void find_xcd(uint32_t *xcc) {
+        xlocal = false;
+        if (G == 1) return;
+        uint32_t *x = xcc + (int64_t)sys * G;
+        if (threadIdx.x == 0)
+            __hip_atomic_store(x + part, xcc_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        barrier();
+        bool same = true;
+        const uint32_t x0 = __hip_atomic_load(x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int p = 1; p < G; ++p)
+            same = same && __hip_atomic_load(x + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == x0;
+        xlocal = same;
     }
 
     // v: this workgroup's sums (valid in every thread, as block_sum leaves
@@ -200,7 +252,7 @@ __device__ void wg_magnet(const LatView<double> &Ld, const SV &X, double *red, d
 // this thread's FP64 energy (the parts combined as hamiltonian.jl:131) and
 // summed spin over the same sites, in the same order, as wg_energy and
 // wg_magnet
-template <class R, bool GEN, class NB, class SV>
+template <class R, bool GEN, bool SP, class NB, class SV>
 __device__ void thread_energy(const LatView<double> &Ld, const NB &nb, const SV &X, double &e,
                               double m[3], int part = 0, int G = 1) {
     double p[5] = {0, 0, 0, 0, 0};
@@ -213,7 +265,7 @@ __device__ void thread_energy(const LatView<double> &Ld, const NB &nb, const SV 
         for (int i = run.x + first; i < run.y; i += step) {
             spin4<double> s = to_d<R>(X.ld(i));
             double F2[3], F3[3], F4[3];
-            field_parts<GEN>(Ld, nb, b, i, [&](int j) { return to_d<R>(X.ld(j)); }, F2, F3, F4);
+            field_parts<GEN, SP>(Ld, nb, b, i, [&](int j) { return to_d<R>(X.ld(j)); }, F2, F3, F4);
             p[0] += s.x * F2[0] + s.y * F2[1] + s.z * F2[2];
             p[1] += s.x * F3[0] + s.y * F3[1] + s.z * F3[2];
             p[2] += s.x * F4[0] + s.y * F4[1] + s.z * F4[2];
@@ -228,24 +280,24 @@ __device__ void thread_energy(const LatView<double> &Ld, const NB &nb, const SV 
 }
 
 // one Metropolis colour sweep; returns this thread's accepted count
-template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
+template <class R, bool SELF, bool GEN, int K2C, bool SP, class NB, class SV>
 __device__ int wg_metro_local(const LatView<R> &L, const NB &nb, const SV &X, R T, int kind, R sigma,
                               uint32_t k0, uint32_t k1, uint32_t stream, uint32_t ctr, Parts &pp) {
     int acc = 0;
     const R invT = R(1) / T;
     wg_colour_sweep_p<R>(L, pp, [&](int b, int i) {
-        acc += metro_update<R, SELF, GEN, K2C>(L, nb, X, b, i, T, invT, kind, sigma, k0, k1, stream,
-                                               ctr);
+        acc += metro_update<R, SELF, GEN, K2C, SP>(L, nb, X, b, i, T, invT, kind, sigma, k0, k1, stream,
+                                                   ctr);
     });
     return acc;
 }
 
-template <class R, bool SELF, bool GEN, int K2C, class NB, class SV>
+template <class R, bool SELF, bool GEN, int K2C, bool SP, class NB, class SV>
 __device__ double wg_metro(const LatView<R> &L, const NB &nb, const SV &X, R T, int kind, R sigma,
                            uint32_t k0, uint32_t k1, uint32_t stream, uint32_t ctr, double *red,
                            Parts &pp) {
-    double v[1] = {(double)wg_metro_local<R, SELF, GEN, K2C>(L, nb, X, T, kind, sigma, k0, k1, stream,
-                                                             ctr, pp)};
+    double v[1] = {(double)wg_metro_local<R, SELF, GEN, K2C, SP>(L, nb, X, T, kind, sigma, k0, k1, stream,
+                                                                 ctr, pp)};
     block_sum<1>(v, red);
     pp.sum<1>(v);
     return v[0];
@@ -254,7 +306,7 @@ __device__ double wg_metro(const LatView<R> &L, const NB &nb, const SV &X, R T, 
 // The sweeper of the global / LDS-array storage modes: sweeps walk the runs
 // of each colour class with one thread per site (wg_colour_sweep_p), over
 // the sites of this workgroup's part of the system.
-template <class R, bool SELF, bool GEN, int K2C, class SV, class NB>
+template <class R, bool SELF, bool GEN, int K2C, bool SP, class SV, class NB>
 struct SweeperG {
     const LatView<R> &L;
     const LatView<double> &Ld;
@@ -267,20 +319,20 @@ struct SweeperG {
     __device__ int n_sites() const { return L.N; }
     __device__ bool leader() const { return pp.part == 0; }
     __device__ void or_sweep() {
-        wg_colour_sweep_p<R>(L, pp, [&](int b, int i) { or_update<R, GEN, K2C>(L, nb, X, b, i); });
+        wg_colour_sweep_p<R>(L, pp, [&](int b, int i) { or_update<R, GEN, K2C, SP>(L, nb, X, b, i); });
     }
     __device__ void det_sweep() {
-        wg_colour_sweep_p<R>(L, pp, [&](int b, int i) { det_update<R, GEN, K2C>(L, nb, X, b, i); });
+        wg_colour_sweep_p<R>(L, pp, [&](int b, int i) { det_update<R, GEN, K2C, SP>(L, nb, X, b, i); });
     }
     __device__ double metro(double T, int kind, double sigma, uint32_t ctr) {
-        return wg_metro<R, SELF, GEN, K2C>(L, nb, X, (R)T, kind, (R)sigma, k0, k1, stream, ctr, red, pp);
+        return wg_metro<R, SELF, GEN, K2C, SP>(L, nb, X, (R)T, kind, (R)sigma, k0, k1, stream, ctr, red, pp);
     }
     __device__ double metro_energy(double T, int kind, double sigma, uint32_t ctr, bool with_m,
                                    double &E, double m[3]) {
-        const int acc = wg_metro_local<R, SELF, GEN, K2C>(L, nb, X, (R)T, kind, (R)sigma, k0, k1,
-                                                          stream, ctr, pp);
+        const int acc = wg_metro_local<R, SELF, GEN, K2C, SP>(L, nb, X, (R)T, kind, (R)sigma, k0, k1,
+                                                              stream, ctr, pp);
         double e, mm[3];
-        thread_energy<R, GEN>(Ld, nb, X, e, mm, pp.part, pp.G);
+        thread_energy<R, GEN, SP>(Ld, nb, X, e, mm, pp.part, pp.G);
         if (with_m) {
             double v[5] = {(double)acc, e, mm[0], mm[1], mm[2]};
             block_sum<5>(v, red);
@@ -303,12 +355,12 @@ struct SweeperG {
     }
 };
 
-template <int MODE, class R, bool SELF, bool GEN, int K2C, class SV, class NB>
+template <int MODE, class R, bool SELF, bool GEN, int K2C, bool SP, class SV, class NB>
 __device__ __forceinline__ void wg_run(const WGArgs &a, int sys, const LatView<R> &L,
                                        const LatView<double> &Ld, const SV &X, const NB &nb,
                                        double *red, const Parts &pp) {
     const uint32_t stream = (uint32_t)a.P.sys[sys].stream;
-    SweeperG<R, SELF, GEN, K2C, SV, NB> sw{L, Ld, X, nb, red, stream, a.k0, a.k1, pp};
+    SweeperG<R, SELF, GEN, K2C, SP, SV, NB> sw{L, Ld, X, nb, red, stream, a.k0, a.k1, pp};
     wg_drive<MODE>(a, sys, sw);
 }
 
@@ -320,8 +372,9 @@ __device__ __forceinline__ size_t align16(size_t v) { return (v + 15) & ~(size_t
 // launched together: the general variant is exact for every lattice, the
 // fast one for bilinear-only lattices without self references.  K2C > 0
 // specialises the bilinear slot count (all lattices of the batch have it).
+// SP: no cubic or quartic slot of the batch is contracted in full.
 // ---------------------------------------------------------------------------
-template <class R, int LM, bool SELF, bool GEN, int MODE, int K2C>
+template <class R, int LM, bool SELF, bool GEN, int MODE, int K2C, bool SP>
 __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     double *red = reinterpret_cast<double *>(smem);  // reduction scratch
@@ -337,7 +390,8 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
         sys = x + 8 * (k / nparts);
         part = k % nparts;
     }
-    const Parts pp{part, nparts, sys, a.bar0, a.psum0, a.err0, a.spin_limit, 0};
+    Parts pp{part, nparts, sys, a.bar0, a.psum0, a.err0, a.spin_limit, 0, false};
+    if (nparts > 1) pp.find_xcd(a.xcc0);
     const SysDev sd = a.P.sys[sys];
     const LatDev ld = a.P.lats[sd.lat];
     const int4 *runs = a.P.runs + ld.run_off;
@@ -347,7 +401,7 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
     const int N = L.N;
 
     if constexpr (LM == 0) {
-        wg_run<MODE, R, SELF, GEN, K2C>(a, sys, L, Ld, SpinsAoS<R>{G}, NbrG{L.nbr2, N}, red, pp);
+        wg_run<MODE, R, SELF, GEN, K2C, SP>(a, sys, L, Ld, SpinsAoS<R>{G}, NbrG{L.nbr2, N}, red, pp);
     } else {
         // N + 1 entries per component: the last one is the zero pad site
         R *sx = reinterpret_cast<R *>(smem + RED_BYTES);
@@ -355,7 +409,7 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
         wg_stage<8>(N + 1, [&](int i) { return G[i]; }, [&](int i, const spin4<R> &v) { X.st(i, v); });
         if constexpr (LM == 1) {
             __syncthreads();
-            wg_run<MODE, R, SELF, GEN, K2C>(a, sys, L, Ld, X, NbrG{L.nbr2, N}, red, pp);
+            wg_run<MODE, R, SELF, GEN, K2C, SP>(a, sys, L, Ld, X, NbrG{L.nbr2, N}, red, pp);
         } else {
             uint16_t *idx =
                 reinterpret_cast<uint16_t *>(smem + RED_BYTES + align16(3 * sizeof(R) * (N + 1)));
@@ -363,7 +417,7 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
             wg_stage<8>(L.K2 * N, [&](int t) { return nbr2[t]; },
                         [&](int t, int32_t v) { idx[t] = (uint16_t)v; });
             __syncthreads();
-            wg_run<MODE, R, SELF, GEN, K2C>(a, sys, L, Ld, X, NbrL{idx, N, L.K2}, red, pp);
+            wg_run<MODE, R, SELF, GEN, K2C, SP>(a, sys, L, Ld, X, NbrL{idx, N, L.K2}, red, pp);
         }
     }
     // the exchange goes before the LDS modes store the spins back, so that
@@ -413,10 +467,10 @@ hipError_t launch_cooperative(const void *fn, int nblocks, int threads, size_t l
     return hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(threads), args, (unsigned)lds_bytes, st);
 }
 
-template <class R, int LM, bool GENERAL, int MODE, int K2C>
+template <class R, int LM, bool GENERAL, int MODE, int K2C, bool SP>
 static hipError_t launch_wg_t(int nsys, size_t lds_bytes, const WGArgs &a, hipStream_t st) {
     size_t bytes = RED_BYTES + (LM ? lds_bytes : 0);
-    const void *fn = (const void *)k_wg<R, LM, GENERAL, GENERAL, MODE, K2C>;
+    const void *fn = (const void *)k_wg<R, LM, GENERAL, GENERAL, MODE, K2C, SP>;
     if (bytes > 65536) {
         hipError_t e = allow_large_lds(fn);
         if (e != hipSuccess) return e;
@@ -426,53 +480,57 @@ static hipError_t launch_wg_t(int nsys, size_t lds_bytes, const WGArgs &a, hipSt
     if constexpr (MODE == WG_PT_CHAIN) return launch_cooperative(fn, nsys, threads, bytes, a, st);
     if constexpr (LM == 0)
         if (a.parts0 > 1) return launch_cooperative(fn, nsys, threads, bytes, a, st);
-    hipLaunchKernelGGL((k_wg<R, LM, GENERAL, GENERAL, MODE, K2C>), dim3(nsys), dim3(threads), bytes,
+    hipLaunchKernelGGL((k_wg<R, LM, GENERAL, GENERAL, MODE, K2C, SP>), dim3(nsys), dim3(threads), bytes,
                        st, a);
     return hipGetLastError();
 }
 
 template <class R, int LM, bool GENERAL, int MODE>
-static hipError_t launch_wg_k(int k2c, int nsys, size_t lds_bytes, const WGArgs &a,
+static hipError_t launch_wg_k(int k2c, bool sparse, int nsys, size_t lds_bytes, const WGArgs &a,
                               hipStream_t st) {
     // slot-count specialisations exist for the FP32 fast path in LDS modes
     if constexpr (sizeof(R) == 4 && !GENERAL && LM > 0) {
-        if (k2c == 3) return launch_wg_t<R, LM, GENERAL, MODE, 3>(nsys, lds_bytes, a, st);
-        if (k2c == 4) return launch_wg_t<R, LM, GENERAL, MODE, 4>(nsys, lds_bytes, a, st);
-        if (k2c == 6) return launch_wg_t<R, LM, GENERAL, MODE, 6>(nsys, lds_bytes, a, st);
+        if (k2c == 3) return launch_wg_t<R, LM, GENERAL, MODE, 3, false>(nsys, lds_bytes, a, st);
+        if (k2c == 4) return launch_wg_t<R, LM, GENERAL, MODE, 4, false>(nsys, lds_bytes, a, st);
+        if (k2c == 6) return launch_wg_t<R, LM, GENERAL, MODE, 6, false>(nsys, lds_bytes, a, st);
     }
-    return launch_wg_t<R, LM, GENERAL, MODE, 0>(nsys, lds_bytes, a, st);
+    // the sparse-pattern variant of the general FP32 kernels (1024 threads,
+    // where the full contractions would spill)
+    if constexpr (sizeof(R) == 4 && GENERAL)
+        if (sparse) return launch_wg_t<R, LM, GENERAL, MODE, 0, true>(nsys, lds_bytes, a, st);
+    return launch_wg_t<R, LM, GENERAL, MODE, 0, false>(nsys, lds_bytes, a, st);
 }
 
 template <class R, int LM, bool GENERAL>
-static hipError_t launch_wg_g(int mode, int k2c, int nsys, size_t lds_bytes, const WGArgs &a,
+static hipError_t launch_wg_g(int mode, int k2c, bool sparse, int nsys, size_t lds_bytes, const WGArgs &a,
                               hipStream_t st) {
-    if (mode == WG_PT) return launch_wg_k<R, LM, GENERAL, WG_PT>(k2c, nsys, lds_bytes, a, st);
+    if (mode == WG_PT) return launch_wg_k<R, LM, GENERAL, WG_PT>(k2c, sparse, nsys, lds_bytes, a, st);
     if (mode == WG_PT_CHAIN)
-        return launch_wg_k<R, LM, GENERAL, WG_PT_CHAIN>(k2c, nsys, lds_bytes, a, st);
-    if (mode == WG_ANNEAL) return launch_wg_k<R, LM, GENERAL, WG_ANNEAL>(k2c, nsys, lds_bytes, a, st);
-    return launch_wg_k<R, LM, GENERAL, WG_SWEEPS>(k2c, nsys, lds_bytes, a, st);
+        return launch_wg_k<R, LM, GENERAL, WG_PT_CHAIN>(k2c, sparse, nsys, lds_bytes, a, st);
+    if (mode == WG_ANNEAL) return launch_wg_k<R, LM, GENERAL, WG_ANNEAL>(k2c, sparse, nsys, lds_bytes, a, st);
+    return launch_wg_k<R, LM, GENERAL, WG_SWEEPS>(k2c, sparse, nsys, lds_bytes, a, st);
 }
 
 template <class R, int LM>
-static hipError_t launch_wg_m(bool general, int k2c, int nsys, size_t lds_bytes, const WGArgs &a,
-                              hipStream_t st) {
-    return general ? launch_wg_g<R, LM, true>(a.mode, k2c, nsys, lds_bytes, a, st)
-                   : launch_wg_g<R, LM, false>(a.mode, k2c, nsys, lds_bytes, a, st);
+static hipError_t launch_wg_m(bool general, int k2c, bool sparse, int nsys, size_t lds_bytes,
+                              const WGArgs &a, hipStream_t st) {
+    return general ? launch_wg_g<R, LM, true>(a.mode, k2c, sparse, nsys, lds_bytes, a, st)
+                   : launch_wg_g<R, LM, false>(a.mode, k2c, sparse, nsys, lds_bytes, a, st);
 }
 
 template <class R>
-static hipError_t launch_wg_r(int lm, bool general, int k2c, int nsys, size_t lds_bytes,
+static hipError_t launch_wg_r(int lm, bool general, int k2c, bool sparse, int nsys, size_t lds_bytes,
                               const WGArgs &a, hipStream_t st) {
-    if (lm == 2) return launch_wg_m<R, 2>(general, k2c, nsys, lds_bytes, a, st);
-    if (lm == 1) return launch_wg_m<R, 1>(general, k2c, nsys, lds_bytes, a, st);
-    return launch_wg_m<R, 0>(general, k2c, nsys, lds_bytes, a, st);
+    if (lm == 2) return launch_wg_m<R, 2>(general, k2c, sparse, nsys, lds_bytes, a, st);
+    if (lm == 1) return launch_wg_m<R, 1>(general, k2c, sparse, nsys, lds_bytes, a, st);
+    return launch_wg_m<R, 0>(general, k2c, sparse, nsys, lds_bytes, a, st);
 }
 
-hipError_t launch_wg(int prec, int lm, bool self, bool gen, int k2c, int nsys, size_t lds_bytes,
+hipError_t launch_wg(int prec, int lm, bool self, bool gen, int k2c, bool sparse, int nsys, size_t lds_bytes,
                      const WGArgs &a, hipStream_t st) {
     const bool general = self || gen;
-    if (prec == 32) return launch_wg_r<float>(lm, general, k2c, nsys, lds_bytes, a, st);
-    return launch_wg_r<double>(lm, general, k2c, nsys, lds_bytes, a, st);
+    if (prec == 32) return launch_wg_r<float>(lm, general, k2c, sparse, nsys, lds_bytes, a, st);
+    return launch_wg_r<double>(lm, general, k2c, sparse, nsys, lds_bytes, a, st);
 }
 
 // ---------------------------------------------------------------------------
