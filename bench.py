@@ -182,35 +182,39 @@ def finish(out, args):
     print(json.dumps(out))
 
 
-def fp32_error(csmc, W, lat, temps, ctx, R=8, steps=(1, 10)):
-    """Error of the FP32 path against FP64, max over R replicas, relative:
-    - evaluation_energy_rel: the energy of the FP32 run's own state (after
-      the last period) from the FP32 batch against an FP64 batch holding the
-      same spins -- the error of an FP32 result;
-    - trajectory_*: an FP32 and an FP64 batch from the same start with the
-      same schedule and Philox stream, energies after 1 and 10 exchange
-      periods.  Trajectories separate where a Metropolis decision falls
-      differently, so these bound the drift of a run, not an evaluation
-      error."""
-    # t_thermalization = 1: with 0, sweep 0 is a checkpoint sweep and the
-    # first advance stops there
-    p = dict(t_thermalization=1, t_measurement=10 ** 6, probe_rate=10, swap_rate=10, overrelaxation_rate=10,
+def fp32_error(csmc, W, lat, temps, ctx, t_th=2000, t_meas=20000):
+    """Error of the FP32 path against FP64 on the headline lattice and ladder:
+    - evaluation_energy_rel: the FP64 total energy the FP32 batch reports for
+      its own state against an FP64 batch holding the same spins (the
+      reduction is FP64, so this is the error of an FP32 energy result);
+    - per slot, <E> of an FP32 and an FP64 parallel-tempering run of the same
+      schedule (t_th thermalisation + t_meas measurement sweeps, independent
+      seeds) with their binning errors: the largest relative difference
+      |<E>_32 - <E>_64| / |<E>_64|, the combined error bar at that slot, and
+      the largest |z| = |difference| / combined error over the slots."""
+    R = len(temps)
+    p = dict(t_thermalization=t_th, t_measurement=t_meas, probe_rate=10, swap_rate=10, overrelaxation_rate=10,
              report_interval=10 ** 9, checkpoint_rate=10 ** 9)
-    b32 = csmc.SpinBatch([lat] * R, precision=32, ctx=ctx)
-    b64 = csmc.SpinBatch([lat] * R, precision=64, ctx=ctx)
-    b32.randomize(11)
-    for i in range(R):
-        b64.set_spins(i, b32.get_spins(i))
-    t = temps[:: max(1, len(temps) // R)][:R]
-    out = {"start_energy_rel": float(np.max(np.abs(b32.total_energy() - b64.total_energy())
-                                            / np.abs(b64.total_energy())))}
-    for b in (b32, b64):
-        b.pt_init(t, p, csmc.METROPOLIS, seed=4)
-    for k in steps:
-        for b in (b32, b64):
-            assert b.pt_advance(k * 10)[0] == k * 10
-        e32, e64 = b32.total_energy(), b64.total_energy()
-        out[f"trajectory_after_{k}_periods_energy_rel"] = float(np.max(np.abs(e32 - e64) / np.abs(e64)))
+    runs = {}
+    for prec, seed in ((32, 41), (64, 43)):
+        b = csmc.SpinBatch([lat] * R, precision=prec, ctx=ctx)
+        b.randomize(seed)
+        b.pt_init(temps, p, csmc.METROPOLIS, seed=seed)
+        while b.pt_advance(-1)[1] != 3:
+            pass
+        runs[prec] = (b, b.pt_observables())
+    b32, o32 = runs[32]
+    b64, o64 = runs[64]
+    diff = o32["energy"] - o64["energy"]
+    err = np.sqrt(o32["energy_err"] ** 2 + o64["energy_err"] ** 2)
+    rel = np.abs(diff) / np.abs(o64["energy"])
+    k = int(np.argmax(rel))
+    out = {"per_slot_energy_rel_max": float(rel[k]), "at_slot": k,
+           "combined_error_rel_at_slot": float(err[k] / abs(o64["energy"][k])),
+           "max_abs_z": float(np.max(np.abs(diff) / err)),
+           "slots_beyond_3_sigma": int(np.sum(np.abs(diff) > 3 * err)), "slots": int(R),
+           "sweeps": f"{t_th} + {t_meas} per run, FP32 and FP64 PT with independent seeds"}
+    # the FP32 batch's own energy against FP64 evaluation of the same spins
     for i in range(R):
         b64.set_spins(i, b32.get_spins(i))
     e32, e64 = b32.total_energy(), b64.total_energy()
