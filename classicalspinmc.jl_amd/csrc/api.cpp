@@ -200,6 +200,18 @@ struct Batch {
     int lm_cap = 3;
     bool passes = false;
     PTHost pt;
+    // packed schedule of sweeps and annealing in storage mode 0 for batches
+    // of uneven lattices (pack_plan)
+    struct Pack {
+        int lm = -1, cap = -1;   // made for this storage mode and cap
+        bool use = false;
+        int nwg = 0, gmax = 1;
+        double makespan = 0, naive = 0;
+        DevBuf<int32_t> off;
+        DevBuf<int4> tasks;
+        DevBuf<uint32_t> bar0, xcc0;
+        DevBuf<double> psum0;
+    } pack;
     // metropolis_constraint_adaptive! (csmc_constraint_set): the per-basis
     // coefficients, the multiplier lambda, constraint value and accepted
     // counts per system, and per spin slot the proposal, energy change,
@@ -252,6 +264,7 @@ struct csmc_ctx {
     bool dense_tensors = false;  // contract every cubic / quartic coupling in
                                  // full (no sparsity patterns)
     bool verbose = false;        // CSMC_VERBOSE=1: device and lattice notes on stderr
+    bool pack = true;            // packed schedules for uneven batches (CSMC_PACK=0: off)
     int parts0 = 0;              // workgroups per system of the chained PT kernel in
                                  // storage mode 0: 0 chosen, 1 off, 2..8 forced
     uint32_t spin_limit = 1u << 26;  // polls of a wait for another workgroup
@@ -780,7 +793,7 @@ int setup_parts0(csmc_ctx *c, Batch *b) {
 // the parts gave up (the call's results are then not valid)
 int parts_check(Batch *b) {
     PTHost &pt = b->pt;
-    if (pt.parts0 < 2 || !pt.err0.p) return CSMC_OK;
+    if (!pt.err0.p || (pt.parts0 < 2 && !(b->pack.use && b->pack.gmax > 1))) return CSMC_OK;
     uint32_t err = 0;
     HIPCHK(hipMemcpy(&err, pt.err0.p, 4, hipMemcpyDeviceToHost));
     if (err) {
@@ -788,6 +801,142 @@ int parts_check(Batch *b) {
         HIPCHK(hipMemcpy(pt.err0.p, &zero, 4, hipMemcpyHostToDevice));
         return fail(CSMC_EHIP, "a wait for another workgroup of the same system timed out");
     }
+    return CSMC_OK;
+}
+
+// Packed schedule for sweeps and annealing in storage mode 0 (WGArgs::tasks).
+// With one workgroup per system, a batch of lattices of uneven size takes as
+// long as its costliest lattice while the CUs of the others idle.  A sweep of
+// a system costs about ncol colour phases of ceil(largest colour class /
+// (parts x workgroup size)) rounds of one site per thread (a phase has a
+// fixed latency even when few threads have a site), plus one round per phase
+// for the barrier when the system is split into parts.  The plan gives every
+// CU one workgroup: for a target M it splits every system whose cost exceeds
+// M into the fewest parts (at most 8) that bring it under M, on workgroups of
+// one XCD (block b runs on XCD b % 8) that take nothing else first, and deals
+// the other systems, costliest first, to the workgroup with the least cost
+// so far (LPT scheduling); M rises from the total cost over the workgroups
+// until a plan fits, and the plan is kept if its largest workgroup cost is
+// below 0.9 of the costliest system's.  Each system's results do not depend
+// on where, or in how many parts, it runs.
+int pack_plan(csmc_ctx *c, Batch *b, int lm) {
+    Batch::Pack &pk = b->pack;
+    const int cap = std::min(c->lds_cap, b->lm_cap);
+    if (pk.lm == lm && pk.cap == cap) return CSMC_OK;
+    pk.lm = lm;
+    pk.cap = cap;
+    pk.use = false;
+    const int R = b->nsys;
+    const int threads = b->prec == 32 ? 1024 : 512;
+    const int nwg = c->ncu > 0 ? c->ncu : 256;
+    if (!c->pack || lm != 0 || R < 2 || nwg % 8) return CSMC_OK;
+    std::vector<int> ncol(R), maxcol(R, 0);
+    for (int s = 0; s < R; ++s) {
+        const HostLattice &L = *c->lats[b->lat[s]];
+        ncol[s] = L.ncol;
+        for (int col = 0; col < L.ncol; ++col) {
+            int n = 0;
+            for (int r = L.colrun[col]; r < L.colrun[col + 1]; ++r) n += L.runs[r][1] - L.runs[r][0];
+            maxcol[s] = std::max(maxcol[s], n);
+        }
+    }
+    auto cost = [&](int s, int g) {
+        const int rounds = (maxcol[s] + g * threads - 1) / (g * threads);
+        return (double)ncol[s] * (rounds + (g > 1 ? 0.25 : 0.0));
+    };
+    double total = 0, naive = 0;
+    for (int s = 0; s < R; ++s) {
+        total += cost(s, 1);
+        naive = std::max(naive, cost(s, 1));
+    }
+    std::vector<int> order(R);
+    for (int s = 0; s < R; ++s) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x, 1) > cost(y, 1); });
+    std::vector<std::vector<int4>> best_list;
+    double best_span = naive;
+    int best_gmax = 1;
+    for (double M = std::max(1.0, total / nwg); M < naive; M *= 1.05) {
+        std::vector<int> G(R, 1);
+        int used = 0, gmax = 1;
+        bool ok = true;
+        for (int s = 0; s < R && ok; ++s) {
+            if (cost(s, 1) <= M) continue;
+            int g = 2;
+            while (g <= 8 && cost(s, g) > M) ++g;
+            if (g > 8 || (maxcol[s] + threads - 1) / threads < g) ok = false;
+            G[s] = g;
+            used += g;
+            gmax = std::max(gmax, g);
+        }
+        if (!ok || used > nwg) continue;
+        std::vector<std::vector<int>> free_wg(8);
+        for (int w = nwg - 1; w >= 0; --w) free_wg[w % 8].push_back(w);
+        std::vector<std::vector<int4>> list(nwg);
+        std::vector<double> load(nwg, 0.0);
+        for (int s : order) {
+            if (G[s] == 1) continue;
+            int x = 0;
+            for (int k = 1; k < 8; ++k)
+                if (free_wg[k].size() > free_wg[x].size()) x = k;
+            if ((int)free_wg[x].size() < G[s]) {
+                ok = false;
+                break;
+            }
+            for (int p = 0; p < G[s]; ++p) {
+                const int w = free_wg[x].back();
+                free_wg[x].pop_back();
+                list[w].push_back(make_int4(s, p, G[s], 0));
+                load[w] += cost(s, G[s]);
+            }
+        }
+        if (!ok) continue;
+        for (int s : order) {
+            if (G[s] != 1) continue;
+            int w = 0;
+            for (int k = 1; k < nwg; ++k)
+                if (load[k] < load[w]) w = k;
+            list[w].push_back(make_int4(s, 0, 1, 0));
+            load[w] += cost(s, 1);
+        }
+        double span = 0;
+        for (double l : load) span = std::max(span, l);
+        if (span < best_span) {
+            best_span = span;
+            best_list = std::move(list);
+            best_gmax = gmax;
+        }
+        if (span <= M) break;  // the target is met: a larger one cannot do better
+    }
+    pk.makespan = best_span;
+    pk.naive = naive;
+    if (best_list.empty() || best_span > 0.9 * naive) return CSMC_OK;
+    std::vector<int32_t> off(nwg + 1, 0);
+    std::vector<int4> tasks;
+    for (int w = 0; w < nwg; ++w) {
+        off[w] = (int32_t)tasks.size();
+        tasks.insert(tasks.end(), best_list[w].begin(), best_list[w].end());
+    }
+    off[nwg] = (int32_t)tasks.size();
+    HIPCHK(pk.off.alloc(off.size()));
+    HIPCHK(hipMemcpy(pk.off.p, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(pk.tasks.alloc(tasks.size()));
+    HIPCHK(hipMemcpy(pk.tasks.p, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice));
+    const int gmax = best_gmax;
+    if (gmax > 1) {
+        HIPCHK(pk.bar0.alloc(2 * (size_t)R));
+        HIPCHK(pk.xcc0.alloc((size_t)R * gmax));
+        HIPCHK(pk.psum0.alloc((size_t)R * 2 * gmax * 8));
+        if (!b->pt.err0.p) {
+            HIPCHK(b->pt.err0.alloc(1));
+            HIPCHK(hipMemset(b->pt.err0.p, 0, 4));
+        }
+    }
+    pk.nwg = nwg;
+    pk.gmax = gmax;
+    pk.use = true;
+    if (c->verbose)
+        fprintf(stderr, "csmc: packed schedule of %d systems on %d workgroups, largest cost %.0f rounds "
+                        "per sweep (costliest system %.0f), up to %d parts\n", R, nwg, best_span, naive, gmax);
     return CSMC_OK;
 }
 
@@ -844,7 +993,32 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
             const int rc = setup_parts0(c, b);
             if (rc) return rc;
         }
-        if (lm == 0 && pt.parts0 > 1) {
+        if (lm == 0 && (a.mode == WG_SWEEPS || a.mode == WG_ANNEAL)) {
+            const int rc = pack_plan(c, b, lm);
+            if (rc) return rc;
+        }
+        if (lm == 0 && (a.mode == WG_SWEEPS || a.mode == WG_ANNEAL) && b->pack.use) {
+            Batch::Pack &pk = b->pack;
+            WGArgs s = a;
+            s.tasks = pk.tasks.p;
+            s.task_off = pk.off.p;
+            s.parts0 = pk.gmax;
+            s.bar0 = pk.bar0.p;
+            s.psum0 = pk.psum0.p;
+            s.xcc0 = pk.xcc0.p;
+            s.err0 = pt.err0.p;
+            s.spin_limit = c->spin_limit;
+            if (pk.gmax > 1) HIPCHK(hipMemsetAsync(pk.bar0.p, 0, pk.bar0.n * sizeof(uint32_t), c->stream));
+            hipError_t e = launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0, sparse, pk.nwg,
+                                     lds_bytes, s, c->stream);
+            if (e == hipSuccess) {
+                parts_done = true;
+            } else {
+                (void)hipGetLastError();  // not co-resident: the plain schedule from now on
+                pk.use = false;
+            }
+        }
+        if (!parts_done && lm == 0 && pt.parts0 > 1) {
             // a launch that gave up on a wait may have left arrivals behind
             HIPCHK(hipMemsetAsync(pt.bar0.p, 0, pt.bar0.n * sizeof(uint32_t), c->stream));
             WGArgs s = a;
@@ -937,6 +1111,8 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
     if (sf && std::string(sf) == "blas") c->ssf_blas = true;
     const char *p0 = getenv("CSMC_PARTS0");
     if (p0 && p0[0] >= '1' && p0[0] <= '8' && p0[1] == 0) c->parts0 = p0[0] - '0';
+    const char *pk = getenv("CSMC_PACK");
+    if (pk && pk[0] == '0') c->pack = false;
     const char *dt = getenv("CSMC_DENSE_TENSORS");
     if (dt && dt[0] == '1') c->dense_tensors = true;
     const char *vb = getenv("CSMC_VERBOSE");

@@ -80,6 +80,13 @@ struct WGArgs {
     double *psum0;
     uint32_t *err0;  // set when a barrier wait of the parts gives up
     uint32_t *xcc0;  // [nsys][parts0]: the XCD each part runs on, written at launch
+    // WG_SWEEPS / WG_ANNEAL in storage mode 0 with a packed schedule:
+    // workgroup w runs tasks[task_off[w] .. task_off[w + 1] - 1] in turn,
+    // each {system, part, parts of that system, 0}; psum0 and xcc0 are laid
+    // out with parts0 parts per system.  Null: workgroup w is part w % parts0
+    // of system w / parts0.
+    const int32_t *task_off;
+    const int4 *tasks;
 };
 
 // Arguments of the one-colour-pass kernel (plain / stream form).

@@ -87,6 +87,7 @@ struct Parts {
     uint32_t limit;   // polls of a wait
     int buf;          // half of psum the next sum writes
     bool xlocal;      // every part of the system runs on one XCD (found at launch)
+    int gs;           // parts per system in the layout of psum and xcc (>= G)
 
     // every thread; the other parts' global stores before their barrier
     // call are visible after it.  With every part on one XCD the fences are
@@ -147,7 +148,7 @@ struct Parts {
     __device__ void find_xcd(uint32_t *xcc) {
         xlocal = false;
         if (G == 1) return;
-        uint32_t *x = xcc + (int64_t)sys * G;
+        uint32_t *x = xcc + (int64_t)sys * gs;
         if (threadIdx.x == 0)
             __hip_atomic_store(x + part, xcc_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         barrier();
@@ -163,7 +164,7 @@ struct Parts {
     template <int NV>
     __device__ void sum(double *v) {
         if (G == 1) return;
-        uint64_t *q = reinterpret_cast<uint64_t *>(psum) + ((int64_t)(sys * 2 + buf) * G) * 8;
+        uint64_t *q = reinterpret_cast<uint64_t *>(psum) + ((int64_t)(sys * 2 + buf) * gs) * 8;
         if (threadIdx.x == 0)
             for (int k = 0; k < NV; ++k)
                 __hip_atomic_store(q + part * 8 + k, (uint64_t)__double_as_longlong(v[k]), __ATOMIC_RELAXED,
@@ -374,6 +375,20 @@ __device__ __forceinline__ size_t align16(size_t v) { return (v + 15) & ~(size_t
 // specialises the bilinear slot count (all lattices of the batch have it).
 // SP: no cubic or quartic slot of the batch is contracted in full.
 // ---------------------------------------------------------------------------
+// one system (or one part of it) in storage mode 0
+template <class R, bool SELF, bool GEN, int MODE, int K2C, bool SP>
+__device__ __forceinline__ void wg_system0(const WGArgs &a, int sys, int part, int nparts, double *red) {
+    Parts pp{part, nparts, sys, a.bar0, a.psum0, a.err0, a.spin_limit, 0, false, a.parts0};
+    if (nparts > 1) pp.find_xcd(a.xcc0);
+    const SysDev sd = a.P.sys[sys];
+    const LatDev ld = a.P.lats[sd.lat];
+    const int4 *runs = a.P.runs + ld.run_off;
+    const LatView<R> L = lat_view<R>(ld, a.P.i32, runs, coef_of<R>(a.P));
+    const LatView<double> Ld = lat_view<double>(ld, a.P.i32, runs, a.P.coefD);
+    spin4<R> *G = reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off;
+    wg_run<MODE, R, SELF, GEN, K2C, SP>(a, sys, L, Ld, SpinsAoS<R>{G}, NbrG{L.nbr2, L.N}, red, pp);
+}
+
 template <class R, int LM, bool SELF, bool GEN, int MODE, int K2C, bool SP>
 __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -390,19 +405,33 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
         sys = x + 8 * (k / nparts);
         part = k % nparts;
     }
-    Parts pp{part, nparts, sys, a.bar0, a.psum0, a.err0, a.spin_limit, 0, false};
-    if (nparts > 1) pp.find_xcd(a.xcc0);
-    const SysDev sd = a.P.sys[sys];
-    const LatDev ld = a.P.lats[sd.lat];
-    const int4 *runs = a.P.runs + ld.run_off;
-    const LatView<R> L = lat_view<R>(ld, a.P.i32, runs, coef_of<R>(a.P));
-    const LatView<double> Ld = lat_view<double>(ld, a.P.i32, runs, a.P.coefD);
-    spin4<R> *G = reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off;
-    const int N = L.N;
 
     if constexpr (LM == 0) {
-        wg_run<MODE, R, SELF, GEN, K2C, SP>(a, sys, L, Ld, SpinsAoS<R>{G}, NbrG{L.nbr2, N}, red, pp);
+        // a packed schedule (sweeps and annealing of uneven batches):
+        // workgroup w runs the tasks task_off[w] .. task_off[w + 1] - 1 in
+        // turn, each a system or one part of one; otherwise one task, the
+        // system (part) this block stands for
+        int t0 = 0, t1 = 1;
+        if ((MODE == WG_SWEEPS || MODE == WG_ANNEAL) && a.tasks) {
+            t0 = a.task_off[blockIdx.x];
+            t1 = a.task_off[blockIdx.x + 1];
+        }
+        for (int t = t0; t < t1; ++t) {
+            int4 tk = make_int4(sys, part, nparts, 0);
+            if ((MODE == WG_SWEEPS || MODE == WG_ANNEAL) && a.tasks) tk = a.tasks[t];
+            wg_system0<R, SELF, GEN, MODE, K2C, SP>(a, tk.x, tk.y, tk.z, red);
+            __syncthreads();  // the reduction scratch is reused by the next task
+        }
+        if (MODE == WG_PT && a.exchange) pt_exchange_last(a, reinterpret_cast<int *>(red + RED_FLAG));
     } else {
+        const Parts pp{0, 1, sys, a.bar0, a.psum0, a.err0, a.spin_limit, 0, false, 1};
+        const SysDev sd = a.P.sys[sys];
+        const LatDev ld = a.P.lats[sd.lat];
+        const int4 *runs = a.P.runs + ld.run_off;
+        const LatView<R> L = lat_view<R>(ld, a.P.i32, runs, coef_of<R>(a.P));
+        const LatView<double> Ld = lat_view<double>(ld, a.P.i32, runs, a.P.coefD);
+        spin4<R> *G = reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off;
+        const int N = L.N;
         // N + 1 entries per component: the last one is the zero pad site
         R *sx = reinterpret_cast<R *>(smem + RED_BYTES);
         SpinsSoA<R> X{sx, sx + (N + 1), sx + 2 * (N + 1)};
@@ -419,14 +448,10 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
             __syncthreads();
             wg_run<MODE, R, SELF, GEN, K2C, SP>(a, sys, L, Ld, X, NbrL{idx, N, L.K2}, red, pp);
         }
-    }
-    // the exchange goes before the LDS modes store the spins back, so that
-    // its fence writes back only the small PT state
-    if (MODE == WG_PT && a.exchange) pt_exchange_last(a, reinterpret_cast<int *>(red + RED_FLAG));
-    if constexpr (LM != 0) {
+        // the exchange goes before the spins are stored back, so that its
+        // fence writes back only the small PT state
+        if (MODE == WG_PT && a.exchange) pt_exchange_last(a, reinterpret_cast<int *>(red + RED_FLAG));
         __syncthreads();
-        R *sx = reinterpret_cast<R *>(smem + RED_BYTES);
-        SpinsSoA<R> X{sx, sx + (N + 1), sx + 2 * (N + 1)};
         for (int i = threadIdx.x; i < N; i += blockDim.x) G[i] = X.ld(i);
     }
 }

@@ -94,3 +94,47 @@ def test_uneven_batch_fp32_anneal_lowers_energy():
     for k, lat in enumerate(lats):
         assert np.allclose(np.linalg.norm(b.get_spins(k), axis=1), 1.0, atol=1e-5)
     b.close()
+
+
+def _ctx(env):
+    import os
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return csmc.Context(0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_packed_schedule_equals_one_workgroup_per_system(precision):
+    # a batch of one large lattice (L = 128, 16384 sites: storage mode 0) and
+    # many small ones: the packed schedule splits the large lattice into parts
+    # on one XCD and gives the small ones to other workgroups.  Sweeps and a
+    # whole annealing schedule must equal, bit for bit, the schedule with one
+    # workgroup per system (CSMC_PACK=0): each system's draws and updates do
+    # not depend on where, or in how many parts, it runs.
+    cells = [(W.square_dm(-1.0, 0.05, [0.1, 0.0, 0.2]), (128, 128))]
+    cells += W.uneven_batch(n=40, sizes=(16, 24, 32), seed=9)
+    lats = [csmc.Lattice(shape, uc, 1.0, seed=30 + k) for k, (uc, shape) in enumerate(cells)]
+    out = []
+    for env in ({}, {"CSMC_PACK": "0"}):
+        print(f"[packed] precision={precision} env={env}", flush=True)
+        ctx = _ctx(dict(env, CSMC_VERBOSE="1"))
+        b = csmc.SpinBatch(lats, precision=precision, ctx=ctx)
+        b.randomize(11)
+        assert b.storage_mode() == 0
+        b.sweep(csmc.OVERRELAX, n_sweeps=3)
+        acc_s = b.sweep(csmc.METROPOLIS, T=np.linspace(0.2, 2.0, len(lats)), n_sweeps=2, seed=5)
+        acc_a = b.anneal([1.0, 0.6, 0.3], 12, 3, csmc.METROPOLIS, seed=7)
+        out.append(([b.get_spins(k) for k in range(len(lats))], b.counters(), acc_s, acc_a))
+        b.close()
+        ctx.close()
+    (s0, c0, as0, aa0), (s1, c1, as1, aa1) = out
+    assert np.array_equal(c0, c1) and np.array_equal(as0, as1) and np.array_equal(aa0, aa1)
+    for k in range(len(lats)):
+        assert np.array_equal(s0[k], s1[k]), k
