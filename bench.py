@@ -742,8 +742,46 @@ def bench_adjoint(args, csmc, W):
                                          f"threads, {sum(calls)} configurations",
                                "gpu_over_cpu": (n * N / (avg_ms * 1e-3)) / cpu}
     b.close()
-    log(f"[bench] adjoint: {value / 1e9:.2f} G spin-updates/s, k_gd_step {avg_ms:.3f} ms ({achieved:.0f} GB/s)")
+    out["thermal_dE_dJ2"] = thermal_dE_dJ2(csmc, W)
+    log(f"[bench] adjoint: {value / 1e9:.2f} G spin-updates/s, k_gd_step {avg_ms:.3f} ms ({achieved:.0f} GB/s); "
+        f"thermal d<E>/dJ2 {out['thermal_dE_dJ2']}")
     finish(out, args)
+
+
+def thermal_dE_dJ2(csmc, W, L=96, n=128, T=0.1, J2=0.1, d=0.005, samples=80):
+    """The thermal gradient of the differentiation workload: d<E>/dJ2 of the
+    triangular J1-J2 lattice (J1=1, J2=0.1, L=96, S=1) at T=0.1 from the
+    covariance of E and the adjoint dE/dJ2 over 128 chains x 80 samples,
+    beside a central finite difference of <E> from runs at J2 +- 0.005 with
+    the same seeds.  Each run anneals to T (22 temperatures x 100 sweeps),
+    then 200 + 80 x 10 cone-move Metropolis sweeps, each after 5
+    overrelaxation sweeps."""
+    from csmc.thermal import thermal_coupling_gradient
+
+    def run(j2):
+        uc = W.triangular_j1j2(1.0, j2)
+        w = np.zeros(uc.n_params())
+        for t in range(6, 12):
+            w[9 * t + np.array([0, 4, 8])] = 1.0
+        b = csmc.SpinBatch([csmc.Lattice((L, L), uc, 1.0, seed=5)] * n, precision=64)
+        b.randomize(7)
+        b.set_counters(0)
+        b.anneal(W.anneal_temperatures(1.0, 0.9, T), 100, 5, csmc.CONE_ADAPTIVE, seed=11)
+        out = thermal_coupling_gradient(b, T, samples, sweeps_between=10, overrelaxation=5, thermalize=200,
+                                        alg=csmc.CONE, sigma=0.3, seed=11, directions=w)
+        b.close()
+        return out
+
+    t0 = time.perf_counter()
+    g, err, m = run(J2)
+    secs = time.perf_counter() - t0
+    mp, mm = run(J2 + d)[2], run(J2 - d)[2]
+    fd = (mp["energy_per_system"] - mm["energy_per_system"]) / (2 * d)
+    return {"value": round(float(g), 2), "err": round(float(err), 2),
+            "finite_difference": round(float(fd.mean()), 2),
+            "finite_difference_err": round(float(fd.std(ddof=1) / np.sqrt(n)), 2),
+            "T": T, "J2": J2, "samples": m["samples"], "seconds": round(secs, 3),
+            "T0_limit": 3 * L * L}
 
 
 PEAK_FP64_TFLOPS = 78.6

@@ -16,15 +16,19 @@ from . import _lib
 
 
 def thermal_coupling_gradient(batch, T, n_samples, sweeps_between=10, overrelaxation=0,
-                              thermalize=200, alg=_lib.METROPOLIS, sigma=60.0, seed=0):
+                              thermalize=200, alg=_lib.METROPOLIS, sigma=60.0, seed=0, directions=None):
     """d<E>/dJ for every coupling entry of the batch's lattice (layout of
     UnitCell.params()) at temperature T.
 
     Each system runs `thermalize` Metropolis sweeps, then `n_samples` samples
     `sweeps_between` Metropolis sweeps apart (each preceded by
-    `overrelaxation` overrelaxation sweeps).  Returns (grad, err, means) with
-    err the jackknife standard error over the independent chains and means a
-    dict of <E>, <dE/dJ> and the sample count."""
+    `overrelaxation` overrelaxation sweeps).  With `directions` (a (P,) or
+    (k, P) array), the derivatives along those directions of coupling space
+    are returned instead: a coupling scaled into several entries (J2 times
+    the identity in six bilinear terms, say) is one direction.  Returns
+    (grad, err, means) with err the jackknife standard error over the
+    independent chains and means a dict of <E>, <dE/dJ>, the per-system mean
+    energies and the sample count."""
     first = batch.lattices[0]
     if any(lat is not first for lat in batch.lattices):
         raise ValueError("the systems must share one lattice")
@@ -33,6 +37,12 @@ def thermal_coupling_gradient(batch, T, n_samples, sweeps_between=10, overrelaxa
     n = batch.n
     Tv = np.full(n, float(T))
     sg = np.full(n, float(sigma))
+    P = first.unit_cell.n_params()
+    W = None
+    if directions is not None:
+        W = np.atleast_2d(np.asarray(directions, dtype=np.float64))
+        if W.shape[1] != P:
+            raise ValueError(f"directions need {P} entries each, got {W.shape[1]}")
 
     def advance(k):
         for _ in range(k):
@@ -41,14 +51,16 @@ def thermal_coupling_gradient(batch, T, n_samples, sweeps_between=10, overrelaxa
             batch.sweep(alg, T=Tv, sigma=sg, n_sweeps=1, seed=seed)
 
     advance(thermalize)
-    P = first.unit_cell.n_params()
+    Q = P if W is None else W.shape[0]
     sE = np.zeros(n)
-    sG = np.zeros((n, P))
-    sEG = np.zeros((n, P))
+    sG = np.zeros((n, Q))
+    sEG = np.zeros((n, Q))
     for _ in range(n_samples):
         advance(sweeps_between)
         E = batch.total_energy()
         G = batch.grad_couplings()
+        if W is not None:
+            G = G @ W.T
         sE += E
         sG += G
         sEG += E[:, None] * G
@@ -65,5 +77,8 @@ def thermal_coupling_gradient(batch, T, n_samples, sweeps_between=10, overrelaxa
         jk = np.array([estimate(np.arange(n) != k)[0] for k in range(n)])
         err = np.sqrt((n - 1) / n * ((jk - jk.mean(axis=0)) ** 2).sum(axis=0))
     else:
-        err = np.full(P, np.nan)
-    return full, err, {"energy": mE, "dE_dJ": mG, "samples": n * n_samples}
+        err = np.full(Q, np.nan)
+    if W is not None and np.ndim(directions) == 1:
+        full, err, mG = full[0], err[0], mG[0]
+    return full, err, {"energy": mE, "dE_dJ": mG, "energy_per_system": sE / n_samples,
+                       "samples": n * n_samples}
