@@ -214,6 +214,13 @@ class SpinBatch:
                                                 int(n_sweeps), C.c_uint64(int(seed)), _lib.ptr(acc)))
         return acc
 
+    def tiles(self):
+        """tiles per system of the batch-wide sweeps (0: one launch per
+        colour pass) and, for 0, the reason"""
+        n = C.c_int32(0)
+        _lib.check(_lib.lib().csmc_batch_tiles(self._h, self.id, C.byref(n)))
+        return n.value, ("" if n.value else _lib.lib().csmc_last_error().decode())
+
     AUTOTUNE_FORMS = ("persistent, storage mode cap 3", "persistent, cap 2", "persistent, cap 1",
                       "persistent, cap 0", "batch-wide colour passes")
 

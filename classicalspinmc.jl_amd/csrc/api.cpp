@@ -23,6 +23,7 @@
 #include "kernels.h"
 #include "lattice.h"
 #include "split.h"
+#include "tile.h"
 
 using namespace csmc;
 
@@ -187,7 +188,7 @@ struct Batch {
     std::vector<SysDev> sys;
     DevBuf<SysDev> sys_d;
     DevBuf<unsigned char> spins;
-    DevBuf<unsigned char> spins_alt;  // the other buffer of csmc_gradient_descent
+    DevBuf<unsigned char> spins_alt;  // the other buffer of csmc_gradient_descent and the tiled sweeps
     DevBuf<uint32_t> mcount;
     DevBuf<double> T, sigma, acc;  // per-system scratch
     DevBuf<uint32_t> accU;
@@ -221,7 +222,17 @@ struct Batch {
     double cons_c0 = 0.0, cons_w = 0.0, cons_sigma0 = 60.0;
     DevBuf<double> cons_coef, cons_lambda, cons_c, cons_acc, cons_total, cons_dE, cons_dC, cons_u;
     DevBuf<unsigned char> cons_prop, cons_dec;
-    size_t spin_bytes() const { return prec == 32 ? 16 : 32; }
+    // tiled sweeps (tile.cpp / tile.hip) of the batch-wide sweep form:
+    // state 0 not planned yet, 1 every lattice has a plan, -1 some has none
+    struct Tiles {
+        int state = 0;
+        std::string why;
+        size_t lds = 0;
+        int max_tiles = 0;
+        DevBuf<TileDev> td;
+        DevBuf<int32_t> tab;
+    } tiles;
+    size_t spin_bytes() const { return prec == 32 ? 12 : 24; }  // packed {x,y,z} records
 };
 
 }  // namespace
@@ -232,6 +243,7 @@ struct csmc_graph {
     csmc_ctx *ctx = nullptr;
     int32_t batch = -1;
     int32_t kind = 0;
+    void *spins = nullptr;  // the spin buffer the recorded launches use
 };
 
 struct csmc_ctx {
@@ -265,6 +277,8 @@ struct csmc_ctx {
                                  // full (no sparsity patterns)
     bool verbose = false;        // CSMC_VERBOSE=1: device and lattice notes on stderr
     bool pack = true;            // packed schedules for uneven batches (CSMC_PACK=0: off)
+    bool tile = true;            // tiled batch-wide sweeps where a lattice allows (CSMC_TILE=0: off)
+    size_t tile_lds = 48 << 10;  // LDS budget of one tile (CSMC_TILE_LDS, KiB)
     int parts0 = 0;              // workgroups per system of the chained PT kernel in
                                  // storage mode 0: 0 chosen, 1 off, 2..8 forced
     uint32_t spin_limit = 1u << 26;  // polls of a wait for another workgroup
@@ -1056,6 +1070,7 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
 namespace {
 int sweep_passes(csmc_ctx *c, Batch *b, int32_t kind, const double *T, const double *sigma, int32_t n_sweeps,
                  uint64_t seed, double *accepted);
+bool tiles_ready(csmc_ctx *c, Batch *b);
 int constraint_run(csmc_ctx *c, Batch *b, uint64_t seed, int64_t n_sweeps);
 }  // namespace
 
@@ -1113,6 +1128,10 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
     if (p0 && p0[0] >= '1' && p0[0] <= '8' && p0[1] == 0) c->parts0 = p0[0] - '0';
     const char *pk = getenv("CSMC_PACK");
     if (pk && pk[0] == '0') c->pack = false;
+    const char *tl = getenv("CSMC_TILE");
+    if (tl && tl[0] == '0') c->tile = false;
+    const char *tb = getenv("CSMC_TILE_LDS");
+    if (tb && tb[0] >= '1' && tb[0] <= '9') c->tile_lds = (size_t)strtoul(tb, nullptr, 10) << 10;
     const char *dt = getenv("CSMC_DENSE_TENSORS");
     if (dt && dt[0] == '1') c->dense_tensors = true;
     const char *vb = getenv("CSMC_VERBOSE");
@@ -1219,6 +1238,28 @@ int32_t csmc_split_plan(const csmc_unit_cell *uc, const int32_t *shape, int32_t 
         stats[9 + 4 * g] = (int64_t)sp.halo.size();
         stats[10 + 4 * g] = (int64_t)sp.bnd.size();
     }
+    return CSMC_OK;
+}
+
+int32_t csmc_tile_plan(const csmc_unit_cell *uc, const int32_t *shape, int32_t periodic, int32_t prec,
+                       int64_t lds_budget, int32_t *head, int32_t *tab, int32_t tab_cap, int32_t *tab_len) {
+    if (!head || !tab_len || (prec != 32 && prec != 64) || lds_budget < 0)
+        return fail(CSMC_EINVAL, "tile plan: head and tab_len needed, prec 32 or 64");
+    HostLattice L;
+    std::string err;
+    int rc = build_lattice(uc, shape, periodic, 0.5, L, err);
+    if (rc) return fail(rc, err);
+    const TilePlan P = plan_tiles(L, 3 * (prec == 32 ? 4 : 8), (size_t)lds_budget);
+    static_assert(sizeof(TileDev) == CSMC_TILE_HEAD * 4, "tile plan header");
+    if (!P.ok) {
+        std::memset(head, 0, sizeof(TileDev));
+        *tab_len = 0;
+        g_err = P.why;
+        return CSMC_OK;
+    }
+    std::memcpy(head, &P.d, sizeof(TileDev));
+    *tab_len = (int32_t)P.tab.size();
+    if (tab && tab_cap >= (int32_t)P.tab.size()) std::memcpy(tab, P.tab.data(), P.tab.size() * 4);
     return CSMC_OK;
 }
 
@@ -1551,6 +1592,17 @@ int32_t csmc_sweep_passes(csmc_ctx *c, int32_t id, int32_t kind, const double *T
     return sweep_passes(c, b, kind, T, sigma, n_sweeps, seed, accepted);
 }
 
+int32_t csmc_batch_tiles(csmc_ctx *c, int32_t id, int32_t *tiles) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (!tiles) return fail(CSMC_EINVAL, "null output");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    *tiles = (c->tile && tiles_ready(c, b)) ? b->tiles.max_tiles : 0;
+    if (!*tiles) g_err = c->tile ? b->tiles.why : "tiles switched off (CSMC_TILE=0)";
+    return CSMC_OK;
+}
+
 int32_t csmc_autotune(csmc_ctx *c, int32_t id, int32_t kind, const double *T, int32_t n_sweeps, double *times,
                       int32_t *choice) {
     CHECK_CTX(c);
@@ -1687,6 +1739,48 @@ int constraint_run(csmc_ctx *c, Batch *b, uint64_t seed, int64_t n_sweeps) {
     return CSMC_OK;
 }
 
+// the tile plans of every lattice of the batch (once per batch); false if
+// some lattice has no tiled form
+bool tiles_ready(csmc_ctx *c, Batch *b) {
+    Batch::Tiles &tl = b->tiles;
+    if (tl.state != 0) return tl.state > 0;
+    tl.state = -1;
+    const size_t rec = 3 * (b->prec == 32 ? 4 : 8);
+    std::vector<TileDev> td(c->lats.size(), TileDev{});
+    std::vector<int32_t> tab;
+    std::vector<char> done(c->lats.size(), 0);
+    for (int s = 0; s < b->nsys; ++s) {
+        const int id = b->lat[s];
+        if (done[id]) continue;
+        done[id] = 1;
+        TilePlan P = plan_tiles(*c->lats[id], rec, c->tile_lds);
+        if (!P.ok) {
+            tl.why = "lattice " + std::to_string(id) + ": " + P.why;
+            if (c->verbose) fprintf(stderr, "csmc: batch-wide sweeps without tiles (%s)\n", tl.why.c_str());
+            return false;
+        }
+        P.d.tab_off = (int32_t)tab.size();
+        tab.insert(tab.end(), P.tab.begin(), P.tab.end());
+        td[id] = P.d;
+        tl.lds = std::max(tl.lds, (size_t)(P.d.nbox + 1) * rec + 4 * (size_t)P.d.p * P.d.nb);
+        tl.max_tiles = std::max(tl.max_tiles, P.d.ntiles);
+        if (c->verbose)
+            fprintf(stderr,
+                    "csmc: lattice %d tiled: tile %dx%dx%d cells, box %dx%dx%d x %d bases (%.3f sites loaded, "
+                    "%.3f updated per site), %d tiles, %zu B LDS\n",
+                    id, P.d.T[0], P.d.T[1], P.d.T[2], P.d.B[0], P.d.B[1], P.d.B[2], P.d.nb, P.load_per_site,
+                    P.work_per_site, P.d.ntiles, (size_t)(P.d.nbox + 1) * rec);
+    }
+    if (tl.lds > (size_t)c->max_lds) return false;
+    if (tl.td.alloc(td.size()) != hipSuccess || tl.tab.alloc(std::max<size_t>(tab.size(), 1)) != hipSuccess)
+        return false;
+    if (hipMemcpy(tl.td.p, td.data(), td.size() * sizeof(TileDev), hipMemcpyHostToDevice) != hipSuccess ||
+        (!tab.empty() && hipMemcpy(tl.tab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice) != hipSuccess))
+        return false;
+    tl.state = 1;
+    return true;
+}
+
 int sweep_passes(csmc_ctx *c, Batch *b, int32_t kind, const double *T, const double *sigma, int32_t n_sweeps,
                  uint64_t seed, double *accepted) {
     if (!valid_kind(kind)) return fail(CSMC_EINVAL, "unknown sweep kind");
@@ -1727,10 +1821,41 @@ int sweep_passes(csmc_ctx *c, Batch *b, int32_t kind, const double *T, const dou
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, c->stream));
     }
+    // tiled: one launch per sweep, every colour step of a tile from LDS
+    const bool tiled = c->tile && tiles_ready(c, b);
+    TileArgs ta{};
+    if (tiled) {
+        // the tiled sweep writes the other buffer; the pad records (never
+        // written) start at zero
+        if (!b->spins_alt.p || b->spins_alt.n != b->spins.n) {
+            HIPCHK(b->spins_alt.alloc(b->spins.n));
+            HIPCHK(hipMemsetAsync(b->spins_alt.p, 0, b->spins.n, c->stream));
+        }
+        ta.P = a.P;
+        ta.td = b->tiles.td.p;
+        ta.tab = b->tiles.tab.p;
+        ta.kind = kind;
+        ta.k0 = a.k0;
+        ta.k1 = a.k1;
+        ta.ctr = a.ctr;
+        ta.T_sys = a.T_sys;
+        ta.sigma_sys = a.sigma_sys;
+        ta.acc = a.acc;
+    }
     for (int sw = 0; sw < n_sweeps; ++sw) {
-        for (int col = 0; col < ncol; ++col) {  // systems with fewer colours skip the extra passes
+        // systems go on the grid's y dimension, at most 65535 a launch
+        if (tiled) {
+            ta.spins_in = b->spins.p;
+            ta.spins_out = b->spins_alt.p;
+            for (int s0 = 0; s0 < b->nsys; s0 += 65535) {
+                ta.sys0 = s0;
+                HIPCHK(launch_tile(b->prec, b->self, std::min(65535, b->nsys - s0), b->tiles.max_tiles,
+                                   b->tiles.lds, ta, c->stream));
+            }
+            std::swap(b->spins.p, b->spins_alt.p);  // the same size: only the roles change
+        }
+        for (int col = 0; !tiled && col < ncol; ++col) {  // systems with fewer colours skip the extra passes
             a.colour = col;
-            // systems go on the grid's y dimension, at most 65535 a launch
             for (int s0 = 0; s0 < b->nsys; s0 += 65535) {
                 a.sys0 = s0;
                 HIPCHK(launch_pass(b->prec, b->self, b->gen, std::min(65535, b->nsys - s0), maxcol, a, c->stream));
@@ -1818,6 +1943,7 @@ int32_t csmc_graph_sweeps(csmc_ctx *c, int32_t id, int32_t kind, const double *T
     gr->ctx = c;
     gr->batch = id;
     gr->kind = kind;
+    gr->spins = b->spins.p;
     *out = gr;
     return CSMC_OK;
 }
@@ -1826,7 +1952,13 @@ int32_t csmc_graph_launch(csmc_ctx *c, csmc_graph *g) {
     CHECK_CTX(c);
     if (!g || g->ctx != c) return fail(CSMC_EINVAL, "graph of another context");
     GET_BATCH(c, g->batch, b);
-    (void)b;
+    if (b->spins.p != g->spins) {
+        // tiled sweeps since the capture left the state in the batch's other
+        // buffer: move it back to the one the graph was recorded with
+        if (g->spins != b->spins_alt.p) return fail(CSMC_ESTATE, "the graph's spin buffer is gone");
+        HIPCHK(hipMemcpyAsync(b->spins_alt.p, b->spins.p, b->spins.n, hipMemcpyDeviceToDevice, c->stream));
+        std::swap(b->spins.p, b->spins_alt.p);
+    }
     HIPCHK(hipGraphLaunch(g->exec, c->stream));
     return CSMC_OK;
 }
@@ -2311,7 +2443,7 @@ std::vector<unsigned char> pt_head(Batch *b) {
     PTHost &pt = b->pt;
     PtSaveHead h{};
     std::memcpy(h.magic, PT_MAGIC, 8);
-    h.version = 1;
+    h.version = 2;  // 2: packed 12 / 24-byte spin records
     h.nsys = b->nsys;
     h.prec = b->prec;
     h.nsec = PT_NSEC;
@@ -2426,7 +2558,7 @@ int32_t csmc_pt_load(csmc_ctx *c, int32_t id, const void *buf, int64_t bytes) {
     const unsigned char *in = (const unsigned char *)buf;
     PtSaveHead h;
     std::memcpy(&h, in, sizeof h);
-    if (std::memcmp(h.magic, PT_MAGIC, 8) != 0 || h.version != 1 || h.nsec != PT_NSEC)
+    if (std::memcmp(h.magic, PT_MAGIC, 8) != 0 || h.version != 2 || h.nsec != PT_NSEC)
         return fail(CSMC_EINVAL, "not a saved parallel-tempering state of this library version");
     // the run must be set up as the saved one was
     std::vector<unsigned char> mine = pt_head(b);

@@ -49,10 +49,10 @@ __global__ __launch_bounds__(1024) void k_cons_begin(ConsArgs a) {
     const SysDev sd = a.P.sys[sys];
     const LatDev ld = a.P.lats[sd.lat];
     const int32_t *sub = a.P.i32 + ld.sub_off;
-    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(a.spins) + sd.spin_off;
+    const SpinsG<R> S = spins_at<R>(a.spins, sd.spin_off);
     double v[1] = {0.0};
     for (int i = threadIdx.x; i < ld.N; i += blockDim.x) {
-        const spin4<R> s = S[i];
+        const spin4<R> s = S.ld(i);
         v[0] += cons_term(a, sub[i], (double)s.x, (double)s.y, (double)s.z);
     }
     block_sum<1>(v, red);
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void k_cons_propose(ConsArgs a) {
     const int end = runs[L.colrun[a.colour + 1] - 1].y;
     const int i = start + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= end) return;
-    const SpinsAoS<R> X{reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off};
+    const SpinsG<R> X = spins_at<R>(a.spins, sd.spin_off);
     const NbrG nb{L.nbr2, L.N};
     const int b = L.sub[i];
     const spin4<R> s = X.ld(i);
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_cons_propose(ConsArgs a) {
     const spin4<R> n = metro_propose<R>(L, s, r, CSMC_CONE, (R)a.sigma_sys[sys]);
     const R dE = metro_eval<R, SELF, GEN, 0>(L, nb, X, b, i, s, n);
     const int64_t k = sd.spin_off + i;
-    reinterpret_cast<spin4<R> *>(a.prop)[k] = n;
+    spins_at<R>(a.prop, 0).st(k, n);
     a.dE[k] = (double)dE;
     a.dC[k] = cons_term(a, b, (double)n.x, (double)n.y, (double)n.z) -
               cons_term(a, b, (double)s.x, (double)s.y, (double)s.z);
@@ -147,19 +147,19 @@ __global__ __launch_bounds__(256) void k_cons_commit(ConsArgs a) {
     const int end = runs[colrun[a.colour + 1] - 1].y;
     const int i = start + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= end) return;
-    spin4<R> *X = reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off;
+    const SpinsG<R> X = spins_at<R>(a.spins, sd.spin_off);
     const int64_t k = sd.spin_off + i;
     if (a.dec[k]) {
-        X[i] = reinterpret_cast<const spin4<R> *>(a.prop)[k];
+        X.st(i, spins_at<R>(a.prop, 0).ld(k));
     } else if (sizeof(R) == 4) {
         // an FP32 state pulls a rejected spin back to |s| = S (as the other
         // Metropolis kinds do)
-        spin4<R> s = X[i];
+        spin4<R> s = X.ld(i);
         const R inv = Math<R>::rsqrt_(s.x * s.x + s.y * s.y + s.z * s.z) * (R)ld.S;
         s.x *= inv;
         s.y *= inv;
         s.z *= inv;
-        X[i] = s;
+        X.st(i, s);
     }
 }
 

@@ -23,8 +23,9 @@
 //              c4[nb][K4][81]: the couplings of each (basis, slot) with the
 //              role permutation of lattice.jl:184-193, 215-227, 251-272
 //              already applied; padding slots are zero.
-//   spins      spin4<R> {x,y,z,pad} per device site plus one zero pad site
-//              (N + 1 records per system), systems back to back.
+//   spins      packed {x,y,z} records of R per device site (12 B FP32,
+//              24 B FP64) plus one zero pad site (N + 1 records per
+//              system), systems back to back.
 #pragma once
 #include <stdint.h>
 
@@ -89,7 +90,7 @@ constexpr int SPLIT_HDR = 16;   // header ints per part in the split tables
 struct SysDev {
     int32_t lat;      // lattice id
     int32_t stream;   // Philox stream word of this system
-    int64_t spin_off; // first spin4 of this system (N + 1 records per system)
+    int64_t spin_off; // first spin record of this system (N + 1 records per system)
     int64_t site_off; // first site of this system in host-facing outputs
 };
 
@@ -122,6 +123,31 @@ constexpr int32_t TP_C3_ALT = 2;  // cubic: p, q, r pairwise distinct (S.(S x S)
 constexpr int32_t TP_Q4_PQ = 2;   // quartic: p = q and r = t ((Si.Sj)(Sk.Sl))
 constexpr int32_t TP_Q4_PR = 3;   // quartic: p = r and q = t ((Si.Sk)(Sj.Sl))
 constexpr int32_t TP_Q4_PT = 4;   // quartic: p = t and q = r ((Si.Sl)(Sj.Sk))
+
+// Tiled sweep of a periodic Bravais lattice (tile.hip): the lattice is cut
+// into tiles of T cells, each swept from a box of B cells (the tile plus the
+// halo whose sites a whole sweep of the tile reads or recomputes) staged in
+// LDS.  Box-local site index = b * boxcells + cell, cell last-dimension
+// fastest.  Tables at tab_off in the tile table pool:
+//   e[nb]                colour offset of each basis
+//   runbase[p * nb]      device index of the first site of the run of raw
+//                        colour r and basis b (-1: no such run)
+//   delta[nb * K2]       box-local index step to the partner of slot k of
+//                        basis b (TILE_PAD: the slot names no site)
+//   rect[ncol * nb * 6]  box cells {lo, hi} per dimension where colour step
+//                        c updates basis b (lo >= hi: none)
+//   rawc[ncol]           raw colour (c . x + e_b) mod p of each colour
+constexpr int32_t TILE_PAD = 0x7fffffff;
+
+struct TileDev {
+    int32_t D, nb, K2, p;
+    int32_t cz;       // 1: the colour depends on the basis only
+    int32_t cinv;     // inverse of c[D-1] mod p (cz == 0)
+    int32_t ncol, ntiles;
+    int32_t L[3], T[3], lo[3], B[3], nt[3], c[3];
+    int32_t boxcells, nbox, tcells, ncell;
+    int32_t tab_off, pad_;
+};
 
 // incidence entry: owner device site and a packed (kind, slot, role)
 //   code = kind(2 bits: 0 bil, 1 cub, 2 qua) | role << 2 (2 bits) | slot << 4

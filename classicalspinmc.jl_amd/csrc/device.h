@@ -78,15 +78,34 @@ __device__ __forceinline__ LatView<R> lat_view(const LatDev &d, const int32_t *i
 }
 
 // ---------------------------------------------------------------------------
-// spin storage views: spin4 records in global memory, or three component
-// arrays (12 B per FP32 site) in LDS
+// spin storage views: packed three-component records in global memory (12 B
+// per FP32 site, 24 B per FP64 site, no padding word), or three component
+// arrays in LDS.  In registers a spin is a spin4 with w = 0.
 // ---------------------------------------------------------------------------
 template <class R>
-struct SpinsAoS {
-    spin4<R> *p;
-    __device__ __forceinline__ spin4<R> ld(int i) const { return p[i]; }
-    __device__ __forceinline__ void st(int i, const spin4<R> &v) const { p[i] = v; }
+struct rec3 {
+    R x, y, z;
 };
+
+template <class R>
+struct SpinsG {
+    rec3<R> *p;
+    __device__ __forceinline__ spin4<R> ld(int64_t i) const {
+        const rec3<R> r = p[i];
+        return spin4<R>{r.x, r.y, r.z, R(0)};
+    }
+    __device__ __forceinline__ void st(int64_t i, const spin4<R> &v) const { p[i] = rec3<R>{v.x, v.y, v.z}; }
+};
+
+// the records of the system whose first record is `off` in a spin buffer
+template <class R>
+__device__ __forceinline__ SpinsG<R> spins_at(void *buf, int64_t off) {
+    return SpinsG<R>{reinterpret_cast<rec3<R> *>(buf) + off};
+}
+template <class R>
+__device__ __forceinline__ SpinsG<R> spins_at(const void *buf, int64_t off) {
+    return SpinsG<R>{const_cast<rec3<R> *>(reinterpret_cast<const rec3<R> *>(buf)) + off};
+}
 
 template <class R>
 struct SpinsSoA {

@@ -104,6 +104,23 @@ struct PassArgs {
     uint32_t *acc;           // [nsys] accepted proposals
 };
 
+// one whole coloured sweep of every system from LDS tiles (tile.hip)
+struct TileArgs {
+    DevPools P;
+    const void *spins_in;    // the state before the sweep
+    void *spins_out;         // the state after it (every site is written)
+    const TileDev *td;       // [lattice id] tile plan (ntiles 0: not used)
+    const int32_t *tab;      // tile table pool
+    int sys0;
+    int kind;
+    uint32_t k0, k1;
+    const uint32_t *ctr;     // [nsys] Philox sweep counter word per system, or null
+    uint32_t ctr0;           // counter word used when ctr is null
+    const double *T_sys;
+    const double *sigma_sys;
+    uint32_t *acc;           // [nsys] accepted proposals
+};
+
 // metropolis_constraint_adaptive! sweeps (constraint.hip) for the constraint
 // c = sum_i (a_b . s_i + s_i . Q_b s_i) - c0
 struct ConsArgs {
@@ -161,6 +178,9 @@ hipError_t launch_wg_reg(int k2c, int spt, bool diag, bool two_colour, int nsys,
 // sites) + 8 * (most halo sites)
 hipError_t launch_wg_split(int k2c, int spt, bool diag, int G, int R, size_t lds_bytes,
                            const WGArgs &a, hipStream_t st);
+// grid (grid_x, nsys): workgroup x of system y sweeps tiles x, x + grid_x, ...
+hipError_t launch_tile(int prec, bool self, int nsys, int grid_x, size_t lds_bytes, const TileArgs &a,
+                       hipStream_t st);
 hipError_t launch_pass(int prec, bool self, bool gen, int nsys, int max_colour_sites,
                        const PassArgs &a, hipStream_t st);
 // captured plain-form sweeps: the per-sweep counter (and adaptive cone)

@@ -385,8 +385,8 @@ __device__ __forceinline__ void wg_system0(const WGArgs &a, int sys, int part, i
     const int4 *runs = a.P.runs + ld.run_off;
     const LatView<R> L = lat_view<R>(ld, a.P.i32, runs, coef_of<R>(a.P));
     const LatView<double> Ld = lat_view<double>(ld, a.P.i32, runs, a.P.coefD);
-    spin4<R> *G = reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off;
-    wg_run<MODE, R, SELF, GEN, K2C, SP>(a, sys, L, Ld, SpinsAoS<R>{G}, NbrG{L.nbr2, L.N}, red, pp);
+    wg_run<MODE, R, SELF, GEN, K2C, SP>(a, sys, L, Ld, spins_at<R>(a.spins, sd.spin_off), NbrG{L.nbr2, L.N}, red,
+                                        pp);
 }
 
 template <class R, int LM, bool SELF, bool GEN, int MODE, int K2C, bool SP>
@@ -430,12 +430,12 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
         const int4 *runs = a.P.runs + ld.run_off;
         const LatView<R> L = lat_view<R>(ld, a.P.i32, runs, coef_of<R>(a.P));
         const LatView<double> Ld = lat_view<double>(ld, a.P.i32, runs, a.P.coefD);
-        spin4<R> *G = reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off;
+        const SpinsG<R> G = spins_at<R>(a.spins, sd.spin_off);
         const int N = L.N;
         // N + 1 entries per component: the last one is the zero pad site
         R *sx = reinterpret_cast<R *>(smem + RED_BYTES);
         SpinsSoA<R> X{sx, sx + (N + 1), sx + 2 * (N + 1)};
-        wg_stage<8>(N + 1, [&](int i) { return G[i]; }, [&](int i, const spin4<R> &v) { X.st(i, v); });
+        wg_stage<8>(N + 1, [&](int i) { return G.ld(i); }, [&](int i, const spin4<R> &v) { X.st(i, v); });
         if constexpr (LM == 1) {
             __syncthreads();
             wg_run<MODE, R, SELF, GEN, K2C, SP>(a, sys, L, Ld, X, NbrG{L.nbr2, N}, red, pp);
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
         // fence writes back only the small PT state
         if (MODE == WG_PT && a.exchange) pt_exchange_last(a, reinterpret_cast<int *>(red + RED_FLAG));
         __syncthreads();
-        for (int i = threadIdx.x; i < N; i += blockDim.x) G[i] = X.ld(i);
+        for (int i = threadIdx.x; i < N; i += blockDim.x) G.st(i, X.ld(i));
     }
 }
 
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a) {
     const int i = start + blockIdx.x * blockDim.x + threadIdx.x;
     int acc = 0;
     if (i < end) {
-        const SpinsAoS<R> X{reinterpret_cast<spin4<R> *>(a.spins) + sd.spin_off};
+        const SpinsG<R> X = spins_at<R>(a.spins, sd.spin_off);
         const NbrG nb{L.nbr2, L.N};
         with_wave_basis(L.sub[i], [&](int b) __attribute__((always_inline)) {
             if (a.kind == CSMC_OVERRELAX) {
@@ -664,13 +664,13 @@ __global__ __launch_bounds__(256) void k_site_eval(DevPools P, const void *spins
     const SysDev sd = P.sys[sys];
     const LatDev ld = P.lats[sd.lat];
     const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
-    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
+    const SpinsG<R> S = spins_at<R>(spins, sd.spin_off);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= L.N) return;
     int b = L.sub[i];
-    spin4<double> s = to_d<R>(S[i]);
+    spin4<double> s = to_d<R>(S.ld(i));
     double F[3], Hl[3];
-    interaction_field<double, double>(L, NbrG{L.nbr2, L.N}, b, i, [&](int j) { return to_d<R>(S[j]); }, F);
+    interaction_field<double, double>(L, NbrG{L.nbr2, L.N}, b, i, [&](int j) { return to_d<R>(S.ld(j)); }, F);
     local_field<double, double>(L, b, s, F, Hl);
     int r = L.ref[i];
     if (H) {
@@ -701,7 +701,7 @@ __global__ __launch_bounds__(1024) void k_energy(DevPools P, const void *spins, 
     const SysDev sd = P.sys[sys];
     const LatDev ld = P.lats[sd.lat];
     const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
-    const SpinsAoS<R> S{const_cast<spin4<R> *>(reinterpret_cast<const spin4<R> *>(spins)) + sd.spin_off};
+    const SpinsG<R> S = spins_at<R>(spins, sd.spin_off);
     double p[8];
     wg_energy<R, true, false>(L, NbrG{L.nbr2, L.N}, S, red, p);
     if (threadIdx.x == 0) {
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(1024) void k_magnet(DevPools P, const void *spins, 
     const SysDev sd = P.sys[sys];
     const LatDev ld = P.lats[sd.lat];
     const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
-    const SpinsAoS<R> S{const_cast<spin4<R> *>(reinterpret_cast<const spin4<R> *>(spins)) + sd.spin_off};
+    const SpinsG<R> S = spins_at<R>(spins, sd.spin_off);
     double m[3];
     wg_magnet<R>(L, S, red, m);
     if (threadIdx.x == 0)
@@ -758,7 +758,7 @@ __global__ __launch_bounds__(256) void k_randomize(DevPools P, void *spins, uint
     s.y = o[1];
     s.z = o[2];
     s.w = R(0);
-    reinterpret_cast<spin4<R> *>(spins)[sd.spin_off + i] = s;
+    spins_at<R>(spins, sd.spin_off).st(i, s);
 }
 
 hipError_t launch_randomize(int prec, const DevPools &P, void *spins, int nsys, int maxN,
@@ -782,7 +782,7 @@ __global__ __launch_bounds__(256) void k_scatter(DevPools P, void *spins, int sy
     s.y = (R)src[3 * (int64_t)r + 1];
     s.z = (R)src[3 * (int64_t)r + 2];
     s.w = R(0);
-    reinterpret_cast<spin4<R> *>(spins)[sd.spin_off + i] = s;
+    spins_at<R>(spins, sd.spin_off).st(i, s);
 }
 
 template <class R>
@@ -793,7 +793,7 @@ __global__ __launch_bounds__(256) void k_gather(DevPools P, const void *spins, i
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ld.N) return;
     int r = P.i32[ld.ref_off + i];
-    spin4<R> s = reinterpret_cast<const spin4<R> *>(spins)[sd.spin_off + i];
+    spin4<R> s = spins_at<R>(spins, sd.spin_off).ld(i);
     dst[3 * (int64_t)r] = s.x;
     dst[3 * (int64_t)r + 1] = s.y;
     dst[3 * (int64_t)r + 2] = s.z;
@@ -825,8 +825,8 @@ hipError_t launch_gather_spins(int prec, const DevPools &P, const void *spins, i
 // contributes (O + O^T) s and Zeeman -h.
 template <class R>
 __device__ __forceinline__ void site_gradient(const DevPools &P, const LatDev &ld, const LatView<double> &L,
-                                              const spin4<R> *S, int x, double g[3]) {
-    auto sp = [&](int j) { return to_d<R>(S[j]); };
+                                              const SpinsG<R> &S, int x, double g[3]) {
+    auto sp = [&](int j) { return to_d<R>(S.ld(j)); };
     if (ld.flags & LAT_GRAD_LF) {
         // the gradient is the local field (the host checked the pairing of
         // every bond with its transpose): one gather over x's own bonds
@@ -928,7 +928,7 @@ __global__ __launch_bounds__(256) void k_grad_spins(DevPools P, const void *spin
     int x = blk * blockDim.x + threadIdx.x;
     if (x >= ld.N) return;
     const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
-    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
+    const SpinsG<R> S = spins_at<R>(spins, sd.spin_off);
     double g[3];
     site_gradient<R>(P, ld, L, S, x, g);
     double *o = dEdS + 3 * (sd.site_off + L.ref[x]);
@@ -959,8 +959,8 @@ __global__ __launch_bounds__(1024) void k_grad_couplings(DevPools P, const void 
     const SysDev sd = P.sys[sys];
     const LatDev ld = P.lats[sd.lat];
     const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
-    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins) + sd.spin_off;
-    auto sp = [&](int j) { return to_d<R>(S[j]); };
+    const SpinsG<R> S = spins_at<R>(spins, sd.spin_off);
+    auto sp = [&](int j) { return to_d<R>(S.ld(j)); };
     for (int k = threadIdx.x; k < grad_len; k += blockDim.x) acc[k] = 0.0;
     __syncthreads();
     const int nb = L.nb;
@@ -1085,8 +1085,8 @@ __global__ __launch_bounds__(256) void k_gd(DevPools P, void *spins, const doubl
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ld.N) return;
     int r = P.i32[ld.ref_off + i];
-    spin4<R> *S = reinterpret_cast<spin4<R> *>(spins) + sd.spin_off;
-    spin4<R> s = S[i];
+    const SpinsG<R> S = spins_at<R>(spins, sd.spin_off);
+    spin4<R> s = S.ld(i);
     const double *g = dEdS + 3 * (sd.site_off + r);
     double sx = s.x, sy = s.y, sz = s.z;
     double n = sqrt(sx * sx + sy * sy + sz * sz);
@@ -1098,7 +1098,7 @@ __global__ __launch_bounds__(256) void k_gd(DevPools P, void *spins, const doubl
     s.x = (R)(nx / nn * ld.S);
     s.y = (R)(ny / nn * ld.S);
     s.z = (R)(nz / nn * ld.S);
-    S[i] = s;
+    S.st(i, s);
 }
 
 // one projected gradient step of every site from the old spins into a
@@ -1114,10 +1114,10 @@ __global__ __launch_bounds__(256) void k_gd_step(DevPools P, const void *spins_o
     int i = blk * blockDim.x + threadIdx.x;
     if (i >= ld.N) return;
     const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
-    const spin4<R> *S = reinterpret_cast<const spin4<R> *>(spins_old) + sd.spin_off;
+    const SpinsG<R> S = spins_at<R>(spins_old, sd.spin_off);
     double g[3];
     site_gradient<R>(P, ld, L, S, i, g);
-    spin4<R> s = S[i];
+    spin4<R> s = S.ld(i);
     double sx = s.x, sy = s.y, sz = s.z;
     double n = sqrt(sx * sx + sy * sy + sz * sz);
     double ux = sx / n, uy = sy / n, uz = sz / n;
@@ -1128,7 +1128,7 @@ __global__ __launch_bounds__(256) void k_gd_step(DevPools P, const void *spins_o
     s.x = (R)(nx / nn * ld.S);
     s.y = (R)(ny / nn * ld.S);
     s.z = (R)(nz / nn * ld.S);
-    reinterpret_cast<spin4<R> *>(spins_new)[sd.spin_off + i] = s;
+    spins_at<R>(spins_new, sd.spin_off).st(i, s);
 }
 
 hipError_t launch_gd_step(int prec, const DevPools &P, const void *spins_old, void *spins_new, int nsys, int maxN,

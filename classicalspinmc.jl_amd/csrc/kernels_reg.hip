@@ -242,13 +242,13 @@ __global__ __launch_bounds__(REG_THREADS) void k_wg_reg(WGArgs a) {
     const int4 *runs = a.P.runs + ld.run_off;
     const LatView<float> L = lat_view<float>(ld, a.P.i32, runs, a.P.coefF);
     const LatView<double> Ld = lat_view<double>(ld, a.P.i32, runs, a.P.coefD);
-    spin4<float> *G = reinterpret_cast<spin4<float> *>(a.spins) + sd.spin_off;
+    const SpinsG<float> G = spins_at<float>(a.spins, sd.spin_off);
     const int N = L.N;
 
     const uint32_t base = lds_address(smem + RED_BYTES);
     SpinsV4 X;
     // stage the system and its zero pad site N into LDS
-    wg_stage<8>(N + 1, [&](int i) { return G[i]; },
+    wg_stage<8>(N + 1, [&](int i) { return G.ld(i); },
                 [&](int i, const spin4<float> &v) { X.st(base + 16u * (uint32_t)i, v); });
     SweeperR<K2C, SPT, DIAG, SPC> sw(L, Ld, base, red,
                                 reinterpret_cast<const int32_t *>(a.P.runs + ld.slot_off),
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(REG_THREADS) void k_wg_reg(WGArgs a) {
     // PT state, not the spins stored below
     if (MODE == WG_PT && a.exchange) pt_exchange_last(a, reinterpret_cast<int *>(red + RED_FLAG));
     __syncthreads();
-    for (int i = threadIdx.x; i < N; i += REG_THREADS) G[i] = X.ld(base + 16u * (uint32_t)i);
+    for (int i = threadIdx.x; i < N; i += REG_THREADS) G.st(i, X.ld(base + 16u * (uint32_t)i));
 }
 
 template <int K2C, int SPT, bool DIAG, int MODE, int SPC>

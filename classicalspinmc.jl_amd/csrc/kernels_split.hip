@@ -322,7 +322,7 @@ __global__ __launch_bounds__(REG_THREADS) void k_wg_split(WGArgs a) {
     const int4 *runs = a.P.runs + ld.run_off;
     const LatView<float> L = lat_view<float>(ld, a.P.i32, runs, a.P.coefF);
     const LatView<double> Ld = lat_view<double>(ld, a.P.i32, runs, a.P.coefD);
-    spin4<float> *G = reinterpret_cast<spin4<float> *>(a.spins) + sd.spin_off;
+    const SpinsG<float> G = spins_at<float>(a.spins, sd.spin_off);
     const int32_t *tab = a.split.tab;
     const int32_t *hdr = tab + part * SPLIT_HDR;
     const int n_loc = ldc(hdr + 0);
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(REG_THREADS) void k_wg_split(WGArgs a) {
     SpinsV4 X;
     // stage the local sites and the zero pad site n_loc, then the boundary
     // list and the halo pairs
-    wg_stage<8>(n_loc + 1, [&](int i) { return i < n_loc ? G[locdev[i]] : spin4<float>{0, 0, 0, 0}; },
+    wg_stage<8>(n_loc + 1, [&](int i) { return i < n_loc ? G.ld(locdev[i]) : spin4<float>{0, 0, 0, 0}; },
                 [&](int i, const spin4<float> &v) { X.st(base + 16u * (uint32_t)i, v); });
     const int n_bnd = ldc(hdr + 3), n_halo = ldc(hdr + 4);
     const uint32_t bnd_l = base + 16u * (uint32_t)(n_loc + 1);
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(REG_THREADS) void k_wg_split(WGArgs a) {
     for (int c = 0; c < L.ncol; ++c) {
         const int cs = ldc(colstart + c), n = ldc(owncnt + c);
         for (int i = threadIdx.x; i < n; i += REG_THREADS)
-            G[locdev[cs + i]] = X.ld(base + 16u * (uint32_t)(cs + i));
+            G.st(locdev[cs + i], X.ld(base + 16u * (uint32_t)(cs + i)));
     }
 }
 

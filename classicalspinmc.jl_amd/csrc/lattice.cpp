@@ -578,7 +578,7 @@ int greedy(long N, const std::vector<long> &off, const std::vector<int> &adj,
 // term partner (basis bj at offset o from basis b) gets the same colour,
 // i.e. (c.o + e_bj - e_b) mod p != 0, and, for periodic boundaries, when
 // it is well defined on the torus (p | c_d L_d).  Returns p, or 0.
-int pattern_colour(const HostLattice &L, std::vector<int> &col) {
+int pattern_colour(const HostLattice &L, std::vector<int> &col, ColourPattern &pat) {
     const int D = L.dim, nb = L.nb;
     struct Con {
         int b, bj, o[3];
@@ -662,6 +662,10 @@ int pattern_colour(const HostLattice &L, std::vector<int> &col) {
             for (int q = 0; q < p; ++q)
                 if (used[q]) remap[q] = n++;
             for (long s = 0; s < L.N; ++s) col[s] = remap[col[s]];
+            pat.p = p;
+            for (int d = 0; d < 3; ++d) pat.c[d] = d < D ? c[d] : 0;
+            pat.e = e;
+            pat.remap = remap;
             return n;
         }
     }
@@ -677,7 +681,9 @@ void colour_lattice(HostLattice &L) {
     std::vector<int> best;
     int nbest;
     std::vector<int> pat;
-    int npat = pattern_colour(L, pat);
+    ColourPattern cp;
+    int npat = pattern_colour(L, pat, cp);
+    L.pattern = ColourPattern{};
     if (off[L.N] == 0) {
         best.assign(L.N, 0);
         nbest = 1;
@@ -696,6 +702,7 @@ void colour_lattice(HostLattice &L) {
         if (npat > 0 && npat <= nbest) {
             best.swap(pat);
             nbest = npat;
+            L.pattern = cp;
         }
     }
     L.colour = best;

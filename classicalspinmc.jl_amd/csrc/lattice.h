@@ -11,6 +11,14 @@
 
 namespace csmc {
 
+// A lattice-periodic colouring colour(b, x) = remap[(c . x + e_b) mod p]
+// (p = 0: the colouring is not of this form)
+struct ColourPattern {
+    int p = 0;
+    int c[3] = {0, 0, 0};
+    std::vector<int> e, remap;
+};
+
 struct HostLattice {
     int dim = 0, nb = 0, periodic = 1;
     int shape[3] = {1, 1, 1};
@@ -44,6 +52,7 @@ struct HostLattice {
     std::vector<int> dev_of_ref, ref_of_dev;  // permutations
     std::vector<std::array<int, 4>> runs;    // {start, end, basis, colour}
     std::vector<int> colrun;                 // [ncol+1] run boundaries
+    ColourPattern pattern;                   // how `colour` was made, if periodic
     int has_self = 0;
 
     // gradient layout: field calls*3, onsite calls*9, n2*9, n3*27, n4*81

@@ -51,7 +51,7 @@ __global__ void k_ssf_spins(const void *spins, const SysDev *sys, int nsys, int 
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nsys * N) return;
     const int r = (int)(t / N), i = (int)(t % N);
-    const spin4<R> s = reinterpret_cast<const spin4<R> *>(spins)[sys[r].spin_off + i];
+    const spin4<R> s = spins_at<R>(spins, sys[r].spin_off).ld(i);
     B[i + (int64_t)N * (3 * r + 0)] = (double)s.x;
     B[i + (int64_t)N * (3 * r + 1)] = (double)s.y;
     B[i + (int64_t)N * (3 * r + 2)] = (double)s.z;
@@ -86,7 +86,7 @@ __global__ void k_ssf_spins_rows(const void *spins, const SysDev *sys, const int
     const int i = (int)(t / ldb), c = (int)(t % ldb);
     double v = 0.0;
     if (i < N && c < 3 * nsys) {
-        const spin4<R> s = reinterpret_cast<const spin4<R> *>(spins)[sys[c / 3].spin_off + perm[i]];
+        const spin4<R> s = spins_at<R>(spins, sys[c / 3].spin_off).ld(perm[i]);
         const int a = c % 3;
         v = (double)(a == 0 ? s.x : a == 1 ? s.y : s.z);
     }

@@ -130,6 +130,21 @@ int32_t csmc_build_tables(const csmc_unit_cell *uc, const int32_t *shape, int32_
  * hold 7 + 4 * parts values.  CSMC_EINVAL if the lattice cannot be split so. */
 int32_t csmc_split_plan(const csmc_unit_cell *uc, const int32_t *shape, int32_t periodic,
                         int32_t parts, int32_t or_sweeps, int64_t *stats);
+/* Host only: the plan of the tiled batch-wide sweeps of a lattice (tiles of
+ * cells swept from LDS with partners found by index arithmetic).  Every
+ * term of a lattice built from a unit cell joins cells at fixed offsets
+ * under the periodic mod of lattice.jl:101-109 (tables of
+ * lattice.jl:176-206); a plan exists when the lattice is periodic, has
+ * bilinear terms only, and its colouring is lattice-periodic.  head receives
+ * CSMC_TILE_HEAD ints: D, nb, K2, p, cz, cinv, ncol, ntiles, L[3], T[3],
+ * lo[3], B[3], nt[3], c[3], boxcells, nbox, tcells, ncell, tab_off, pad
+ * (all zero and the reason in csmc_last_error when there is no plan); tab
+ * receives tab_len ints e[nb], runbase[p nb], delta[nb K2], rect[ncol nb 6],
+ * rawc[ncol] when tab_cap is large enough.  prec: 32 or 64 (the record size
+ * in LDS); lds_budget: LDS bytes a tile may use. */
+#define CSMC_TILE_HEAD 32
+int32_t csmc_tile_plan(const csmc_unit_cell *uc, const int32_t *shape, int32_t periodic, int32_t prec,
+                       int64_t lds_budget, int32_t *head, int32_t *tab, int32_t tab_cap, int32_t *tab_len);
 /* Lattice(shape, uc, S; bc) (lattice.jl:65-291): builds the per-site tables,
  * colours the interaction hypergraph and uploads device tables.
  * periodic: 1 = "periodic", 0 = "open", other = CSMC_EBC. */
@@ -259,13 +274,23 @@ int32_t csmc_pt_state(csmc_ctx *ctx, int32_t batch_id, int32_t *system_of_slot, 
  * by CSMC_PARTS0=1..8).  No counterpart in the reference (its replicas are
  * MPI ranks). */
 int32_t csmc_pt_split_parts(csmc_ctx *ctx, int32_t batch_id, int32_t *parts);
-/* The plain form over the whole batch: one launch per colour pass covering
- * every system (one thread per site), the Philox keys and counters of
- * csmc_sweep_plain, so the results are the same bit for bit.  For systems
- * too large for one workgroup to sweep well; not for the adaptive cone.
- * With csmc_timing on, events bracket the passes. */
+/* The plain form over the whole batch: the Philox keys and counters of
+ * csmc_sweep_plain, so the results are the same bit for bit.  When every
+ * lattice of the batch has a tile plan (csmc_batch_tiles), one launch per
+ * sweep sweeps LDS tiles of every system (index-free partners, one read and
+ * one write of each spin record per sweep); otherwise one launch per colour
+ * pass covers every system (one thread per site).  For systems too large
+ * for one workgroup to sweep well; not for the adaptive cone.  With
+ * csmc_timing on, events bracket the sweeps. */
 int32_t csmc_sweep_passes(csmc_ctx *ctx, int32_t batch_id, int32_t kind, const double *T, const double *sigma,
                           int32_t n_sweeps, uint64_t seed, double *accepted);
+/* Tiles per system of the batch-wide sweeps (csmc_sweep_passes): > 0 when
+ * every lattice of the batch has a tile plan (csmc_tile_plan), so that a
+ * sweep runs as one launch of LDS tiles reading one spin buffer and writing
+ * the other; 0 when the sweeps run as one launch per colour pass (the reason
+ * in csmc_last_error).  No reference counterpart: a query of this engine's
+ * schedule. */
+int32_t csmc_batch_tiles(csmc_ctx *ctx, int32_t batch_id, int32_t *tiles);
 
 /* Autotuner (no reference counterpart): time n_sweeps sweeps of `kind` (T
  * for the Metropolis kinds) from the batch's current state in each form --

@@ -1,0 +1,276 @@
+"""Host plan of the tiled batch-wide sweeps (csmc_tile_plan, no device
+needed) and a CPU replay of the tile kernel's index arithmetic.
+
+The kernel (tile.hip) sweeps a lattice tile by tile from LDS: it stages a box
+of cells around the tile from the device-order spin records, runs every
+colour step over the step's region (the tile plus the halo sites later steps
+read), finds partners by adding a per-(basis, slot) constant to the box
+index, and stores the tile.  `replay_tiles` below walks the same loops with
+the same integer arithmetic in Python and applies, at each update, an
+order-independent hash of the site's reference index, its value and its
+partners' values.  The result must equal the colour-pass schedule over the
+reference tables (lattice.jl:15-22: partners from bsite, colours from the
+colouring) bit for bit, and every index the kernel would form must stay in
+its buffer.  A tile plan that dropped a halo site, used a wrong partner
+offset, or mapped a cell to the wrong device record fails here."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from csmc import _lib, workloads as W
+from csmc.lattice import Lattice
+from csmc.unitcell import Pyrochlore, addBasisSite, addBilinear
+
+HEAD = ["D", "nb", "K2", "p", "cz", "cinv", "ncol", "ntiles"]
+TILE_PAD = 0x7FFFFFFF
+M64 = (1 << 64) - 1
+
+
+def tile_plan(uc, shape, prec=32, budget=48 << 10, periodic=1):
+    if len(uc.basis) == 0:  # the Lattice constructor's default site (lattice.jl:68-70)
+        addBasisSite(uc, np.zeros(uc.D))
+    s, keep = uc._pack()
+    sh = np.ascontiguousarray(np.array(shape, np.int32))
+    head = np.zeros(32, np.int32)
+    n = C.c_int32(0)
+    _lib.check(_lib.lib().csmc_tile_plan(C.byref(s), _lib.ptr(sh), periodic, prec, budget, _lib.ptr(head), None,
+                                         0, C.byref(n)))
+    if head[1] == 0:
+        return None, _lib.lib().csmc_last_error().decode()
+    tab = np.zeros(n.value, np.int32)
+    _lib.check(_lib.lib().csmc_tile_plan(C.byref(s), _lib.ptr(sh), periodic, prec, budget, _lib.ptr(head),
+                                         _lib.ptr(tab), n.value, C.byref(n)))
+    t = {k: int(head[i]) for i, k in enumerate(HEAD)}
+    for i, k in enumerate(["L", "T", "lo", "B", "nt", "c"]):
+        t[k] = [int(v) for v in head[8 + 3 * i: 11 + 3 * i]]
+    t.update(boxcells=int(head[26]), nbox=int(head[27]), tcells=int(head[28]), ncell=int(head[29]))
+    nb, p, K2, ncol = t["nb"], t["p"], t["K2"], t["ncol"]
+    o = 0
+    t["eb"] = tab[o:o + nb].tolist(); o += nb
+    t["runbase"] = tab[o:o + p * nb].tolist(); o += p * nb
+    t["delta"] = tab[o:o + nb * K2].tolist(); o += nb * K2
+    t["rect"] = tab[o:o + ncol * nb * 6].tolist(); o += ncol * nb * 6
+    t["rawc"] = tab[o:o + ncol].tolist(); o += ncol
+    assert o == len(tab)
+    return t, ""
+
+
+def mix(v):
+    """splitmix64 finaliser on Python ints (mod 2^64)"""
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9 & M64
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EB & M64
+    return v ^ (v >> 31)
+
+
+def update(ref, own, partners):
+    """a stand-in for one site update: depends on the site, its value and the
+    multiset of its partners' values"""
+    acc = (own * 0x9E3779B97F4A7C15 + ref * 0xD6E8FEB86659FD93) & M64
+    for v in partners:
+        acc = (acc + mix(v)) & M64
+    return mix(acc)
+
+
+def device_order(colour, ncell):
+    """device order (colour, basis, reference index) of the lattice"""
+    N = len(colour)
+    ref = np.arange(N)
+    order = np.lexsort((ref, ref // ncell, colour))
+    dev_of_ref = np.empty(N, np.int64)
+    dev_of_ref[order] = np.arange(N)
+    return order, dev_of_ref
+
+
+def pass_schedule(vals, bsite, colour, ncol, sweeps):
+    """the colour-pass schedule on reference-indexed values"""
+    vals = list(vals)
+    for _ in range(sweeps):
+        for col in range(ncol):
+            new = {}
+            for s in np.flatnonzero(colour == col):
+                new[s] = update(int(s), vals[s], [vals[j - 1] for j in bsite[s] if j > 0])
+            for s, v in new.items():
+                vals[s] = v
+    return vals
+
+
+def imod(a, m):
+    r = a % m  # Python's % is already non-negative for m > 0
+    return r
+
+
+def replay_tiles(t, G, N, sweeps):
+    """the loops of k_tile over every tile of one system, reading G (device
+    order) and writing a second buffer, once per sweep"""
+    nb, K2, p, cz, cinv = t["nb"], t["K2"], t["p"], t["cz"], t["cinv"]
+    Lx, T, lo, B, nt, c = t["L"], t["T"], t["lo"], t["B"], t["nt"], t["c"]
+    boxcells, nbox, tcells = t["boxcells"], t["nbox"], t["tcells"]
+    eb, runbase, delta, rect, rawc = t["eb"], t["runbase"], t["delta"], t["rect"], t["rawc"]
+
+    def dev_index(b, x):
+        ci = (x[0] * Lx[1] + x[1]) * Lx[2] + x[2]
+        if cz:
+            d = runbase[eb[b] * nb + b] + ci
+        else:
+            r = imod(eb[b] + c[0] * x[0] + c[1] * x[1] + c[2] * x[2], p)
+            d = runbase[r * nb + b] + (ci // Lx[2]) * (Lx[2] // p) + x[2] // p
+        assert 0 <= d < N
+        return d
+
+    def ref_of(b, x):
+        return ((b * Lx[0] + x[0]) * Lx[1] + x[1]) * Lx[2] + x[2]
+
+    for _ in range(sweeps):
+        out = [None] * N
+        for tile in range(t["ntiles"]):
+            org, r = [0, 0, 0], tile
+            for d in (2, 1, 0):
+                q = r // nt[d]
+                org[d] = (r - q * nt[d]) * T[d]
+                r = q
+            X = [0] * (nbox + 1)  # index nbox: the zero pad site
+            for li in range(nbox):
+                b = li // boxcells
+                cc, x = li - b * boxcells, [0, 0, 0]
+                for d in (2, 1, 0):
+                    q = cc // B[d]
+                    v = org[d] + lo[d] + (cc - q * B[d])
+                    v += Lx[d] if v < 0 else 0
+                    v -= Lx[d] if v >= Lx[d] else 0
+                    x[d] = v
+                    cc = q
+                X[li] = G[dev_index(b, x)]
+            for col in range(t["ncol"]):
+                r0 = rawc[col]
+                for b in range(nb):
+                    rc = rect[(col * nb + b) * 6:(col * nb + b) * 6 + 6]
+                    lo0, hi0, lo1, hi1, lo2, hi2 = rc
+                    if lo0 >= hi0 or lo1 >= hi1 or lo2 >= hi2:
+                        continue
+                    w2 = hi2 - lo2 if cz else (hi2 - lo2 + p - 1) // p
+                    n1 = hi1 - lo1
+                    new = {}
+                    for w in range((hi0 - lo0) * n1 * w2):
+                        row, j = w // w2, w % w2
+                        q1 = row // n1
+                        u = [lo0 + q1, lo1 + (row - q1 * n1), lo2 + j]
+                        if not cz:
+                            s = r0 - eb[b] - c[0] * (lo[0] + u[0]) - c[1] * (lo[1] + u[1])
+                            u[2] = lo2 + imod(s * cinv - lo[2] - lo2, p) + j * p
+                            if u[2] >= hi2:
+                                continue
+                        li = b * boxcells + (u[0] * B[1] + u[1]) * B[2] + u[2]
+                        assert 0 <= li < nbox
+                        parts = []
+                        for k in range(K2):
+                            dl = delta[b * K2 + k]
+                            if dl == TILE_PAD:
+                                continue
+                            assert 0 <= li + dl < nbox, (li, dl)
+                            parts.append(X[li + dl])
+                        x = [(org[d] + u[d] + lo[d]) % Lx[d] for d in range(3)]
+                        new[li] = update(ref_of(b, x), X[li], parts)
+                    for li, v in new.items():  # one colour class: independent updates
+                        X[li] = v
+            for oi in range(nb * tcells):
+                b = oi // tcells
+                cc, x, u = oi - b * tcells, [0, 0, 0], [0, 0, 0]
+                for d in (2, 1, 0):
+                    q = cc // T[d]
+                    v = cc - q * T[d]
+                    u[d] = v - lo[d]
+                    x[d] = org[d] + v
+                    cc = q
+                li = b * boxcells + (u[0] * B[1] + u[1]) * B[2] + u[2]
+                d = dev_index(b, x)
+                assert out[d] is None, "a site stored twice"
+                out[d] = X[li]
+        assert all(v is not None for v in out), "a site not stored"
+        G = out
+    return G
+
+
+def pyrochlore_heisenberg():
+    uc = Pyrochlore()
+    for b1, b2, o in W.PYRO_BONDS:
+        addBilinear(uc, b1, b2, np.eye(3), o)
+    return uc
+
+
+CASES = [
+    ("honeycomb_64x64", lambda: W.honeycomb_kitaev(1.0, -2.0), (64, 64)),
+    ("honeycomb_40x8", lambda: W.honeycomb_kitaev(1.0, -2.0), (40, 8)),
+    ("honeycomb_4x4", lambda: W.honeycomb_kitaev(1.0, -2.0), (4, 4)),  # the box is the whole lattice
+    ("square_64x64", lambda: W.square_heisenberg(), (64, 64)),
+    ("triangular_dm_48x48", lambda: W.triangular_dm(-1.0, 0.1, (0.0, 0.0, 0.2)), (48, 48)),
+    ("pyrochlore_8", pyrochlore_heisenberg, (8, 8, 8)),
+]
+
+
+@pytest.mark.parametrize("name,make,shape", CASES, ids=[c[0] for c in CASES])
+def test_tile_replay_equals_colour_passes(name, make, shape):
+    uc = make()
+    t, why = tile_plan(uc, shape, budget=16 << 10)
+    assert t is not None, why
+    assert t["ntiles"] >= 2, "the case must cut the lattice into several tiles"
+    lat = Lattice(shape, uc, 0.5, seed=1)
+    tb = lat.tables()
+    N = lat.size
+    colour = np.asarray(tb["colour"])
+    order, dev_of_ref = device_order(colour, t["ncell"])
+    rng = np.random.default_rng(3)
+    vals = [int(v) for v in rng.integers(0, 1 << 62, size=N)]
+    want = pass_schedule(vals, np.asarray(tb["bsite"]), colour, t["ncol"], sweeps=2)
+    G = [0] * N
+    for s in range(N):
+        G[dev_of_ref[s]] = vals[s]
+    got = replay_tiles(t, G, N, sweeps=2)
+    assert all(got[dev_of_ref[s]] == want[s] for s in range(N))
+
+
+def test_planted_halo_mistake_is_caught():
+    """the replay notices a plan whose update region misses one halo row"""
+    uc = W.square_heisenberg()
+    t, _ = tile_plan(uc, (32, 32), budget=16 << 10)
+    lat = Lattice((32, 32), uc, 0.5, seed=1)
+    tb = lat.tables()
+    colour = np.asarray(tb["colour"])
+    order, dev_of_ref = device_order(colour, t["ncell"])
+    vals = [int(v) for v in np.random.default_rng(4).integers(0, 1 << 62, size=lat.size)]
+    want = pass_schedule(vals, np.asarray(tb["bsite"]), colour, t["ncol"], sweeps=1)
+    G = [0] * lat.size
+    for s in range(lat.size):
+        G[dev_of_ref[s]] = vals[s]
+    # colour 0's region one row shorter at its low end
+    rect = list(t["rect"])
+    rect[2] += 1
+    bad = dict(t, rect=rect)
+    got = replay_tiles(bad, G, lat.size, sweeps=1)
+    assert any(got[dev_of_ref[s]] != want[s] for s in range(lat.size))
+
+
+def test_plan_geometry():
+    """tile and box sizes that the geometry fixes"""
+    th, _ = tile_plan(W.honeycomb_kitaev(1.0, -2.0), (512, 512))
+    # the colour of a honeycomb site is its basis; each site's partners are
+    # one cell away at most, so the box is the 32 x 32 tile plus one cell
+    # on each side
+    assert th["cz"] == 1 and th["ncol"] == 2 and th["T"] == [1, 32, 32]
+    assert th["B"] == [1, 34, 34] and th["lo"] == [0, -1, -1] and th["ntiles"] == 256
+    assert th["nbox"] == 2 * 34 * 34
+    # square lattice, checkerboard colouring c = (1, 1) mod 2: the colour-0
+    # steps reach one cell past the tile and read one more
+    t, _ = tile_plan(W.square_heisenberg(), (64, 64))
+    assert t["cz"] == 0 and t["p"] == 2 and t["c"][1:] == [1, 1]
+    assert t["lo"][1:] == [-2, -2]
+    # FP64 records take twice the LDS: the tile shrinks under the same budget
+    t64, _ = tile_plan(W.honeycomb_kitaev(1.0, -2.0), (512, 512), prec=64)
+    assert (t64["nbox"] + 1) * 24 <= 48 << 10 and t64["nbox"] < th["nbox"]
+
+
+def test_no_plan_reasons():
+    assert tile_plan(W.pyrochlore_mixed(), (8, 8, 8))[1] == "cubic or quartic terms"
+    assert tile_plan(W.honeycomb_kitaev(1.0, -2.0), (64, 64), periodic=0)[1] == "open boundaries"
+    # too small for a box that does not wrap onto itself
+    assert tile_plan(W.honeycomb_kitaev(1.0, -2.0), (2, 2))[0] is None
