@@ -62,16 +62,16 @@ CHIP_CUS = 256
 CLOCK_HZ = 2.4e9
 VALU_LANES_PER_CU_CLK = 64
 LDS_BYTES_PER_CU_CLK = 128
-LDS_BYTES_PER_LANE_INST = 16  # the sweep kernels' LDS traffic is 16-byte spin records
+LDS_BYTES_PER_LANE_INST = 16  # the register-table kernel's LDS traffic is 16-byte spin records
 
 
-def issue_fractions(sq, updates_per_dispatch, value, cus_busy):
+def issue_fractions(sq, updates_per_dispatch, value, cus_busy, lds_bytes_per_lane_inst=LDS_BYTES_PER_LANE_INST):
     """VALU and LDS fractions of the register-table kernel from its counters
     (SQ_INSTS_VALU, SQ_INSTS_LDS per dispatch, wave instructions, 64 lanes
     each) at the live throughput `value` (updates/s): against the whole chip
     and against the CUs the kernel occupies."""
     valu_per_update = sq["SQ_INSTS_VALU"] * 64 / updates_per_dispatch
-    lds_per_update = sq["SQ_INSTS_LDS"] * 64 * LDS_BYTES_PER_LANE_INST / updates_per_dispatch
+    lds_per_update = sq["SQ_INSTS_LDS"] * 64 * lds_bytes_per_lane_inst / updates_per_dispatch
     valu = valu_per_update * value
     lds = lds_per_update * value
     vpeak = VALU_LANES_PER_CU_CLK * CLOCK_HZ
@@ -537,12 +537,12 @@ def bench_pyrochlore(args, csmc, W):
 
 def bench_probe(args, csmc, W):
     """Roofline probe: the headline geometry at L=512 (524,288 sites) x 64
-    FP32 replicas, 537 MB of spin records -- more than the 256 MB Infinity
-    Cache, so every sweep streams the state from HBM.  One step = 10
+    FP32 replicas, 403 MB of packed spin records -- more than the 256 MB
+    Infinity Cache, so every sweep streams the state from HBM.  One step = 10
     overrelaxation sweeps + 1 Metropolis sweep of every replica in the
-    batch-wide form (csmc_sweep_passes: one launch per colour pass over all
-    replicas, one thread per site); the persistent one-workgroup-per-replica
-    kernel on the same state is timed beside it."""
+    batch-wide form (csmc_sweep_passes: one launch per sweep of LDS tiles with
+    recomputed halos, partners by index arithmetic); the persistent kernel
+    (storage mode 0, packed schedule) on the same state is timed beside it."""
     L, R = 512, 64
     uc = W.honeycomb_kitaev(1.0, -2.0)
     lat = csmc.Lattice((L, L), uc, 0.5, seed=2)
@@ -566,12 +566,23 @@ def bench_probe(args, csmc, W):
     updates = steps * 11 * R * N
     value = updates / dt
     achieved = BYTES_PER_UPDATE_FP32 * updates / (kern_ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_pass_probe_honeycomb_L512_R64_traffic.json")
+    tiles, why = b.tiles()
+    traffic, issue = None, {}
+    tpath = os.path.join(ROOT, "profiles", "k_tile_probe_honeycomb_L512_R64_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f)["traffic_bytes_per_update"] * R * N
-    # the persistent kernel (one workgroup per replica) on the same state
+            tj = json.load(f)
+        traffic = tj["traffic_bytes_per_update"] * R * N
+        # the overrelaxation kernel's VALU issue at its own live rate (10 of
+        # the 11 sweeps of a step)
+        ms_or = tile_or_ms(csmc, b, ctx)
+        issue = issue_fractions(tj["sq_per_dispatch"], tj["updates_per_dispatch"], R * N / (ms_or * 1e-3),
+                                CHIP_CUS, lds_bytes_per_lane_inst=4)
+        issue["issue_note"] = ("instruction counts of the overrelaxation tile kernel from its SQ counters "
+                               "(profiles/), times its live updates/s (%.1f us per sweep): the kernel is "
+                               "VALU-issue bound" % (ms_or * 1e3))
+    # the persistent kernel on the same state (storage mode 0: the packed
+    # schedule splits each replica into parts on one XCD)
     b.sweep(csmc.OVERRELAX, n_sweeps=1)
     ctx.sync()
     t0 = time.perf_counter()
@@ -584,21 +595,37 @@ def bench_probe(args, csmc, W):
            "ms_per_step": dt / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "fp32", "data": "synthetic",
            "config": {"workload": f"roofline probe: honeycomb Kitaev-Heisenberg J=1 K=-2, L={L} ({N} spins), "
-                                  f"{R} FP32 replicas ({R * N * 16 / 1e6:.0f} MB of spin records), 10 "
-                                  "overrelaxation + 1 Metropolis sweep per step, batch-wide colour passes",
-                      "sites": N, "replicas": R, "storage_mode": b.storage_mode()},
-           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                        "kernel": "k_pass<float> over all replicas (one launch per colour pass)",
-                        "note": "traffic = measured HBM bytes per sweep of all replicas"},
+                                  f"{R} FP32 replicas ({R * (N + 1) * 12 / 1e6:.0f} MB of spin records), 10 "
+                                  "overrelaxation + 1 Metropolis sweep per step, batch-wide tiled sweeps",
+                      "sites": N, "replicas": R, "storage_mode": b.storage_mode(), "tiles_per_replica": tiles},
+           "roofline": dict({"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                             "kernel": "k_tile<float> over all replicas (one launch per sweep)" if tiles else
+                                       f"k_pass<float> over all replicas (one launch per colour pass: {why})",
+                             "note": "traffic = measured HBM bytes per sweep of all replicas (FETCH_SIZE + "
+                                     "WRITE_SIZE of the overrelaxation tile kernel, copy-kernel scaled)"},
+                            **issue),
            "persistent_kernel": {"value": persistent, "unit": "spin-updates/s",
-                                 "form": "one workgroup per replica (64 CUs), overrelaxation sweeps"},
+                                 "form": "persistent kernel, storage mode 0, packed schedule (parts of a replica "
+                                         "on one XCD, spins gathered from device memory), overrelaxation sweeps"},
            "autotune": {"ms_for_2_overrelaxation_sweeps": {k: round(v, 3) for k, v in tune_times.items()},
                         "chosen": tune_choice}}
     b.close()
     log(f"[bench] probe: {value / 1e9:.2f} G updates/s, {achieved:.0f} GB/s algorithmic; persistent "
         f"{persistent / 1e9:.2f} G updates/s")
     finish(out, args)
+
+
+def tile_or_ms(csmc, b, ctx, sweeps=20):
+    """average ms of one overrelaxation sweep of the batch-wide form, timed
+    with the library's events around the launches"""
+    b.sweep_passes(csmc.OVERRELAX, n_sweeps=2)
+    ctx.sync()
+    ctx.timing(True)
+    b.sweep_passes(csmc.OVERRELAX, n_sweeps=sweeps)
+    ms, _ = ctx.timing_read()
+    ctx.timing(False)
+    return ms / sweeps
 
 
 def bench_anneal_square(args, csmc, W):
