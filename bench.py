@@ -525,12 +525,35 @@ def bench_pyrochlore(args, csmc, W):
     args.warmup = min(args.warmup, 20)
     r = run_pt(args, W, csmc, W.pyrochlore_mixed(), 16, 1.0, R, temps, 32, "pt_pyrochlore")
     value = r["value"]
+    # roofline of the PT kernel: algorithmic bytes of a launch over its
+    # average duration, beside its VALU issue (the counters per exchange
+    # period come from profiles/)
+    ups = r["updates_per_step"]
+    steps_per_launch = steps / r["launches"]
+    avg_ms = r["kern_ms"] / r["launches"]
+    achieved = BYTES_PER_UPDATE_FP32 * ups * steps_per_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "kernel": "k_wg (storage mode 0, chained PT, parts of a replica on one XCD)",
+                "avg_launch_ms": round(avg_ms, 4), "steps_per_launch": steps_per_launch}
+    tpath = os.path.join(ROOT, "profiles", "k_wg_pt_pyrochlore_L16_R32_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            prof = json.load(f)
+        roofline["traffic"] = prof["traffic_bytes_per_step"] * steps_per_launch
+        parts = max(1, r.get("parts", 1))
+        roofline.update(issue_fractions(prof["sq_per_dispatch"], ups * prof["steps_per_dispatch"], value,
+                                        min(CHIP_CUS, R * parts), lds_bytes_per_lane_inst=4))
+        roofline["issue_note"] = ("instruction counts per update from the kernel's SQ counters (profiles/), "
+                                  "times the live updates/s; the spins stay in device memory (L2), so the "
+                                  "gathers' latency and VALU issue on the busy CUs bound it")
     out = {"metric": METRIC, "value": value, "unit": "spin-updates/s", "n_gpus": 1, "steps": steps,
            "warmup": args.warmup, "ms_per_step": r["dt"] / steps * 1e3, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
            "config": {"workload": "pyrochlore L=16 (16384 spins), J1 + biquadratic + chirality + onsite, "
                                   "parallel tempering, 32 replicas, T=0.02..2",
-                      "storage_mode": r["mode"]}}
+                      "storage_mode": r["mode"], "parts_per_replica": r.get("parts", 1)},
+           "roofline": roofline}
     r["b"].close()
     finish(out, args)
 

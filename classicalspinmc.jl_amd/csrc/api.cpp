@@ -291,6 +291,13 @@ struct csmc_ctx {
 
 namespace {
 
+// no cubic / quartic slot of the batch's lattices is contracted in full
+bool batch_sparse(csmc_ctx *c, Batch *b) {
+    for (int id : b->lat)
+        if (c->h_lat[id].flags & LAT_DENSE_T) return false;
+    return true;
+}
+
 int ensure_pools(csmc_ctx *c) {
     if (!c->dirty) return CSMC_OK;
     HIPCHK(c->d_lat.alloc(c->h_lat.size()));
@@ -789,7 +796,7 @@ int setup_parts0(csmc_ctx *c, Batch *b) {
                 per_colour = std::max(per_colour, n);
             }
         }
-        const int threads = b->prec == 32 ? 1024 : 512;
+        const int threads = wg_threads_for(b->prec, b->self || b->gen, batch_sparse(c, b));
         G = std::min({8, c->ncu / R, per_colour / threads});
     }
     if (G < 2) return CSMC_OK;
@@ -841,7 +848,7 @@ int pack_plan(csmc_ctx *c, Batch *b, int lm) {
     pk.cap = cap;
     pk.use = false;
     const int R = b->nsys;
-    const int threads = b->prec == 32 ? 1024 : 512;
+    const int threads = wg_threads_for(b->prec, b->self || b->gen, batch_sparse(c, b));
     const int nwg = c->ncu > 0 ? c->ncu : 256;
     if (!c->pack || lm != 0 || R < 2 || nwg % 8) return CSMC_OK;
     std::vector<int> ncol(R), maxcol(R, 0);
@@ -999,8 +1006,7 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
     } else {
         PTHost &pt = b->pt;
         bool parts_done = false;
-        bool sparse = true;  // no slot of the batch's lattices is contracted in full
-        for (int id : b->lat) sparse = sparse && !(c->h_lat[id].flags & LAT_DENSE_T);
+        const bool sparse = batch_sparse(c, b);
         // chosen when the batch first runs in storage mode 0, also when an
         // earlier choice was made for an LDS mode (the autotuner's caps)
         if (lm == 0 && (!pt.parts_ready || pt.parts_lm != 0)) {

@@ -150,6 +150,10 @@ hipError_t launch_cons_pass(int prec, bool self, bool gen, int nsys, int max_col
                             hipStream_t st);
 
 int max_lds_bytes();
+// threads per workgroup of the persistent sweep kernel for a batch of this
+// precision, with cubic / quartic / self terms (general), all in sparse
+// patterns (sparse)
+int wg_threads_for(int prec, bool general, bool sparse);
 // raise a kernel's dynamic LDS limit to the device maximum (once per kernel
 // and device; later calls return at once)
 hipError_t allow_large_lds(const void *fn);

@@ -389,8 +389,21 @@ __device__ __forceinline__ void wg_system0(const WGArgs &a, int sys, int part, i
                                         pp);
 }
 
+// threads per workgroup of k_wg: 512 in FP64, 1024 in FP32, except the
+// sparse general FP32 kernels (cubic / quartic lattices), 512: 256 VGPRs per
+// lane instead of 128, so that their chained PT form needs no scratch
+template <class R, bool SP>
+constexpr int wg_threads() {
+    return sizeof(R) == 8 ? 512 : (SP ? 512 : 1024);
+}
+
+int wg_threads_for(int prec, bool general, bool sparse) {
+    if (prec == 64) return wg_threads<double, false>();
+    return general && sparse ? wg_threads<float, true>() : wg_threads<float, false>();
+}
+
 template <class R, int LM, bool SELF, bool GEN, int MODE, int K2C, bool SP>
-__global__ __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) void k_wg(WGArgs a) {
+__global__ __launch_bounds__((wg_threads<R, SP>())) void k_wg(WGArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     double *red = reinterpret_cast<double *>(smem);  // reduction scratch
 
@@ -500,7 +513,7 @@ static hipError_t launch_wg_t(int nsys, size_t lds_bytes, const WGArgs &a, hipSt
         hipError_t e = allow_large_lds(fn);
         if (e != hipSuccess) return e;
     }
-    const int threads = sizeof(R) == 8 ? 512 : 1024;
+    const int threads = wg_threads<R, SP>();
     // the waits of a chain and the barriers of parts need every workgroup resident
     if constexpr (MODE == WG_PT_CHAIN) return launch_cooperative(fn, nsys, threads, bytes, a, st);
     if constexpr (LM == 0)
