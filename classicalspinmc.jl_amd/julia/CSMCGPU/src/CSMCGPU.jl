@@ -533,6 +533,15 @@ function pt_split_parts(b::Batch)
     return Int(g[])
 end
 
+"""Tiles per system of the batch-wide sweeps (csmc_batch_tiles): > 0 when
+every lattice of the batch has a tile plan, so that a sweep is one launch of
+LDS tiles; 0 when the sweeps run as colour passes."""
+function batch_tiles(b::Batch)
+    n = Ref{Int32}(0)
+    check(ccall((:csmc_batch_tiles, libcsmc), Int32, (Ptr{Cvoid}, Int32, Ref{Int32}), b.ctx.h, b.id, n))
+    return Int(n[])
+end
+
 """Time each sweep form on the batch's own state and keep the fastest
 (csmc_autotune); returns the per-form times in ms (NaN: skipped) and the
 chosen form's index."""
