@@ -87,6 +87,7 @@ SIGNATURES = {
     "csmc_split_plan": [_P, _P, _I32, _I32, _I32, _P],
     "csmc_tile_plan": [_P, _P, _I32, _I32, _I64, _P, _P, _I32, _P],
     "csmc_batch_tiles": [_P, _I32, _P],
+    "csmc_site_eval": [_P, _I32, _I32, _I32, _P, _P, _I32, _P, _P, _P],
     "csmc_lattice_create": [_P, _P, _P, _I32, _D, _P],
     "csmc_lattice_info_get": [_P, _I32, _P],
     "csmc_lattice_tables": [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P],

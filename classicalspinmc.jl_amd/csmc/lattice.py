@@ -176,7 +176,7 @@ class Lattice:
         self.unit_cell = uc
         self._tables = None
 
-    def _eval_batch(self, ctx=None):
+    def _eval_batch(self, ctx=None, upload=True):
         ctx = ctx or _lib.default_context()
         key = ctx.uid
         if key not in self._eval:
@@ -185,14 +185,41 @@ class Lattice:
             _lib.check(_lib.lib().csmc_batch_create(ctx.h, 1, _lib.ptr(lid), _lib.FP64, C.byref(bid)))
             self._eval[key] = bid.value
         bid = self._eval[key]
-        _lib.check(_lib.lib().csmc_set_spins(ctx.h, bid, 0, _lib.ptr(self._spins)))
+        if upload:
+            _lib.check(_lib.lib().csmc_set_spins(ctx.h, bid, 0, _lib.ptr(self._spins)))
         return ctx, bid
+
+    def _site_and_partners(self, i):
+        """0-based site i and every site a term of it names (the reference
+        tables: bilinear, cubic and quartic partners; 0 = none)"""
+        t = self.tables()
+        parts = [np.array([i + 1], dtype=np.int64)]
+        for k in ("bsite", "csite", "qsite"):
+            a = np.asarray(t[k][i], dtype=np.int64).reshape(-1)
+            parts.append(a[a > 0])
+        return np.unique(np.concatenate(parts)) - 1
+
+    def _eval_site(self, i):
+        """local field and site energy of 0-based site i from the current host
+        spins: only the site and its partners are written to the device, so
+        a call costs the same at any lattice size (csmc_site_eval)"""
+        if not 0 <= i < self.size:
+            raise IndexError(f"site {i + 1} out of 1..{self.size}")
+        ctx, bid = self._eval_batch(upload=False)
+        sites = self._site_and_partners(i)
+        vals = np.ascontiguousarray(self._spins[sites], dtype=np.float64)
+        at = np.array([i], dtype=np.int64)
+        H = np.zeros(3)
+        e = np.zeros(1)
+        _lib.check(_lib.lib().csmc_site_eval(ctx.h, bid, 0, len(sites), _lib.ptr(sites), _lib.ptr(vals), 1,
+                                             _lib.ptr(at), _lib.ptr(H), _lib.ptr(e)))
+        return H, float(e[0])
 
 
 def get_local_field(lattice, point):
     """get_local_field (hamiltonian.jl:3-67); point is 1-based."""
-    H = local_fields(lattice)
-    return tuple(float(v) for v in H[point - 1])
+    H, _ = lattice._eval_site(point - 1)
+    return tuple(float(v) for v in H)
 
 
 def local_fields(lattice):
@@ -213,7 +240,7 @@ def site_energies(lattice):
 
 def energy(lattice, point):
     """energy(lattice, point) (hamiltonian.jl:139-196); point 1-based."""
-    return float(site_energies(lattice)[point - 1])
+    return lattice._eval_site(point - 1)[1]
 
 
 def total_energy(lattice, parts=False):

@@ -193,6 +193,7 @@ struct Batch {
     DevBuf<double> T, sigma, acc;  // per-system scratch
     DevBuf<uint32_t> accU;
     DevBuf<double> stage;          // host<->device staging (total*3 doubles)
+    DevBuf<unsigned char> sites;   // csmc_site_eval: indices and values of a few sites
     // equal-time correlations: device positions, momenta and GEMM workspaces
     SsfBufs ssf_in;
     DevBuf<double> ssf_C, ssf_Q, ssf_B, ssf_Cre, ssf_Cim, ssf_P, ssf_out;
@@ -1445,6 +1446,43 @@ int32_t csmc_site_energy(csmc_ctx *c, int32_t id, int32_t sys, double *e) {
     double *st = b->stage.p + 3 * b->sys[sys].spin_off;
     HIPCHK(launch_site_eval(b->prec, pools(c, b), b->spins.p, sys, N, nullptr, st, c->stream));
     HIPCHK(hipMemcpyAsync(e, st, (size_t)N * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CSMC_OK;
+}
+
+int32_t csmc_site_eval(csmc_ctx *c, int32_t id, int32_t sys, int32_t n_set, const int64_t *set_sites,
+                       const double *set_spins, int32_t n_at, const int64_t *at_sites, double *H, double *e) {
+    CHECK_CTX(c);
+    GET_BATCH(c, id, b);
+    if (sys < 0 || sys >= b->nsys) return fail(CSMC_EINVAL, "bad system index");
+    if (n_set < 0 || n_at < 0 || n_set > CSMC_SITE_EVAL_MAX || n_at > CSMC_SITE_EVAL_MAX)
+        return fail(CSMC_EINVAL, "site counts must be in 0..CSMC_SITE_EVAL_MAX");
+    if ((n_set && (!set_sites || !set_spins)) || (n_at && (!at_sites || (!H && !e))))
+        return fail(CSMC_EINVAL, "null pointer");
+    int rc = ensure_pools(c);
+    if (rc) return rc;
+    const HostLattice &L = *c->lats[b->lat[sys]];
+    // device indices (site order on the device is (colour, basis, reference))
+    std::vector<int32_t> idx((size_t)n_set + n_at);
+    for (int k = 0; k < n_set + n_at; ++k) {
+        const int64_t r = k < n_set ? set_sites[k] : at_sites[k - n_set];
+        if (r < 0 || r >= L.N) return fail(CSMC_EBOUNDS, "site index out of range");
+        idx[k] = L.dev_of_ref[r];
+    }
+    // one staging block: indices, then the values, then the outputs
+    const size_t bi = 4 * idx.size(), bv = 24 * (size_t)n_set, bo = 32 * (size_t)n_at;
+    const size_t ov = (bi + 7) & ~(size_t)7, oo = ov + bv;
+    if (b->sites.n < oo + bo) HIPCHK(b->sites.alloc(oo + bo));
+    unsigned char *d = b->sites.p;
+    HIPCHK(hipMemcpyAsync(d, idx.data(), bi, hipMemcpyHostToDevice, c->stream));
+    if (n_set) HIPCHK(hipMemcpyAsync(d + ov, set_spins, bv, hipMemcpyHostToDevice, c->stream));
+    double *dH = reinterpret_cast<double *>(d + oo), *de = dH + 3 * (size_t)n_at;
+    HIPCHK(launch_site_eval_at(b->prec, pools(c, b), b->spins.p, sys, reinterpret_cast<const int32_t *>(d),
+                               reinterpret_cast<const double *>(d + ov), n_set,
+                               reinterpret_cast<const int32_t *>(d) + n_set, n_at, H ? dH : nullptr,
+                               e ? de : nullptr, c->stream));
+    if (n_at && H) HIPCHK(hipMemcpyAsync(H, dH, 24 * (size_t)n_at, hipMemcpyDeviceToHost, c->stream));
+    if (n_at && e) HIPCHK(hipMemcpyAsync(e, de, 8 * (size_t)n_at, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return CSMC_OK;
 }

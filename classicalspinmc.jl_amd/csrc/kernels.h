@@ -193,6 +193,9 @@ hipError_t launch_pass(int prec, bool self, bool gen, int nsys, int max_colour_s
 hipError_t launch_graph_after_sweep(uint32_t *ctr, uint32_t *acc, double *total, double *sigma, int s0,
                                     int count, bool adaptive, double n_sites, hipStream_t st);
 hipError_t launch_graph_clear(uint32_t *acc, double *total, int nsys, hipStream_t st);
+hipError_t launch_site_eval_at(int prec, const DevPools &P, void *spins, int sys, const int32_t *set_dev,
+                               const double *set_val, int n_set, const int32_t *at_dev, int n_at, double *H,
+                               double *e, hipStream_t st);
 hipError_t launch_site_eval(int prec, const DevPools &P, const void *spins, int sys, int N,
                             double *H, double *e, hipStream_t st);
 hipError_t launch_energy(int prec, const DevPools &P, const void *spins, int nsys, double *E,

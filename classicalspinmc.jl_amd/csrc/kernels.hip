@@ -698,6 +698,66 @@ __global__ __launch_bounds__(256) void k_site_eval(DevPools P, const void *spins
     }
 }
 
+// a few sites of one system: first store n_set spins (device indices
+// set_dev, FP64 values set_val[3]), then evaluate the local field / site
+// energy of n_at sites (device indices at_dev) into H[k][3] / e[k]
+template <class R>
+__global__ __launch_bounds__(256) void k_put_sites(DevPools P, void *spins, int sys, const int32_t *set_dev,
+                                                   const double *set_val, int n_set) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_set) return;
+    const SysDev sd = P.sys[sys];
+    spin4<R> s;
+    s.x = (R)set_val[3 * k];
+    s.y = (R)set_val[3 * k + 1];
+    s.z = (R)set_val[3 * k + 2];
+    s.w = R(0);
+    spins_at<R>(spins, sd.spin_off).st(set_dev[k], s);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_site_eval_at(DevPools P, const void *spins, int sys, const int32_t *at_dev,
+                                                      int n_at, double *H, double *e) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_at) return;
+    const SysDev sd = P.sys[sys];
+    const LatDev ld = P.lats[sd.lat];
+    const LatView<double> L = lat_view<double>(ld, P.i32, P.runs + ld.run_off, P.coefD);
+    const SpinsG<R> S = spins_at<R>(spins, sd.spin_off);
+    const int i = at_dev[k];
+    const int b = L.sub[i];
+    const spin4<double> s = to_d<R>(S.ld(i));
+    double F[3], Hl[3];
+    interaction_field<double, double>(L, NbrG{L.nbr2, L.N}, b, i, [&](int j) { return to_d<R>(S.ld(j)); }, F);
+    local_field<double, double>(L, b, s, F, Hl);
+    if (H) {
+        H[3 * k] = Hl[0];
+        H[3 * k + 1] = Hl[1];
+        H[3 * k + 2] = Hl[2];
+    }
+    if (e) {
+        const double *h = L.field + b * 3;
+        e[k] = onsite_energy<double, double>(L, b, s.x, s.y, s.z) + s.x * F[0] + s.y * F[1] + s.z * F[2] -
+               (s.x * h[0] + s.y * h[1] + s.z * h[2]);
+    }
+}
+
+hipError_t launch_site_eval_at(int prec, const DevPools &P, void *spins, int sys, const int32_t *set_dev,
+                               const double *set_val, int n_set, const int32_t *at_dev, int n_at, double *H,
+                               double *e, hipStream_t st) {
+    if (n_set > 0) {
+        const dim3 g((n_set + 255) / 256);
+        if (prec == 32) hipLaunchKernelGGL((k_put_sites<float>), g, dim3(256), 0, st, P, spins, sys, set_dev, set_val, n_set);
+        else hipLaunchKernelGGL((k_put_sites<double>), g, dim3(256), 0, st, P, spins, sys, set_dev, set_val, n_set);
+    }
+    if (n_at > 0) {
+        const dim3 g((n_at + 255) / 256);
+        if (prec == 32) hipLaunchKernelGGL((k_site_eval_at<float>), g, dim3(256), 0, st, P, spins, sys, at_dev, n_at, H, e);
+        else hipLaunchKernelGGL((k_site_eval_at<double>), g, dim3(256), 0, st, P, spins, sys, at_dev, n_at, H, e);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_site_eval(int prec, const DevPools &P, const void *spins, int sys, int N,
                             double *H, double *e, hipStream_t st) {
     dim3 grid((N + 255) / 256);

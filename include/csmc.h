@@ -184,6 +184,18 @@ int32_t csmc_randomize_spins(csmc_ctx *ctx, int32_t batch_id, uint64_t seed);
 int32_t csmc_local_field(csmc_ctx *ctx, int32_t batch_id, int32_t system, double *H);
 /* energy(lattice, point) (hamiltonian.jl:139-196) at every site: e[N] */
 int32_t csmc_site_energy(csmc_ctx *ctx, int32_t batch_id, int32_t system, double *e);
+/* get_local_field(lattice, point) and energy(lattice, point)
+ * (hamiltonian.jl:3-67,139-196) at a few sites, in time independent of the
+ * lattice size: the spins of n_set sites (reference indices set_sites,
+ * 0-based, values set_spins[n_set][3]) are first stored into the system,
+ * then the FP64 local field H[n_at][3] and site energy e[n_at] (either may
+ * be null) of the sites at_sites are evaluated.  A caller whose host copy
+ * of the spins changes between calls passes each evaluated site and its
+ * partners in set_sites, so the device copy is current where it is read.
+ * Counts up to CSMC_SITE_EVAL_MAX; CSMC_EBOUNDS for a site out of range. */
+#define CSMC_SITE_EVAL_MAX 65536
+int32_t csmc_site_eval(csmc_ctx *ctx, int32_t batch_id, int32_t system, int32_t n_set, const int64_t *set_sites,
+                       const double *set_spins, int32_t n_at, const int64_t *at_sites, double *H, double *e);
 /* total_energy (hamiltonian.jl:70-132), FP64 reduction, per system: E[n_sys];
  * parts[n_sys][5] (optional) = bilinear, cubic, quartic, zeeman, onsite sums
  * before the 1/2, 1/3, 1/4 weights of hamiltonian.jl:131 */
