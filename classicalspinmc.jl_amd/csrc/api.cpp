@@ -279,6 +279,8 @@ struct csmc_ctx {
     bool verbose = false;        // CSMC_VERBOSE=1: device and lattice notes on stderr
     bool pack = true;            // packed schedules for uneven batches (CSMC_PACK=0: off)
     bool tile = true;            // tiled batch-wide sweeps where a lattice allows (CSMC_TILE=0: off)
+    bool parttab = true;         // storage mode 0: partners of each workgroup's sites in LDS
+                                 // (CSMC_PARTTAB=0: off)
     size_t tile_lds = 48 << 10;  // LDS budget of one tile (CSMC_TILE_LDS, KiB)
     int parts0 = 0;              // workgroups per system of the chained PT kernel in
                                  // storage mode 0: 0 chosen, 1 off, 2..8 forced
@@ -298,6 +300,26 @@ bool batch_sparse(csmc_ctx *c, Batch *b) {
         if (c->h_lat[id].flags & LAT_DENSE_T) return false;
     return true;
 }
+
+// LDS bytes of the per-workgroup partner tables of storage mode 0 (the
+// sparse general kernels) with G workgroups per system: every lattice of the
+// batch within uint16 indices, the busiest thread's visits x the block size
+// x the partner rows; 0 when some lattice does not qualify or the table does
+// not fit beside the reduction scratch
+size_t part_table_bytes(csmc_ctx *c, Batch *b, int G) {
+    if (!c->parttab || b->prec != 32 || !(b->self || b->gen) || !batch_sparse(c, b)) return 0;
+    const int threads = wg_threads_for(b->prec, true, true);
+    size_t most = 0;
+    for (int id : b->lat) {
+        const HostLattice &L = *c->lats[id];
+        if (L.N > 65535) return 0;
+        size_t nvisit = 0;
+        for (const auto &r : L.runs) nvisit += (size_t)(r[1] - r[0] + G * threads - 1) / (G * threads);
+        most = std::max(most, nvisit * threads * (size_t)(L.K2 + 2 * L.K3 + 3 * L.K4) * 2);
+    }
+    return RED_BYTES + most <= (size_t)c->max_lds ? most : 0;
+}
+
 
 int ensure_pools(csmc_ctx *c) {
     if (!c->dirty) return CSMC_OK;
@@ -1051,8 +1073,10 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
             // every csmc_pt_advance; sweeps and annealing through err0
             s.err0 = (a.mode == WG_PT || a.mode == WG_PT_CHAIN) ? pt.err.p : pt.err0.p;
             s.spin_limit = c->spin_limit;  // only the PT paths set it
+            const size_t tab = part_table_bytes(c, b, pt.parts0);
+            s.ptab = tab > 0;
             hipError_t e = launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0, sparse,
-                                     b->nsys * pt.parts0, lds_bytes, s, c->stream);
+                                     b->nsys * pt.parts0, tab > 0 ? tab : lds_bytes, s, c->stream);
             if (e == hipSuccess) {
                 parts_done = true;
             } else {
@@ -1060,9 +1084,13 @@ int run_wg(csmc_ctx *c, Batch *b, const WGArgs &a) {
                 pt.parts0 = 1;
             }
         }
-        if (!parts_done)
+        if (!parts_done) {
+            const size_t tab = lm == 0 ? part_table_bytes(c, b, 1) : 0;
+            WGArgs s = a;
+            s.ptab = tab > 0;
             HIPCHK(launch_wg(b->prec, lm, b->self, b->gen, c->specialise ? b->k2c : 0, sparse, b->nsys,
-                             lds_bytes, a, c->stream));
+                             tab > 0 ? tab : lds_bytes, s, c->stream));
+        }
     }
     if (c->timing) {
         HIPCHK(hipEventRecord(e1, c->stream));
@@ -1135,6 +1163,8 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
     if (p0 && p0[0] >= '1' && p0[0] <= '8' && p0[1] == 0) c->parts0 = p0[0] - '0';
     const char *pk = getenv("CSMC_PACK");
     if (pk && pk[0] == '0') c->pack = false;
+    const char *ptb = getenv("CSMC_PARTTAB");
+    if (ptb && ptb[0] == '0') c->parttab = false;
     const char *tl = getenv("CSMC_TILE");
     if (tl && tl[0] == '0') c->tile = false;
     const char *tb = getenv("CSMC_TILE_LDS");

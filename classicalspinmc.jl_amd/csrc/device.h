@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "csmc_internal.h"
 
 namespace csmc {
@@ -135,6 +137,31 @@ struct NbrG {
     __device__ __forceinline__ int operator()(int k, int i) const { return p[(int64_t)k * N + i]; }
     __device__ __forceinline__ int at(int k, int i) const { return p[(int64_t)k * N + i]; }
 };
+
+// The partners of the sites one thread visits in a sweep, staged in LDS as
+// uint16 (lattices of up to 65535 sites) when the sites a thread updates stay
+// the same for a whole launch: row s (bilinear slots 0..K2-1, then the two
+// partners of each cubic slot and the three of each quartic slot) holds at
+// column `slot` the partner of the site visited at that slot.  Serves the
+// cubic and quartic partners too (kGen).
+struct NbrPart {
+    static constexpr bool kGen = true;
+    const uint16_t *tab;
+    int stride, slot, K2, K3, N;
+    __device__ __forceinline__ int operator()(int k, int i) const { return k < K2 ? at(k, i) : N; }
+    __device__ __forceinline__ int at(int k, int) const { return tab[k * stride + slot]; }
+    __device__ __forceinline__ int c3(int k, int q, int) const { return tab[(K2 + 2 * k + q) * stride + slot]; }
+    __device__ __forceinline__ int q4(int k, int q, int) const {
+        return tab[(K2 + 2 * K3 + 3 * k + q) * stride + slot];
+    }
+};
+
+// whether a neighbour functor serves the cubic and quartic partners (else
+// they come from the lattice's global tables)
+template <class T, class = void>
+struct nb_has_gen : std::false_type {};
+template <class T>
+struct nb_has_gen<T, std::void_t<decltype(T::kGen)>> : std::bool_constant<T::kGen> {};
 
 struct NbrL {
     const uint16_t *p;
@@ -397,8 +424,13 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
             const int k = min(k0 + u, L.K3 - 1);
             const int p = ldc(L.pat3 + b * L.K3 + k);
             pat[u] = k0 + u < L.K3 ? p : TP_ZERO;
-            j[u][0] = L.nbr3[((int64_t)k * 2) * L.N + i];
-            j[u][1] = L.nbr3[((int64_t)k * 2 + 1) * L.N + i];
+            if constexpr (nb_has_gen<NB>::value) {
+                j[u][0] = nb.c3(k, 0, i);
+                j[u][1] = nb.c3(k, 1, i);
+            } else {
+                j[u][0] = L.nbr3[((int64_t)k * 2) * L.N + i];
+                j[u][1] = L.nbr3[((int64_t)k * 2 + 1) * L.N + i];
+            }
         }
         spin4<R> sp[GS][2];
 #pragma unroll
@@ -426,7 +458,12 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
             const int p = ldc(L.pat4 + b * L.K4 + k);
             pat[u] = k0 + u < L.K4 ? p : TP_ZERO;
 #pragma unroll
-            for (int q = 0; q < 3; ++q) j[u][q] = L.nbr4[((int64_t)k * 3 + q) * L.N + i];
+            for (int q = 0; q < 3; ++q) {
+                if constexpr (nb_has_gen<NB>::value)
+                    j[u][q] = nb.q4(k, q, i);
+                else
+                    j[u][q] = L.nbr4[((int64_t)k * 3 + q) * L.N + i];
+            }
         }
         spin4<R> sp[GS][3];
 #pragma unroll

@@ -80,6 +80,8 @@ struct WGArgs {
     double *psum0;
     uint32_t *err0;  // set when a barrier wait of the parts gives up
     uint32_t *xcc0;  // [nsys][parts0]: the XCD each part runs on, written at launch
+    int ptab;        // storage mode 0: stage each workgroup's partners in LDS (the
+                     // launch's LDS bytes hold the table)
     // WG_SWEEPS / WG_ANNEAL in storage mode 0 with a packed schedule:
     // workgroup w runs tasks[task_off[w] .. task_off[w + 1] - 1] in turn,
     // each {system, part, parts of that system, 0}; psum0 and xcc0 are laid

@@ -145,7 +145,7 @@ struct Parts {
     // one (agent-scope) barrier, and take the XCD fences from then on if all
     // parts of the system are on one XCD.  Every part reads the same values,
     // so all reach the same choice.
-    __device__ void find_xcd(uint32_t *xcc) {
+    __device__ __forceinline__ void find_xcd(uint32_t *xcc) {
         xlocal = false;
         if (G == 1) return;
         uint32_t *x = xcc + (int64_t)sys * gs;
@@ -183,19 +183,49 @@ struct Parts {
     }
 };
 
-// one colour sweep of a part: op(b, i) for the part's sites of each colour
+// one colour sweep of a part: op(b, i, slot) for the part's sites of each colour
 // class, then the parts' barrier
 template <class R, class OP>
 __device__ __forceinline__ void wg_colour_sweep_p(const LatView<R> &L, Parts &pp, OP &&op) {
     const int bd = (int)blockDim.x, first = pp.part * bd + (int)threadIdx.x, step = pp.G * bd;
+    // slot: the thread's visit number in the sweep x the block size + the
+    // thread (the same for every sweep of a launch: see stage_part_table)
+    int slot = (int)threadIdx.x;
     for (int c = 0; c < L.ncol; ++c) {
         for (int r = L.colrun[c]; r < L.colrun[c + 1]; ++r) {
             int4 run = L.runs[r];
             int b = __builtin_amdgcn_readfirstlane(run.z);
-            for (int i = run.x + first; i < run.y; i += step) op(b, i);
+            for (int i = run.x + first; i < run.y; i += step, slot += bd) op(b, i, slot);
         }
         pp.barrier();
     }
+}
+
+// The partners of every site this thread visits in a sweep (the visit order
+// of wg_colour_sweep_p) copied from the global tables into LDS as uint16
+// (NbrPart); returns the table's row stride.  Every thread of the workgroup
+// calls it; a barrier follows.
+template <class R>
+__device__ __forceinline__ int stage_part_table(const LatView<R> &L, const Parts &pp, uint16_t *tab) {
+    const int bd = (int)blockDim.x, first = pp.part * bd + (int)threadIdx.x, step = pp.G * bd;
+    int nvisit = 0;  // visits of the busiest thread (wave-uniform)
+    for (int r = L.colrun[0]; r < L.colrun[L.ncol]; ++r) {
+        const int4 run = L.runs[r];
+        nvisit += (run.y - run.x + step - 1) / step;
+    }
+    const int stride = nvisit * bd, rows = L.K2 + 2 * L.K3 + 3 * L.K4;
+    int slot = (int)threadIdx.x;
+    for (int r = L.colrun[0]; r < L.colrun[L.ncol]; ++r) {
+        const int4 run = L.runs[r];
+        for (int i = run.x + first; i < run.y; i += step, slot += bd)
+            for (int s = 0; s < rows; ++s) {
+                // global row: the bilinear slots, then (after the padding
+                // slots) the cubic and quartic partner rows
+                const int g = s < L.K2 ? s : s + (L.K2P - L.K2);
+                tab[s * stride + slot] = (uint16_t)L.nbr2[(int64_t)g * L.N + i];
+            }
+    }
+    return stride;
 }
 
 // FP64 energy parts (bilinear, cubic, quartic, zeeman, onsite) of a system
@@ -281,23 +311,23 @@ __device__ void thread_energy(const LatView<double> &Ld, const NB &nb, const SV 
 }
 
 // one Metropolis colour sweep; returns this thread's accepted count
-template <class R, bool SELF, bool GEN, int K2C, bool SP, class NB, class SV>
-__device__ int wg_metro_local(const LatView<R> &L, const NB &nb, const SV &X, R T, int kind, R sigma,
+template <class R, bool SELF, bool GEN, int K2C, bool SP, class NBAT, class SV>
+__device__ __forceinline__ int wg_metro_local(const LatView<R> &L, const NBAT &nbat, const SV &X, R T, int kind, R sigma,
                               uint32_t k0, uint32_t k1, uint32_t stream, uint32_t ctr, Parts &pp) {
     int acc = 0;
     const R invT = R(1) / T;
-    wg_colour_sweep_p<R>(L, pp, [&](int b, int i) {
-        acc += metro_update<R, SELF, GEN, K2C, SP>(L, nb, X, b, i, T, invT, kind, sigma, k0, k1, stream,
-                                                   ctr);
+    wg_colour_sweep_p<R>(L, pp, [&](int b, int i, int slot) {
+        acc += metro_update<R, SELF, GEN, K2C, SP>(L, nbat(slot), X, b, i, T, invT, kind, sigma, k0, k1,
+                                                   stream, ctr);
     });
     return acc;
 }
 
-template <class R, bool SELF, bool GEN, int K2C, bool SP, class NB, class SV>
-__device__ double wg_metro(const LatView<R> &L, const NB &nb, const SV &X, R T, int kind, R sigma,
+template <class R, bool SELF, bool GEN, int K2C, bool SP, class NBAT, class SV>
+__device__ __forceinline__ double wg_metro(const LatView<R> &L, const NBAT &nbat, const SV &X, R T, int kind, R sigma,
                            uint32_t k0, uint32_t k1, uint32_t stream, uint32_t ctr, double *red,
                            Parts &pp) {
-    double v[1] = {(double)wg_metro_local<R, SELF, GEN, K2C, SP>(L, nb, X, T, kind, sigma, k0, k1, stream,
+    double v[1] = {(double)wg_metro_local<R, SELF, GEN, K2C, SP>(L, nbat, X, T, kind, sigma, k0, k1, stream,
                                                                  ctr, pp)};
     block_sum<1>(v, red);
     pp.sum<1>(v);
@@ -307,7 +337,13 @@ __device__ double wg_metro(const LatView<R> &L, const NB &nb, const SV &X, R T, 
 // The sweeper of the global / LDS-array storage modes: sweeps walk the runs
 // of each colour class with one thread per site (wg_colour_sweep_p), over
 // the sites of this workgroup's part of the system.
-template <class R, bool SELF, bool GEN, int K2C, bool SP, class SV, class NB>
+// the LDS partner table of a workgroup's sites (NbrPart), when staged
+struct PartTab {
+    const uint16_t *tab;
+    int stride;
+};
+
+template <class R, bool SELF, bool GEN, int K2C, bool SP, class SV, class NB, bool PTAB = false>
 struct SweeperG {
     const LatView<R> &L;
     const LatView<double> &Ld;
@@ -316,21 +352,36 @@ struct SweeperG {
     double *red;
     uint32_t stream, k0, k1;
     Parts pp;
+    PartTab pt;
 
+    // the partners of the site at a visit slot: the LDS table of this
+    // workgroup's sites, or the global tables
+    __device__ __forceinline__ auto nbat(int slot) const {
+        if constexpr (PTAB)
+            return NbrPart{pt.tab, pt.stride, slot, L.K2, L.K3, L.N};
+        else
+            return nb;
+    }
     __device__ int n_sites() const { return L.N; }
     __device__ bool leader() const { return pp.part == 0; }
-    __device__ void or_sweep() {
-        wg_colour_sweep_p<R>(L, pp, [&](int b, int i) { or_update<R, GEN, K2C, SP>(L, nb, X, b, i); });
+    __device__ __forceinline__ void or_sweep() {
+        wg_colour_sweep_p<R>(L, pp, [&](int b, int i, int slot) {
+            or_update<R, GEN, K2C, SP>(L, nbat(slot), X, b, i);
+        });
     }
-    __device__ void det_sweep() {
-        wg_colour_sweep_p<R>(L, pp, [&](int b, int i) { det_update<R, GEN, K2C, SP>(L, nb, X, b, i); });
+    __device__ __forceinline__ void det_sweep() {
+        wg_colour_sweep_p<R>(L, pp, [&](int b, int i, int slot) {
+            det_update<R, GEN, K2C, SP>(L, nbat(slot), X, b, i);
+        });
     }
-    __device__ double metro(double T, int kind, double sigma, uint32_t ctr) {
-        return wg_metro<R, SELF, GEN, K2C, SP>(L, nb, X, (R)T, kind, (R)sigma, k0, k1, stream, ctr, red, pp);
+    __device__ __forceinline__ double metro(double T, int kind, double sigma, uint32_t ctr) {
+        auto at = [&](int slot) { return nbat(slot); };
+        return wg_metro<R, SELF, GEN, K2C, SP>(L, at, X, (R)T, kind, (R)sigma, k0, k1, stream, ctr, red, pp);
     }
-    __device__ double metro_energy(double T, int kind, double sigma, uint32_t ctr, bool with_m,
+    __device__ __forceinline__ double metro_energy(double T, int kind, double sigma, uint32_t ctr, bool with_m,
                                    double &E, double m[3]) {
-        const int acc = wg_metro_local<R, SELF, GEN, K2C, SP>(L, nb, X, (R)T, kind, (R)sigma, k0, k1,
+        auto at = [&](int slot) { return nbat(slot); };
+        const int acc = wg_metro_local<R, SELF, GEN, K2C, SP>(L, at, X, (R)T, kind, (R)sigma, k0, k1,
                                                               stream, ctr, pp);
         double e, mm[3];
         thread_energy<R, GEN, SP>(Ld, nb, X, e, mm, pp.part, pp.G);
@@ -350,18 +401,18 @@ struct SweeperG {
         E = v[1];
         return v[0];
     }
-    __device__ void magnet(double m[3]) {
+    __device__ __forceinline__ void magnet(double m[3]) {
         wg_magnet<R>(Ld, X, red, m, pp.part, pp.G);
         pp.sum<3>(m);
     }
 };
 
-template <int MODE, class R, bool SELF, bool GEN, int K2C, bool SP, class SV, class NB>
+template <int MODE, class R, bool SELF, bool GEN, int K2C, bool SP, class SV, class NB, bool PTAB = false>
 __device__ __forceinline__ void wg_run(const WGArgs &a, int sys, const LatView<R> &L,
                                        const LatView<double> &Ld, const SV &X, const NB &nb,
-                                       double *red, const Parts &pp) {
+                                       double *red, const Parts &pp, PartTab pt = PartTab{nullptr, 0}) {
     const uint32_t stream = (uint32_t)a.P.sys[sys].stream;
-    SweeperG<R, SELF, GEN, K2C, SP, SV, NB> sw{L, Ld, X, nb, red, stream, a.k0, a.k1, pp};
+    SweeperG<R, SELF, GEN, K2C, SP, SV, NB, PTAB> sw{L, Ld, X, nb, red, stream, a.k0, a.k1, pp, pt};
     wg_drive<MODE>(a, sys, sw);
 }
 
@@ -376,8 +427,9 @@ __device__ __forceinline__ size_t align16(size_t v) { return (v + 15) & ~(size_t
 // SP: no cubic or quartic slot of the batch is contracted in full.
 // ---------------------------------------------------------------------------
 // one system (or one part of it) in storage mode 0
-template <class R, bool SELF, bool GEN, int MODE, int K2C, bool SP>
-__device__ __forceinline__ void wg_system0(const WGArgs &a, int sys, int part, int nparts, double *red) {
+template <class R, bool SELF, bool GEN, int MODE, int K2C, bool SP, bool PTAB = false>
+__device__ __forceinline__ void wg_system0(const WGArgs &a, int sys, int part, int nparts, double *red,
+                                           uint16_t *ptab = nullptr) {
     Parts pp{part, nparts, sys, a.bar0, a.psum0, a.err0, a.spin_limit, 0, false, a.parts0};
     if (nparts > 1) pp.find_xcd(a.xcc0);
     const SysDev sd = a.P.sys[sys];
@@ -385,8 +437,17 @@ __device__ __forceinline__ void wg_system0(const WGArgs &a, int sys, int part, i
     const int4 *runs = a.P.runs + ld.run_off;
     const LatView<R> L = lat_view<R>(ld, a.P.i32, runs, coef_of<R>(a.P));
     const LatView<double> Ld = lat_view<double>(ld, a.P.i32, runs, a.P.coefD);
-    wg_run<MODE, R, SELF, GEN, K2C, SP>(a, sys, L, Ld, spins_at<R>(a.spins, sd.spin_off), NbrG{L.nbr2, L.N}, red,
-                                        pp);
+    if constexpr (PTAB) {
+        // the partners of this workgroup's sites, staged once for the launch
+        const int stride = stage_part_table<R>(L, pp, ptab);
+        __syncthreads();
+        wg_run<MODE, R, SELF, GEN, K2C, SP, SpinsG<R>, NbrG, true>(a, sys, L, Ld, spins_at<R>(a.spins, sd.spin_off),
+                                                                  NbrG{L.nbr2, L.N}, red, pp,
+                                                                  PartTab{ptab, stride});
+    } else {
+        wg_run<MODE, R, SELF, GEN, K2C, SP>(a, sys, L, Ld, spins_at<R>(a.spins, sd.spin_off), NbrG{L.nbr2, L.N},
+                                            red, pp);
+    }
 }
 
 // threads per workgroup of k_wg: 512 in FP64, 1024 in FP32, except the
@@ -402,7 +463,9 @@ int wg_threads_for(int prec, bool general, bool sparse) {
     return general && sparse ? wg_threads<float, true>() : wg_threads<float, false>();
 }
 
-template <class R, int LM, bool SELF, bool GEN, int MODE, int K2C, bool SP>
+// PTAB (storage mode 0): the partners of each workgroup's sites are staged
+// in LDS for the launch (NbrPart), so a site's update gathers only spins
+template <class R, int LM, bool SELF, bool GEN, int MODE, int K2C, bool SP, bool PTAB = false>
 __global__ __launch_bounds__((wg_threads<R, SP>())) void k_wg(WGArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     double *red = reinterpret_cast<double *>(smem);  // reduction scratch
@@ -432,7 +495,8 @@ __global__ __launch_bounds__((wg_threads<R, SP>())) void k_wg(WGArgs a) {
         for (int t = t0; t < t1; ++t) {
             int4 tk = make_int4(sys, part, nparts, 0);
             if ((MODE == WG_SWEEPS || MODE == WG_ANNEAL) && a.tasks) tk = a.tasks[t];
-            wg_system0<R, SELF, GEN, MODE, K2C, SP>(a, tk.x, tk.y, tk.z, red);
+            wg_system0<R, SELF, GEN, MODE, K2C, SP, PTAB>(a, tk.x, tk.y, tk.z, red,
+                                                          reinterpret_cast<uint16_t *>(smem + RED_BYTES));
             __syncthreads();  // the reduction scratch is reused by the next task
         }
         if (MODE == WG_PT && a.exchange) pt_exchange_last(a, reinterpret_cast<int *>(red + RED_FLAG));
@@ -505,10 +569,10 @@ hipError_t launch_cooperative(const void *fn, int nblocks, int threads, size_t l
     return hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(threads), args, (unsigned)lds_bytes, st);
 }
 
-template <class R, int LM, bool GENERAL, int MODE, int K2C, bool SP>
+template <class R, int LM, bool GENERAL, int MODE, int K2C, bool SP, bool PTAB = false>
 static hipError_t launch_wg_t(int nsys, size_t lds_bytes, const WGArgs &a, hipStream_t st) {
-    size_t bytes = RED_BYTES + (LM ? lds_bytes : 0);
-    const void *fn = (const void *)k_wg<R, LM, GENERAL, GENERAL, MODE, K2C, SP>;
+    size_t bytes = RED_BYTES + ((LM || PTAB) ? lds_bytes : 0);
+    const void *fn = (const void *)k_wg<R, LM, GENERAL, GENERAL, MODE, K2C, SP, PTAB>;
     if (bytes > 65536) {
         hipError_t e = allow_large_lds(fn);
         if (e != hipSuccess) return e;
@@ -518,7 +582,7 @@ static hipError_t launch_wg_t(int nsys, size_t lds_bytes, const WGArgs &a, hipSt
     if constexpr (MODE == WG_PT_CHAIN) return launch_cooperative(fn, nsys, threads, bytes, a, st);
     if constexpr (LM == 0)
         if (a.parts0 > 1) return launch_cooperative(fn, nsys, threads, bytes, a, st);
-    hipLaunchKernelGGL((k_wg<R, LM, GENERAL, GENERAL, MODE, K2C, SP>), dim3(nsys), dim3(threads), bytes,
+    hipLaunchKernelGGL((k_wg<R, LM, GENERAL, GENERAL, MODE, K2C, SP, PTAB>), dim3(nsys), dim3(threads), bytes,
                        st, a);
     return hipGetLastError();
 }
@@ -534,8 +598,12 @@ static hipError_t launch_wg_k(int k2c, bool sparse, int nsys, size_t lds_bytes, 
     }
     // the sparse-pattern variant of the general FP32 kernels (1024 threads,
     // where the full contractions would spill)
-    if constexpr (sizeof(R) == 4 && GENERAL)
+    if constexpr (sizeof(R) == 4 && GENERAL) {
+        // storage mode 0 with the partners of each workgroup's sites in LDS
+        if constexpr (LM == 0)
+            if (sparse && a.ptab) return launch_wg_t<R, LM, GENERAL, MODE, 0, true, true>(nsys, lds_bytes, a, st);
         if (sparse) return launch_wg_t<R, LM, GENERAL, MODE, 0, true>(nsys, lds_bytes, a, st);
+    }
     return launch_wg_t<R, LM, GENERAL, MODE, 0, false>(nsys, lds_bytes, a, st);
 }
 
