@@ -415,8 +415,10 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
     // GS slots at a time: their patterns, partner indices and partner spins
     // are loaded before any of them is used, so the loads of a group are in
     // flight together (a group past the last slot repeats the last slot's
-    // loads and skips its product)
-    constexpr int GS = 2;
+    // loads and skips its product).  With the partner indices in LDS
+    // (NbrPart) a group of three measured faster; with indices gathered from
+    // global memory, two.
+    constexpr int GS = nb_has_gen<NB>::value ? 3 : 2;
     for (int k0 = 0; PART != 2 && k0 < L.K3; k0 += GS) {
         int pat[GS], j[GS][2];
 #pragma unroll
