@@ -283,20 +283,23 @@ __device__ void wg_magnet(const LatView<double> &Ld, const SV &X, double *red, d
 // this thread's FP64 energy (the parts combined as hamiltonian.jl:131) and
 // summed spin over the same sites, in the same order, as wg_energy and
 // wg_magnet
-template <class R, bool GEN, bool SP, class NB, class SV>
-__device__ void thread_energy(const LatView<double> &Ld, const NB &nb, const SV &X, double &e,
-                              double m[3], int part = 0, int G = 1) {
+// nbat(slot): the partners of the site at a visit slot (the sites and their
+// order are those of wg_colour_sweep_p)
+template <class R, bool GEN, bool SP, class NBAT, class SV>
+__device__ __forceinline__ void thread_energy(const LatView<double> &Ld, const NBAT &nbat, const SV &X,
+                                              double &e, double m[3], int part = 0, int G = 1) {
     double p[5] = {0, 0, 0, 0, 0};
     m[0] = m[1] = m[2] = 0.0;
     const int bd = (int)blockDim.x, first = part * bd + (int)threadIdx.x, step = G * bd;
+    int slot = (int)threadIdx.x;
     for (int r = Ld.colrun[0]; r < Ld.colrun[Ld.ncol]; ++r) {
         int4 run = Ld.runs[r];
         int b = __builtin_amdgcn_readfirstlane(run.z);
         const double *h = Ld.field + b * 3;
-        for (int i = run.x + first; i < run.y; i += step) {
+        for (int i = run.x + first; i < run.y; i += step, slot += bd) {
             spin4<double> s = to_d<R>(X.ld(i));
             double F2[3], F3[3], F4[3];
-            field_parts<GEN, SP>(Ld, nb, b, i, [&](int j) { return to_d<R>(X.ld(j)); }, F2, F3, F4);
+            field_parts<GEN, SP>(Ld, nbat(slot), b, i, [&](int j) { return to_d<R>(X.ld(j)); }, F2, F3, F4);
             p[0] += s.x * F2[0] + s.y * F2[1] + s.z * F2[2];
             p[1] += s.x * F3[0] + s.y * F3[1] + s.z * F3[2];
             p[2] += s.x * F4[0] + s.y * F4[1] + s.z * F4[2];
@@ -384,7 +387,7 @@ struct SweeperG {
         const int acc = wg_metro_local<R, SELF, GEN, K2C, SP>(L, at, X, (R)T, kind, (R)sigma, k0, k1,
                                                               stream, ctr, pp);
         double e, mm[3];
-        thread_energy<R, GEN, SP>(Ld, nb, X, e, mm, pp.part, pp.G);
+        thread_energy<R, GEN, SP>(Ld, at, X, e, mm, pp.part, pp.G);
         if (with_m) {
             double v[5] = {(double)acc, e, mm[0], mm[1], mm[2]};
             block_sum<5>(v, red);
