@@ -85,8 +85,8 @@ SIGNATURES = {
     "csmc_timing_read": [_P, _P, _P],
     "csmc_build_tables": [_P, _P, _I32, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "csmc_split_plan": [_P, _P, _I32, _I32, _I32, _P],
-    "csmc_tile_plan": [_P, _P, _I32, _I32, _I64, _P, _P, _I32, _P],
-    "csmc_batch_tiles": [_P, _I32, _P],
+    "csmc_tile_plan": [_P, _P, _I32, _I32, _I64, _I32, _P, _P, _I32, _P],
+    "csmc_batch_tiles": [_P, _I32, _P, _P],
     "csmc_site_eval": [_P, _I32, _I32, _I32, _P, _P, _I32, _P, _P, _P],
     "csmc_lattice_create": [_P, _P, _P, _I32, _D, _P],
     "csmc_lattice_info_get": [_P, _I32, _P],

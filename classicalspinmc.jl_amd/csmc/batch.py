@@ -218,8 +218,15 @@ class SpinBatch:
         """tiles per system of the batch-wide sweeps (0: one launch per
         colour pass) and, for 0, the reason"""
         n = C.c_int32(0)
-        _lib.check(_lib.lib().csmc_batch_tiles(self._h, self.id, C.byref(n)))
+        _lib.check(_lib.lib().csmc_batch_tiles(self._h, self.id, C.byref(n), None))
         return n.value, ("" if n.value else _lib.lib().csmc_last_error().decode())
+
+    def tile_sweeps(self):
+        """sweeps one tiled launch runs while that many remain (1: one
+        launch per sweep, or no tiles)"""
+        n, k = C.c_int32(0), C.c_int32(1)
+        _lib.check(_lib.lib().csmc_batch_tiles(self._h, self.id, C.byref(n), C.byref(k)))
+        return k.value
 
     AUTOTUNE_FORMS = ("persistent, storage mode cap 3", "persistent, cap 2", "persistent, cap 1",
                       "persistent, cap 0", "batch-wide colour passes")

@@ -225,13 +225,18 @@ struct Batch {
     DevBuf<unsigned char> cons_prop, cons_dec;
     // tiled sweeps (tile.cpp / tile.hip) of the batch-wide sweep form:
     // state 0 not planned yet, 1 every lattice has a plan, -1 some has none
-    struct Tiles {
-        int state = 0;
-        std::string why;
+    struct TileSet {
         size_t lds = 0;
         int max_tiles = 0;
         DevBuf<TileDev> td;
         DevBuf<int32_t> tab;
+    };
+    struct Tiles {
+        int state = 0;
+        std::string why;
+        TileSet one;    // one sweep per launch
+        int sweeps = 1;  // > 1: every lattice also has a plan of this many
+        TileSet many;   // sweeps per launch
     } tiles;
     size_t spin_bytes() const { return prec == 32 ? 12 : 24; }  // packed {x,y,z} records
 };
@@ -282,6 +287,7 @@ struct csmc_ctx {
     bool parttab = true;         // storage mode 0: partners of each workgroup's sites in LDS
                                  // (CSMC_PARTTAB=0: off)
     size_t tile_lds = 48 << 10;  // LDS budget of one tile (CSMC_TILE_LDS, KiB)
+    int tile_sweeps = 2;         // sweeps per tiled launch where a plan allows (CSMC_TILE_SWEEPS, 1..8)
     int parts0 = 0;              // workgroups per system of the chained PT kernel in
                                  // storage mode 0: 0 chosen, 1 off, 2..8 forced
     uint32_t spin_limit = 1u << 26;  // polls of a wait for another workgroup
@@ -1169,6 +1175,8 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
     if (tl && tl[0] == '0') c->tile = false;
     const char *tb = getenv("CSMC_TILE_LDS");
     if (tb && tb[0] >= '1' && tb[0] <= '9') c->tile_lds = (size_t)strtoul(tb, nullptr, 10) << 10;
+    const char *tsw = getenv("CSMC_TILE_SWEEPS");
+    if (tsw && tsw[0] >= '1' && tsw[0] <= '8' && tsw[1] == 0) c->tile_sweeps = tsw[0] - '0';
     const char *dt = getenv("CSMC_DENSE_TENSORS");
     if (dt && dt[0] == '1') c->dense_tensors = true;
     const char *vb = getenv("CSMC_VERBOSE");
@@ -1279,14 +1287,15 @@ int32_t csmc_split_plan(const csmc_unit_cell *uc, const int32_t *shape, int32_t 
 }
 
 int32_t csmc_tile_plan(const csmc_unit_cell *uc, const int32_t *shape, int32_t periodic, int32_t prec,
-                       int64_t lds_budget, int32_t *head, int32_t *tab, int32_t tab_cap, int32_t *tab_len) {
-    if (!head || !tab_len || (prec != 32 && prec != 64) || lds_budget < 0)
-        return fail(CSMC_EINVAL, "tile plan: head and tab_len needed, prec 32 or 64");
+                       int64_t lds_budget, int32_t sweeps, int32_t *head, int32_t *tab, int32_t tab_cap,
+                       int32_t *tab_len) {
+    if (!head || !tab_len || (prec != 32 && prec != 64) || lds_budget < 0 || sweeps < 1 || sweeps > 8)
+        return fail(CSMC_EINVAL, "tile plan: head and tab_len needed, prec 32 or 64, sweeps 1..8");
     HostLattice L;
     std::string err;
     int rc = build_lattice(uc, shape, periodic, 0.5, L, err);
     if (rc) return fail(rc, err);
-    const TilePlan P = plan_tiles(L, 3 * (prec == 32 ? 4 : 8), (size_t)lds_budget);
+    const TilePlan P = plan_tiles(L, 3 * (prec == 32 ? 4 : 8), (size_t)lds_budget, sweeps);
     static_assert(sizeof(TileDev) == CSMC_TILE_HEAD * 4, "tile plan header");
     if (!P.ok) {
         std::memset(head, 0, sizeof(TileDev));
@@ -1666,13 +1675,14 @@ int32_t csmc_sweep_passes(csmc_ctx *c, int32_t id, int32_t kind, const double *T
     return sweep_passes(c, b, kind, T, sigma, n_sweeps, seed, accepted);
 }
 
-int32_t csmc_batch_tiles(csmc_ctx *c, int32_t id, int32_t *tiles) {
+int32_t csmc_batch_tiles(csmc_ctx *c, int32_t id, int32_t *tiles, int32_t *sweeps) {
     CHECK_CTX(c);
     GET_BATCH(c, id, b);
     if (!tiles) return fail(CSMC_EINVAL, "null output");
     int rc = ensure_pools(c);
     if (rc) return rc;
-    *tiles = (c->tile && tiles_ready(c, b)) ? b->tiles.max_tiles : 0;
+    *tiles = (c->tile && tiles_ready(c, b)) ? b->tiles.one.max_tiles : 0;
+    if (sweeps) *sweeps = *tiles ? b->tiles.sweeps : 1;
     if (!*tiles) g_err = c->tile ? b->tiles.why : "tiles switched off (CSMC_TILE=0)";
     return CSMC_OK;
 }
@@ -1813,44 +1823,68 @@ int constraint_run(csmc_ctx *c, Batch *b, uint64_t seed, int64_t n_sweeps) {
     return CSMC_OK;
 }
 
-// the tile plans of every lattice of the batch (once per batch); false if
-// some lattice has no tiled form
-bool tiles_ready(csmc_ctx *c, Batch *b) {
-    Batch::Tiles &tl = b->tiles;
-    if (tl.state != 0) return tl.state > 0;
-    tl.state = -1;
+// the tile plans of nsw sweeps per launch of every lattice of the batch;
+// false (and why) if some lattice has none
+bool plan_tile_set(csmc_ctx *c, Batch *b, int nsw, Batch::TileSet &ts, std::string &why) {
     const size_t rec = 3 * (b->prec == 32 ? 4 : 8);
     std::vector<TileDev> td(c->lats.size(), TileDev{});
     std::vector<int32_t> tab;
     std::vector<char> done(c->lats.size(), 0);
+    size_t lds = 0;
+    int max_tiles = 0;
     for (int s = 0; s < b->nsys; ++s) {
         const int id = b->lat[s];
         if (done[id]) continue;
         done[id] = 1;
-        TilePlan P = plan_tiles(*c->lats[id], rec, c->tile_lds);
+        TilePlan P = plan_tiles(*c->lats[id], rec, c->tile_lds, nsw);
         if (!P.ok) {
-            tl.why = "lattice " + std::to_string(id) + ": " + P.why;
-            if (c->verbose) fprintf(stderr, "csmc: batch-wide sweeps without tiles (%s)\n", tl.why.c_str());
+            why = "lattice " + std::to_string(id) + ": " + P.why;
             return false;
         }
         P.d.tab_off = (int32_t)tab.size();
         tab.insert(tab.end(), P.tab.begin(), P.tab.end());
         td[id] = P.d;
-        tl.lds = std::max(tl.lds, (size_t)(P.d.nbox + 1) * rec + 4 * (size_t)P.d.p * P.d.nb);
-        tl.max_tiles = std::max(tl.max_tiles, P.d.ntiles);
+        lds = std::max(lds, (size_t)(P.d.nbox + 1) * rec + 4 * (size_t)P.d.p * P.d.nb);
+        max_tiles = std::max(max_tiles, P.d.ntiles);
         if (c->verbose)
             fprintf(stderr,
-                    "csmc: lattice %d tiled: tile %dx%dx%d cells, box %dx%dx%d x %d bases (%.3f sites loaded, "
-                    "%.3f updated per site), %d tiles, %zu B LDS\n",
-                    id, P.d.T[0], P.d.T[1], P.d.T[2], P.d.B[0], P.d.B[1], P.d.B[2], P.d.nb, P.load_per_site,
-                    P.work_per_site, P.d.ntiles, (size_t)(P.d.nbox + 1) * rec);
+                    "csmc: lattice %d tiled, %d sweep(s) a launch: tile %dx%dx%d cells, box %dx%dx%d x %d bases "
+                    "(%.3f sites loaded, %.3f updated per site and sweep), %d tiles, %zu B LDS\n",
+                    id, nsw, P.d.T[0], P.d.T[1], P.d.T[2], P.d.B[0], P.d.B[1], P.d.B[2], P.d.nb,
+                    P.load_per_site, P.work_per_site, P.d.ntiles, (size_t)(P.d.nbox + 1) * rec);
     }
-    if (tl.lds > (size_t)c->max_lds) return false;
-    if (tl.td.alloc(td.size()) != hipSuccess || tl.tab.alloc(std::max<size_t>(tab.size(), 1)) != hipSuccess)
+    if (lds > (size_t)c->max_lds) {
+        why = "LDS";
         return false;
-    if (hipMemcpy(tl.td.p, td.data(), td.size() * sizeof(TileDev), hipMemcpyHostToDevice) != hipSuccess ||
-        (!tab.empty() && hipMemcpy(tl.tab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice) != hipSuccess))
+    }
+    if (ts.td.alloc(td.size()) != hipSuccess || ts.tab.alloc(std::max<size_t>(tab.size(), 1)) != hipSuccess ||
+        hipMemcpy(ts.td.p, td.data(), td.size() * sizeof(TileDev), hipMemcpyHostToDevice) != hipSuccess ||
+        (!tab.empty() && hipMemcpy(ts.tab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice) != hipSuccess)) {
+        why = "device memory";
         return false;
+    }
+    ts.lds = lds;
+    ts.max_tiles = max_tiles;
+    return true;
+}
+
+// the tile plans of every lattice of the batch (once per batch): one sweep
+// per launch, and c->tile_sweeps per launch where every lattice allows;
+// false if some lattice has no tiled form
+bool tiles_ready(csmc_ctx *c, Batch *b) {
+    Batch::Tiles &tl = b->tiles;
+    if (tl.state != 0) return tl.state > 0;
+    tl.state = -1;
+    if (!plan_tile_set(c, b, 1, tl.one, tl.why)) {
+        if (c->verbose) fprintf(stderr, "csmc: batch-wide sweeps without tiles (%s)\n", tl.why.c_str());
+        return false;
+    }
+    tl.sweeps = 1;
+    std::string why;
+    if (c->tile_sweeps > 1 && plan_tile_set(c, b, c->tile_sweeps, tl.many, why))
+        tl.sweeps = c->tile_sweeps;
+    else if (c->tile_sweeps > 1 && c->verbose)
+        fprintf(stderr, "csmc: tiles of one sweep a launch (%s)\n", why.c_str());
     tl.state = 1;
     return true;
 }
@@ -1895,7 +1929,8 @@ int sweep_passes(csmc_ctx *c, Batch *b, int32_t kind, const double *T, const dou
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, c->stream));
     }
-    // tiled: one launch per sweep, every colour step of a tile from LDS
+    // tiled: one launch per sweep or per tiles.sweeps sweeps, every colour
+    // step of a tile from LDS
     const bool tiled = c->tile && tiles_ready(c, b);
     TileArgs ta{};
     if (tiled) {
@@ -1906,8 +1941,6 @@ int sweep_passes(csmc_ctx *c, Batch *b, int32_t kind, const double *T, const dou
             HIPCHK(hipMemsetAsync(b->spins_alt.p, 0, b->spins.n, c->stream));
         }
         ta.P = a.P;
-        ta.td = b->tiles.td.p;
-        ta.tab = b->tiles.tab.p;
         ta.kind = kind;
         ta.k0 = a.k0;
         ta.k1 = a.k1;
@@ -1916,15 +1949,21 @@ int sweep_passes(csmc_ctx *c, Batch *b, int32_t kind, const double *T, const dou
         ta.sigma_sys = a.sigma_sys;
         ta.acc = a.acc;
     }
-    for (int sw = 0; sw < n_sweeps; ++sw) {
+    for (int sw = 0; sw < n_sweeps;) {
         // systems go on the grid's y dimension, at most 65535 a launch
+        int done = 1;
         if (tiled) {
+            const bool many = b->tiles.sweeps > 1 && sw + b->tiles.sweeps <= n_sweeps;
+            const Batch::TileSet &ts = many ? b->tiles.many : b->tiles.one;
+            done = many ? b->tiles.sweeps : 1;
+            ta.td = ts.td.p;
+            ta.tab = ts.tab.p;
             ta.spins_in = b->spins.p;
             ta.spins_out = b->spins_alt.p;
             for (int s0 = 0; s0 < b->nsys; s0 += 65535) {
                 ta.sys0 = s0;
-                HIPCHK(launch_tile(b->prec, b->self, std::min(65535, b->nsys - s0), b->tiles.max_tiles,
-                                   b->tiles.lds, ta, c->stream));
+                HIPCHK(launch_tile(b->prec, b->self, std::min(65535, b->nsys - s0), ts.max_tiles, ts.lds, ta,
+                                   c->stream));
             }
             std::swap(b->spins.p, b->spins_alt.p);  // the same size: only the roles change
         }
@@ -1935,9 +1974,12 @@ int sweep_passes(csmc_ctx *c, Batch *b, int32_t kind, const double *T, const dou
                 HIPCHK(launch_pass(b->prec, b->self, b->gen, std::min(65535, b->nsys - s0), maxcol, a, c->stream));
             }
         }
-        if (metro)
+        // the Philox counter steps once per sweep (a tiled launch of several
+        // sweeps draws with counter + sweep index)
+        for (int k = 0; metro && k < done; ++k)
             HIPCHK(launch_graph_after_sweep(b->mcount.p, b->accU.p, b->acc.p, b->sigma.p, 0, b->nsys, false, 1.0,
                                             c->stream));
+        sw += done;
     }
     if (c->timing) {
         HIPCHK(hipEventRecord(e1, c->stream));

@@ -134,8 +134,9 @@ constexpr int32_t TP_Q4_PT = 4;   // quartic: p = t and q = r ((Si.Sl)(Sj.Sk))
 //                        colour r and basis b (-1: no such run)
 //   delta[nb * K2]       box-local index step to the partner of slot k of
 //                        basis b (TILE_PAD: the slot names no site)
-//   rect[ncol * nb * 6]  box cells {lo, hi} per dimension where colour step
-//                        c updates basis b (lo >= hi: none)
+//   rect[nsw * ncol * nb * 6]  box cells {lo, hi} per dimension where step
+//                        s (sweep s / ncol, colour s % ncol) updates basis b
+//                        (lo >= hi: none)
 //   rawc[ncol]           raw colour (c . x + e_b) mod p of each colour
 constexpr int32_t TILE_PAD = 0x7fffffff;
 
@@ -146,7 +147,8 @@ struct TileDev {
     int32_t ncol, ntiles;
     int32_t L[3], T[3], lo[3], B[3], nt[3], c[3];
     int32_t boxcells, nbox, tcells, ncell;
-    int32_t tab_off, pad_;
+    int32_t tab_off;
+    int32_t nsw;      // sweeps per launch (one staging of the box)
 };
 
 // incidence entry: owner device site and a packed (kind, slot, role)

@@ -58,7 +58,7 @@ struct Rect {
 
 }  // namespace
 
-TilePlan plan_tiles(const HostLattice &L, size_t rec_bytes, size_t lds_budget) {
+TilePlan plan_tiles(const HostLattice &L, size_t rec_bytes, size_t lds_budget, int nsw) {
     TilePlan P;
     auto no = [&](const char *w) {
         P.ok = false;
@@ -73,6 +73,7 @@ TilePlan plan_tiles(const HostLattice &L, size_t rec_bytes, size_t lds_budget) {
     const int D = L.dim, nb = L.nb, K2 = L.K2, p = cp.p;
     if (D < 1 || D > 3) return no("dimension");
     if (nb > 64 || K2 > 64) return no("basis or slot count");
+    if (nsw < 1 || nsw > 8) return no("sweeps per launch");
     // dimensions right-aligned in three positions: lattice dimension d is
     // position d + 3 - D, so position 2 is always the fastest (last) one and
     // unused positions have extent 1
@@ -188,35 +189,45 @@ TilePlan plan_tiles(const HostLattice &L, size_t rec_bytes, size_t lds_budget) {
         for (int t1 : cand[1])
             for (int t2 : cand[2]) {
                 const int T[3] = {t0, t1, t2};
-                // update regions, from the last colour back: colour step c
-                // of basis b covers the tile and the partners of every site
-                // a later step updates
-                std::vector<Rect> rect((size_t)ncol * nb);
+                // update regions of the nsw * ncol colour steps, from the
+                // last back: a step of the last sweep covers the tile; a
+                // step also covers the partners of every site a later step
+                // updates, as of their latest update before that step (this
+                // sweep for an earlier colour, the previous sweep for a later
+                // one), and the sites the next sweep's step of its colour
+                // updates (an update reads the site's own value)
+                const int S = nsw * ncol;
+                std::vector<Rect> rect((size_t)S * nb);
                 for (int col = 0; col < ncol; ++col)
                     for (int b = 0; b < nb; ++b)
                         if (!cz || cp.remap[pmod(cp.e[b], p)] == col) {
-                            Rect &r = rect[(size_t)col * nb + b];
+                            Rect &r = rect[((size_t)(nsw - 1) * ncol + col) * nb + b];
                             for (int d = 0; d < 3; ++d) r.hi[d] = T[d];
                         }
-                for (int col = ncol - 1; col >= 0; --col)
+                for (int st = S - 1; st >= 0; --st) {
+                    const int col = st % ncol;
                     for (int b = 0; b < nb; ++b) {
-                        const Rect rc = rect[(size_t)col * nb + b];
+                        const Rect rc = rect[(size_t)st * nb + b];
                         if (rc.empty()) continue;
+                        if (st >= ncol) rect[(size_t)(st - ncol) * nb + b].unite(rc);
                         for (int k = 0; k < K2; ++k) {
                             const Off &f = off[(size_t)b * K2 + k];
                             if (f.bj < 0) continue;
                             long v = (long)rawc[col] + cp.e[f.bj] - cp.e[b];
                             for (int d = 0; d < 3; ++d) v += (long)cc[d] * f.o[d];
                             const int pc = cp.remap[pmod(v, p)];
-                            if (pc < col) rect[(size_t)pc * nb + f.bj].unite(rc.shifted(f.o));
+                            if (pc == col) continue;  // the site itself (a self term)
+                            const int ps = st - col + pc - (pc > col ? ncol : 0);
+                            if (ps >= 0) rect[(size_t)ps * nb + f.bj].unite(rc.shifted(f.o));
                         }
                     }
+                }
                 Rect box;
                 for (int d = 0; d < 3; ++d) box.hi[d] = T[d];
                 long work = 0;
-                for (int col = 0; col < ncol; ++col)
+                for (int st = 0; st < S; ++st)
                     for (int b = 0; b < nb; ++b) {
-                        const Rect &rc = rect[(size_t)col * nb + b];
+                        const Rect &rc = rect[(size_t)st * nb + b];
                         if (rc.empty()) continue;
                         work += rc.cells() / (cz ? 1 : p);
                         box.unite(rc);
@@ -235,7 +246,7 @@ TilePlan plan_tiles(const HostLattice &L, size_t rec_bytes, size_t lds_budget) {
                 if ((size_t)(nbox + 1) * rec_bytes > lds_budget || nbox + 1 > (1L << 20)) continue;
                 const long owned = (long)nb * T[0] * T[1] * T[2];
                 if (owned < std::min<long>(256, N / 4)) continue;
-                const double load = (double)nbox / owned, wk = (double)work / owned;
+                const double load = (double)nbox / owned / nsw, wk = (double)work / owned / nsw;
                 const double score = load + 0.5 * wk;
                 if (found && !(score < best - 1e-9 || (score < best + 1e-9 && owned > (long)nb * P.d.tcells)))
                     continue;
@@ -252,6 +263,7 @@ TilePlan plan_tiles(const HostLattice &L, size_t rec_bytes, size_t lds_budget) {
                 t.cz = cz ? 1 : 0;
                 t.cinv = cinv;
                 t.ncol = ncol;
+                t.nsw = nsw;
                 t.ncell = (int32_t)ncell;
                 t.ntiles = 1;
                 for (int d = 0; d < 3; ++d) {
@@ -282,9 +294,9 @@ TilePlan plan_tiles(const HostLattice &L, size_t rec_bytes, size_t lds_budget) {
                         for (int d = 0; d < 3; ++d) dl += f.o[d] * stride[d];
                         P.tab.push_back((int32_t)dl);
                     }
-                for (int col = 0; col < ncol; ++col)
+                for (int st = 0; st < S; ++st)
                     for (int b = 0; b < nb; ++b) {
-                        const Rect &rc = rect[(size_t)col * nb + b];
+                        const Rect &rc = rect[(size_t)st * nb + b];
                         for (int d = 0; d < 3; ++d) {
                             P.tab.push_back(rc.empty() ? 0 : rc.lo[d] - box.lo[d]);
                             P.tab.push_back(rc.empty() ? 0 : rc.hi[d] - box.lo[d]);

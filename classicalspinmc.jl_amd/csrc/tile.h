@@ -19,7 +19,8 @@ struct TilePlan {
     double work_per_site = 0.0;  // updates (with halo recomputation) per owned site
 };
 
-// rec_bytes: LDS bytes per staged site; lds_budget: LDS bytes a tile may use
-TilePlan plan_tiles(const HostLattice &L, size_t rec_bytes, size_t lds_budget);
+// rec_bytes: LDS bytes per staged site; lds_budget: LDS bytes a tile may
+// use; nsw: sweeps a launch runs from one staging of the box
+TilePlan plan_tiles(const HostLattice &L, size_t rec_bytes, size_t lds_budget, int nsw = 1);
 
 }  // namespace csmc

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
     // run bases in LDS after the spins: runbase[r * nb + b]
     int *sbase = reinterpret_cast<int *>(sx + 3 * (nbox + 1));
     const int32_t *eb = a.tab + t.tab_off, *runbase = eb + nb, *delta = runbase + t.p * nb;
-    const int32_t *rect = delta + nb * t.K2, *rawc = rect + t.ncol * nb * 6;
+    const int32_t *rect = delta + nb * t.K2, *rawc = rect + t.nsw * t.ncol * nb * 6;
     for (int k = threadIdx.x; k < t.p * nb; k += blockDim.x) sbase[k] = runbase[k];
     if (threadIdx.x == 0) X.st(nbox, spin4<R>{R(0), R(0), R(0), R(0)});
     const float r_B1 = 1.0f / (float)t.B[1], r_B2 = 1.0f / (float)t.B[2];
@@ -141,10 +141,12 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
         const R invT = R(1) / T;
         const R sigma = (metro && a.sigma_sys) ? (R)a.sigma_sys[sys] : R(60);
         const uint32_t ctr = metro ? (a.ctr ? a.ctr[sys] : a.ctr0) : 0u;
-        for (int col = 0; col < t.ncol; ++col) {
+        // step st: sweep st / ncol of the launch, colour st % ncol
+        for (int st = 0, col = 0, sw = 0; st < t.nsw * t.ncol; ++st, col = (col + 1 == t.ncol ? 0 : col + 1),
+                 sw += (col == 0 ? 1 : 0)) {
             const int r0 = ldc(rawc + col);
             for (int b = 0; b < nb; ++b) {
-                const int32_t *rc = rect + (col * nb + b) * 6;
+                const int32_t *rc = rect + (st * nb + b) * 6;
                 const int lo0 = ldc(rc), hi0 = ldc(rc + 1), lo1 = ldc(rc + 2), hi1 = ldc(rc + 3);
                 const int lo2 = ldc(rc + 4), hi2 = ldc(rc + 5);
                 if (lo0 >= hi0 || lo1 >= hi1 || lo2 >= hi2) continue;
@@ -181,7 +183,8 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
                                          (unsigned)(u2 + t.lo[2]) < (unsigned)t.T[2];
                         const uint32_t ref = (uint32_t)(((b * t.L[0] + x0) * t.L[1] + x1) * t.L[2] + x2);
                         const spin4<R> s = X.ld(li);
-                        const uint4 r = philox4x32_10(make_uint4(ref, (uint32_t)sd.stream, ctr, TAG_METRO), a.k0,
+                        const uint4 r = philox4x32_10(make_uint4(ref, (uint32_t)sd.stream, ctr + (uint32_t)sw,
+                                                                 TAG_METRO), a.k0,
                                                       a.k1);
                         const spin4<R> n = metro_propose<R>(L, s, r, a.kind, sigma);
                         const R dE = metro_eval<R, SELF, false, 0>(L, nbr, X, b, li, s, n);

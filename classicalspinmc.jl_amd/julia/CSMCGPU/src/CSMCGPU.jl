@@ -538,7 +538,8 @@ every lattice of the batch has a tile plan, so that a sweep is one launch of
 LDS tiles; 0 when the sweeps run as colour passes."""
 function batch_tiles(b::Batch)
     n = Ref{Int32}(0)
-    check(ccall((:csmc_batch_tiles, libcsmc), Int32, (Ptr{Cvoid}, Int32, Ref{Int32}), b.ctx.h, b.id, n))
+    check(ccall((:csmc_batch_tiles, libcsmc), Int32, (Ptr{Cvoid}, Int32, Ref{Int32}, Ptr{Int32}), b.ctx.h, b.id, n,
+                C_NULL))
     return Int(n[])
 end
 

@@ -137,14 +137,17 @@ int32_t csmc_split_plan(const csmc_unit_cell *uc, const int32_t *shape, int32_t 
  * lattice.jl:176-206); a plan exists when the lattice is periodic, has
  * bilinear terms only, and its colouring is lattice-periodic.  head receives
  * CSMC_TILE_HEAD ints: D, nb, K2, p, cz, cinv, ncol, ntiles, L[3], T[3],
- * lo[3], B[3], nt[3], c[3], boxcells, nbox, tcells, ncell, tab_off, pad
+ * lo[3], B[3], nt[3], c[3], boxcells, nbox, tcells, ncell, tab_off, nsw
  * (all zero and the reason in csmc_last_error when there is no plan); tab
- * receives tab_len ints e[nb], runbase[p nb], delta[nb K2], rect[ncol nb 6],
- * rawc[ncol] when tab_cap is large enough.  prec: 32 or 64 (the record size
- * in LDS); lds_budget: LDS bytes a tile may use. */
+ * receives tab_len ints e[nb], runbase[p nb], delta[nb K2],
+ * rect[nsw ncol nb 6], rawc[ncol] when tab_cap is large enough.  prec: 32
+ * or 64 (the record size in LDS); lds_budget: LDS bytes a tile may use;
+ * sweeps: 1..8 sweeps run from one staging of a tile's box (the box then
+ * holds the partners of the partners, sweeps deep). */
 #define CSMC_TILE_HEAD 32
 int32_t csmc_tile_plan(const csmc_unit_cell *uc, const int32_t *shape, int32_t periodic, int32_t prec,
-                       int64_t lds_budget, int32_t *head, int32_t *tab, int32_t tab_cap, int32_t *tab_len);
+                       int64_t lds_budget, int32_t sweeps, int32_t *head, int32_t *tab, int32_t tab_cap,
+                       int32_t *tab_len);
 /* Lattice(shape, uc, S; bc) (lattice.jl:65-291): builds the per-site tables,
  * colours the interaction hypergraph and uploads device tables.
  * periodic: 1 = "periodic", 0 = "open", other = CSMC_EBC. */
@@ -289,9 +292,11 @@ int32_t csmc_pt_split_parts(csmc_ctx *ctx, int32_t batch_id, int32_t *parts);
 /* The plain form over the whole batch: the Philox keys and counters of
  * csmc_sweep_plain, so the results are the same bit for bit.  When every
  * lattice of the batch has a tile plan (csmc_batch_tiles), one launch per
- * sweep sweeps LDS tiles of every system (index-free partners, one read and
- * one write of each spin record per sweep); otherwise one launch per colour
- * pass covers every system (one thread per site).  For systems too large
+ * sweep, or per several sweeps (CSMC_TILE_SWEEPS, default 2, where every
+ * lattice has such a plan), sweeps LDS tiles of every system (index-free
+ * partners, one read and one write of each spin record per launch);
+ * otherwise one launch per colour pass covers every system (one thread per
+ * site).  For systems too large
  * for one workgroup to sweep well; not for the adaptive cone.  With
  * csmc_timing on, events bracket the sweeps. */
 int32_t csmc_sweep_passes(csmc_ctx *ctx, int32_t batch_id, int32_t kind, const double *T, const double *sigma,
@@ -300,9 +305,10 @@ int32_t csmc_sweep_passes(csmc_ctx *ctx, int32_t batch_id, int32_t kind, const d
  * every lattice of the batch has a tile plan (csmc_tile_plan), so that a
  * sweep runs as one launch of LDS tiles reading one spin buffer and writing
  * the other; 0 when the sweeps run as one launch per colour pass (the reason
- * in csmc_last_error).  No reference counterpart: a query of this engine's
- * schedule. */
-int32_t csmc_batch_tiles(csmc_ctx *ctx, int32_t batch_id, int32_t *tiles);
+ * in csmc_last_error).  sweeps (may be null) receives the sweeps a tiled
+ * launch runs while at least that many remain (1 when tiles are off).  No
+ * reference counterpart: a query of this engine's schedule. */
+int32_t csmc_batch_tiles(csmc_ctx *ctx, int32_t batch_id, int32_t *tiles, int32_t *sweeps);
 
 /* Autotuner (no reference counterpart): time n_sweeps sweeps of `kind` (T
  * for the Metropolis kinds) from the batch's current state in each form --

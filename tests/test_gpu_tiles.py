@@ -48,6 +48,7 @@ CASES = [
     ("square_onsite_64", lambda: [(square_onsite(), (64, 64))] * 3),
     ("triangular_dm_48", lambda: [(W.triangular_dm(-1.0, 0.1, (0.0, 0.0, 0.2)), (48, 48))] * 3),
     ("pyrochlore_8", lambda: [(pyrochlore_heisenberg(), (8, 8, 8))] * 2),
+    ("pyrochlore_12", lambda: [(pyrochlore_heisenberg(), (12, 12, 12))] * 2),
     ("mixed_64_32", lambda: [(W.honeycomb_kitaev(1.0, -2.0), (64, 64)), (W.honeycomb_kitaev(1.0, -2.0), (32, 32))]),
 ]
 
@@ -83,6 +84,10 @@ def test_tiled_sweeps_equal_colour_passes(ctx_pair, name, spec, prec):
     nt, why = bt.tiles()
     assert nt > 0, why
     assert bp.tiles()[0] == 0
+    # two sweeps a launch where the box of two sweeps fits the lattice and
+    # the 48 KiB LDS budget (pyrochlore 12^3 in FP64: no tile shape does)
+    one = name == "pyrochlore_8" or (name == "pyrochlore_12" and prec == 64)
+    assert bt.tile_sweeps() == (1 if one else 2)
     n = bt.n
     T = np.linspace(0.2, 1.5, n)
     sg = np.full(n, 0.4)
@@ -116,3 +121,36 @@ def test_graph_after_tiled_sweeps(ctx_pair):
     gp.close()
     bt.close()
     bp.close()
+
+
+@pytest.mark.parametrize("sweeps_per_launch", [1, 3])
+def test_sweeps_per_launch(sweeps_per_launch):
+    """CSMC_TILE_SWEEPS: one and three sweeps a tiled launch give the
+    colour-pass results for sweep counts around the launch size"""
+    import os
+    os.environ["CSMC_TILE_SWEEPS"] = str(sweeps_per_launch)
+    try:
+        tiled = csmc.Context(0)
+    finally:
+        del os.environ["CSMC_TILE_SWEEPS"]
+    os.environ["CSMC_TILE"] = "0"
+    try:
+        plain = csmc.Context(0)
+    finally:
+        del os.environ["CSMC_TILE"]
+    for name, spec in [("honeycomb_64", [(W.honeycomb_kitaev(1.0, -2.0), (64, 64))] * 3),
+                       ("triangular_dm_48", [(W.triangular_dm(-1.0, 0.1, (0.0, 0.0, 0.2)), (48, 48))] * 2)]:
+        print(f"[tiles] {name} {sweeps_per_launch} sweeps a launch", flush=True)
+        bt, bp = _batches((tiled, plain), spec, 32)
+        assert bt.tile_sweeps() == sweeps_per_launch
+        T = np.linspace(0.3, 1.2, bt.n)
+        for kind, kw in [(csmc.OVERRELAX, {}), (csmc.METROPOLIS, dict(T=T))]:
+            for sweeps in (2, 3, 4, 7):
+                at = bt.sweep_passes(kind, n_sweeps=sweeps, seed=9, **kw)
+                ap = bp.sweep_passes(kind, n_sweeps=sweeps, seed=9, **kw)
+                assert np.array_equal(at, ap), (kind, sweeps)
+                _equal(bt, bp)
+        bt.close()
+        bp.close()
+    tiled.close()
+    plain.close()

@@ -27,30 +27,31 @@ TILE_PAD = 0x7FFFFFFF
 M64 = (1 << 64) - 1
 
 
-def tile_plan(uc, shape, prec=32, budget=48 << 10, periodic=1):
+def tile_plan(uc, shape, prec=32, budget=48 << 10, periodic=1, nsw=1):
     if len(uc.basis) == 0:  # the Lattice constructor's default site (lattice.jl:68-70)
         addBasisSite(uc, np.zeros(uc.D))
     s, keep = uc._pack()
     sh = np.ascontiguousarray(np.array(shape, np.int32))
     head = np.zeros(32, np.int32)
     n = C.c_int32(0)
-    _lib.check(_lib.lib().csmc_tile_plan(C.byref(s), _lib.ptr(sh), periodic, prec, budget, _lib.ptr(head), None,
-                                         0, C.byref(n)))
+    _lib.check(_lib.lib().csmc_tile_plan(C.byref(s), _lib.ptr(sh), periodic, prec, budget, nsw, _lib.ptr(head),
+                                         None, 0, C.byref(n)))
     if head[1] == 0:
         return None, _lib.lib().csmc_last_error().decode()
     tab = np.zeros(n.value, np.int32)
-    _lib.check(_lib.lib().csmc_tile_plan(C.byref(s), _lib.ptr(sh), periodic, prec, budget, _lib.ptr(head),
+    _lib.check(_lib.lib().csmc_tile_plan(C.byref(s), _lib.ptr(sh), periodic, prec, budget, nsw, _lib.ptr(head),
                                          _lib.ptr(tab), n.value, C.byref(n)))
     t = {k: int(head[i]) for i, k in enumerate(HEAD)}
     for i, k in enumerate(["L", "T", "lo", "B", "nt", "c"]):
         t[k] = [int(v) for v in head[8 + 3 * i: 11 + 3 * i]]
-    t.update(boxcells=int(head[26]), nbox=int(head[27]), tcells=int(head[28]), ncell=int(head[29]))
+    t.update(boxcells=int(head[26]), nbox=int(head[27]), tcells=int(head[28]), ncell=int(head[29]),
+             nsw=int(head[31]))
     nb, p, K2, ncol = t["nb"], t["p"], t["K2"], t["ncol"]
     o = 0
     t["eb"] = tab[o:o + nb].tolist(); o += nb
     t["runbase"] = tab[o:o + p * nb].tolist(); o += p * nb
     t["delta"] = tab[o:o + nb * K2].tolist(); o += nb * K2
-    t["rect"] = tab[o:o + ncol * nb * 6].tolist(); o += ncol * nb * 6
+    t["rect"] = tab[o:o + t["nsw"] * ncol * nb * 6].tolist(); o += t["nsw"] * ncol * nb * 6
     t["rawc"] = tab[o:o + ncol].tolist(); o += ncol
     assert o == len(tab)
     return t, ""
@@ -63,10 +64,11 @@ def mix(v):
     return v ^ (v >> 31)
 
 
-def update(ref, own, partners):
-    """a stand-in for one site update: depends on the site, its value and the
-    multiset of its partners' values"""
-    acc = (own * 0x9E3779B97F4A7C15 + ref * 0xD6E8FEB86659FD93) & M64
+def update(ref, own, partners, sweep):
+    """a stand-in for one site update: depends on the site, the sweep (the
+    Philox counter of a Metropolis update), its value and the multiset of its
+    partners' values"""
+    acc = (own * 0x9E3779B97F4A7C15 + ref * 0xD6E8FEB86659FD93 + sweep * 0xA0761D6478BD642F) & M64
     for v in partners:
         acc = (acc + mix(v)) & M64
     return mix(acc)
@@ -85,11 +87,11 @@ def device_order(colour, ncell):
 def pass_schedule(vals, bsite, colour, ncol, sweeps):
     """the colour-pass schedule on reference-indexed values"""
     vals = list(vals)
-    for _ in range(sweeps):
+    for sw in range(sweeps):
         for col in range(ncol):
             new = {}
             for s in np.flatnonzero(colour == col):
-                new[s] = update(int(s), vals[s], [vals[j - 1] for j in bsite[s] if j > 0])
+                new[s] = update(int(s), vals[s], [vals[j - 1] for j in bsite[s] if j > 0], sw)
             for s, v in new.items():
                 vals[s] = v
     return vals
@@ -102,7 +104,7 @@ def imod(a, m):
 
 def replay_tiles(t, G, N, sweeps):
     """the loops of k_tile over every tile of one system, reading G (device
-    order) and writing a second buffer, once per sweep"""
+    order) and writing a second buffer, once per launch of nsw sweeps"""
     nb, K2, p, cz, cinv = t["nb"], t["K2"], t["p"], t["cz"], t["cinv"]
     Lx, T, lo, B, nt, c = t["L"], t["T"], t["lo"], t["B"], t["nt"], t["c"]
     boxcells, nbox, tcells = t["boxcells"], t["nbox"], t["tcells"]
@@ -121,7 +123,9 @@ def replay_tiles(t, G, N, sweeps):
     def ref_of(b, x):
         return ((b * Lx[0] + x[0]) * Lx[1] + x[1]) * Lx[2] + x[2]
 
-    for _ in range(sweeps):
+    nsw, ncol = t["nsw"], t["ncol"]
+    assert sweeps % nsw == 0
+    for launch in range(sweeps // nsw):
         out = [None] * N
         for tile in range(t["ntiles"]):
             org, r = [0, 0, 0], tile
@@ -141,10 +145,11 @@ def replay_tiles(t, G, N, sweeps):
                     x[d] = v
                     cc = q
                 X[li] = G[dev_index(b, x)]
-            for col in range(t["ncol"]):
+            for st in range(nsw * ncol):
+                col, sw = st % ncol, launch * nsw + st // ncol
                 r0 = rawc[col]
                 for b in range(nb):
-                    rc = rect[(col * nb + b) * 6:(col * nb + b) * 6 + 6]
+                    rc = rect[(st * nb + b) * 6:(st * nb + b) * 6 + 6]
                     lo0, hi0, lo1, hi1, lo2, hi2 = rc
                     if lo0 >= hi0 or lo1 >= hi1 or lo2 >= hi2:
                         continue
@@ -170,7 +175,7 @@ def replay_tiles(t, G, N, sweeps):
                             assert 0 <= li + dl < nbox, (li, dl)
                             parts.append(X[li + dl])
                         x = [(org[d] + u[d] + lo[d]) % Lx[d] for d in range(3)]
-                        new[li] = update(ref_of(b, x), X[li], parts)
+                        new[li] = update(ref_of(b, x), X[li], parts, sw)
                     for li, v in new.items():  # one colour class: independent updates
                         X[li] = v
             for oi in range(nb * tcells):
@@ -205,13 +210,20 @@ CASES = [
     ("square_64x64", lambda: W.square_heisenberg(), (64, 64)),
     ("triangular_dm_48x48", lambda: W.triangular_dm(-1.0, 0.1, (0.0, 0.0, 0.2)), (48, 48)),
     ("pyrochlore_8", pyrochlore_heisenberg, (8, 8, 8)),
+    ("pyrochlore_12", pyrochlore_heisenberg, (12, 12, 12)),
 ]
 
 
+@pytest.mark.parametrize("nsw", [1, 2])
 @pytest.mark.parametrize("name,make,shape", CASES, ids=[c[0] for c in CASES])
-def test_tile_replay_equals_colour_passes(name, make, shape):
+def test_tile_replay_equals_colour_passes(name, make, shape, nsw):
     uc = make()
-    t, why = tile_plan(uc, shape, budget=16 << 10)
+    t, why = tile_plan(uc, shape, budget=(16 << 10) * nsw * nsw, nsw=nsw)
+    if nsw > 1 and name in ("honeycomb_4x4", "pyrochlore_8"):
+        # two sweeps' halo needs a box wider than the lattice (pyrochlore:
+        # tiles of 2^3 cells own fewer than 256 sites)
+        assert t is None and "no tile shape fits" in why
+        return
     assert t is not None, why
     assert t["ntiles"] >= 2, "the case must cut the lattice into several tiles"
     lat = Lattice(shape, uc, 0.5, seed=1)
@@ -229,16 +241,18 @@ def test_tile_replay_equals_colour_passes(name, make, shape):
     assert all(got[dev_of_ref[s]] == want[s] for s in range(N))
 
 
-def test_planted_halo_mistake_is_caught():
-    """the replay notices a plan whose update region misses one halo row"""
+@pytest.mark.parametrize("nsw", [1, 2])
+def test_planted_halo_mistake_is_caught(nsw):
+    """the replay notices a plan whose update region misses one halo row
+    (with two sweeps a launch: a row of the first sweep's halo)"""
     uc = W.square_heisenberg()
-    t, _ = tile_plan(uc, (32, 32), budget=16 << 10)
+    t, _ = tile_plan(uc, (32, 32), budget=16 << 10, nsw=nsw)
     lat = Lattice((32, 32), uc, 0.5, seed=1)
     tb = lat.tables()
     colour = np.asarray(tb["colour"])
     order, dev_of_ref = device_order(colour, t["ncell"])
     vals = [int(v) for v in np.random.default_rng(4).integers(0, 1 << 62, size=lat.size)]
-    want = pass_schedule(vals, np.asarray(tb["bsite"]), colour, t["ncol"], sweeps=1)
+    want = pass_schedule(vals, np.asarray(tb["bsite"]), colour, t["ncol"], sweeps=nsw)
     G = [0] * lat.size
     for s in range(lat.size):
         G[dev_of_ref[s]] = vals[s]
@@ -246,7 +260,8 @@ def test_planted_halo_mistake_is_caught():
     rect = list(t["rect"])
     rect[2] += 1
     bad = dict(t, rect=rect)
-    got = replay_tiles(bad, G, lat.size, sweeps=1)
+    assert replay_tiles(t, G, lat.size, sweeps=nsw) == [want[s] for s in order]
+    got = replay_tiles(bad, G, lat.size, sweeps=nsw)
     assert any(got[dev_of_ref[s]] != want[s] for s in range(lat.size))
 
 
