@@ -163,6 +163,14 @@ struct nb_has_gen : std::false_type {};
 template <class T>
 struct nb_has_gen<T, std::void_t<decltype(T::kGen)>> : std::bool_constant<T::kGen> {};
 
+// whether a neighbour functor knows the lattice's bilinear slot count (its
+// member K2): the field then skips the padding slots past it, which hold
+// zero couplings and name the zero site (each would add +0 to the field)
+template <class T, class = void>
+struct nb_knows_k2 : std::false_type {};
+template <class T>
+struct nb_knows_k2<T, std::void_t<decltype(T::kKnowsK2)>> : std::bool_constant<T::kKnowsK2> {};
+
 struct NbrL {
     const uint16_t *p;
     int N, K2;
@@ -397,6 +405,8 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
         bilinear_field<R, A, K2C, false>(L, b, sj, F);
     }
     for (int k0 = 0; K2C == 0 && PART != 2 && k0 < L.K2P; k0 += 4) {
+        int ku = 4;
+        if constexpr (nb_knows_k2<NB>::value) ku = min(4, nb.K2 - k0);
         int j[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) j[u] = nb(k0 + u, i);
@@ -405,6 +415,7 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
         for (int u = 0; u < 4; ++u) sj[u] = ld(j[u]);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
+            if (u >= ku) break;
             const R *M = c2 + (b * L.K2P + k0 + u) * 9;
             F[0] += (A)ldc(M + 0) * sj[u].x + (A)ldc(M + 1) * sj[u].y + (A)ldc(M + 2) * sj[u].z;
             F[1] += (A)ldc(M + 3) * sj[u].x + (A)ldc(M + 4) * sj[u].y + (A)ldc(M + 5) * sj[u].z;

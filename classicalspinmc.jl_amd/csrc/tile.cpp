@@ -93,6 +93,11 @@ TilePlan plan_tiles(const HostLattice &L, size_t rec_bytes, size_t lds_budget, i
         if (cinv == 0) return no("the colour pattern is not invertible along the last dimension");
     }
     const long ncell = L.ncell, N = L.N;
+    // the kernel's index products are 24-bit multiplies: the last extent,
+    // the row count and the raw colour's sum c . x + e below 2^24
+    const long lim24 = 1L << 24;
+    if (Ls[2] >= lim24 || ncell / Ls[2] >= lim24 || (long)p * (Ls[0] + Ls[1] + Ls[2] + 1) >= lim24)
+        return no("lattice extents beyond 24-bit index arithmetic");
     auto cellidx = [&](const int *x) { return ((long)x[0] * Ls[1] + x[1]) * Ls[2] + x[2]; };
     // reference site index s = ((b L0 + x0) L1 + x1) L2 + x2
     auto decompose = [&](long s, int *b, int *x) {
@@ -280,7 +285,7 @@ TilePlan plan_tiles(const HostLattice &L, size_t rec_bytes, size_t lds_budget, i
                 t.tcells = T[0] * T[1] * T[2];
                 // tables
                 P.tab.clear();
-                for (int b = 0; b < nb; ++b) P.tab.push_back(cp.e[b]);
+                for (int b = 0; b < nb; ++b) P.tab.push_back(pmod(cp.e[b], p));
                 P.tab.insert(P.tab.end(), runbase.begin(), runbase.end());
                 const long stride[3] = {(long)B[1] * B[2], B[2], 1};
                 for (int b = 0; b < nb; ++b)

@@ -18,29 +18,39 @@
 // neighbour.  The sweep reads one spin buffer and writes the other: a tile's
 // halo must hold the values from before the sweep, which a neighbouring tile
 // of the same launch may already have replaced in place.
+#include <cstdlib>
+#include <type_traits>
+
 #include "sweep.h"
 
 namespace csmc {
 
 // partner of slot k of the current basis: box index + delta, or the zero pad
-// site for a slot that names no site
+// site for a slot that names no site.  Indices are in units of the box
+// view's index (W per site: 3 words for packed FP32 records).
+template <int W>
 struct NbrD {
+    static constexpr bool kKnowsK2 = true;
     const int32_t *d;  // delta row of the basis (wave-uniform)
     int K2, pad;
     __device__ __forceinline__ int operator()(int k, int i) const {
         if (k >= K2) return pad;
         const int v = ldc(d + k);
-        return v == TILE_PAD ? pad : i + v;
+        return v == TILE_PAD ? pad : i + v * W;
     }
     __device__ __forceinline__ int at(int k, int i) const { return (*this)(k, i); }
 };
+
+// a b for 0 <= a, b < 2^24 (a full-rate 24-bit multiply; the product must
+// fit 32 bits)
+__device__ __forceinline__ int mulu24(int a, int b) { return (int)__umul24((unsigned)a, (unsigned)b); }
 
 // a / d for 0 <= a < 2^24 and 1 <= d < 2^24 from a float reciprocal rd of
 // d: the float quotient is within one of the true one (exact for d = 1),
 // and the remainder fixes it; the products fit 24-bit multiplies
 __device__ __forceinline__ int fdiv(int a, int d, float rd) {
     int q = (int)((float)a * rd);
-    const int r = a - __mul24(q, d);
+    const int r = a - mulu24(q, d);
     q += r < 0 ? -1 : (r >= d ? 1 : 0);
     return q;
 }
@@ -68,10 +78,78 @@ __device__ __forceinline__ int wrap1(int v, int Ld) {
     return v - (v >= Ld ? Ld : 0);
 }
 
+// FP32 spins in LDS as packed {x, y, z} records indexed by word (3 per
+// site): one address per site, the components at offsets 0, 4 and 8 (a
+// stride of three words between lanes meets every bank once)
+template <class R>
+struct SpinsLds3 {
+    R *p;
+    __device__ __forceinline__ spin4<R> ld(int w) const {
+        const R *q = p + w;
+        return spin4<R>{q[0], q[1], q[2], R(0)};
+    }
+    __device__ __forceinline__ void st(int w, const spin4<R> &v) const {
+        R *q = p + w;
+        q[0] = v.x;
+        q[1] = v.y;
+        q[2] = v.z;
+    }
+};
+
+// the LDS view of the box: packed records in FP32 (index = 3 x site), and
+// component arrays in FP64 (index = site; 24-byte records would meet banks
+// twice per wave)
+template <class R>
+constexpr int box_w() {
+    return sizeof(R) == 4 ? 3 : 1;
+}
+template <class R>
+using BoxView = std::conditional_t<sizeof(R) == 4, SpinsLds3<R>, SpinsSoA<R>>;
+template <class R>
+__device__ __forceinline__ BoxView<R> box_view(R *sx, int nrec) {
+    if constexpr (sizeof(R) == 4)
+        return SpinsLds3<R>{sx};
+    else
+        return SpinsSoA<R>{sx, sx + nrec, sx + 2 * nrec};
+}
+
+// remainder of 0 <= a < 2^24 by 1 <= m < 2^24 (rm: float reciprocal of m)
+__device__ __forceinline__ int fmodu(int a, int m, float rm) { return a - mulu24(fdiv(a, m, rm), m); }
+
+// A walk over the cells of an n0 x n1 x n2 block, row-major, by a fixed
+// stride: the position (u0, u1, u2) of cell c, then of c + stride, ...
+// (stride and every index below 2^24; one carry per dimension and step
+// because the stride's own digits d1 < n1 and d2 < n2)
+struct Walk3 {
+    int u0, u1, u2, d0, d1, d2, n1, n2;
+    __device__ __forceinline__ Walk3(int c, int stride, int n1_, int n2_) : n1(n1_), n2(n2_) {
+        const int n12 = n1 * n2;
+        const float r12 = 1.0f / (float)n12, r2 = 1.0f / (float)n2;
+        u0 = fdiv(c, n12, r12);
+        const int rc = c - mulu24(u0, n12);
+        u1 = fdiv(rc, n2, r2);
+        u2 = rc - mulu24(u1, n2);
+        d0 = fdiv(stride, n12, r12);
+        const int rs = stride - mulu24(d0, n12);
+        d1 = fdiv(rs, n2, r2);
+        d2 = rs - mulu24(d1, n2);
+    }
+    __device__ __forceinline__ void step() {
+        u2 += d2;
+        const int c2 = u2 >= n2 ? 1 : 0;
+        u2 -= c2 ? n2 : 0;
+        u1 += d1 + c2;
+        const int c1 = u1 >= n1 ? 1 : 0;
+        u1 -= c1 ? n1 : 0;
+        u0 += d0 + c1;
+    }
+};
+
 // KIND 0: overrelaxation, 1: deterministic, 2: Metropolis (a.kind picks the
-// uniform or the cone proposal)
-template <class R, bool SELF, int KIND>
-__global__ __launch_bounds__(256) void k_tile(TileArgs a) {
+// uniform or the cone proposal).  Index products use 24-bit multiplies: the
+// plan keeps every extent, the row count and the box below 2^24.
+template <class R, bool SELF, int KIND, int NT>
+__global__ __launch_bounds__(NT) void k_tile(TileArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr uint32_t RB = 3 * sizeof(R);  // bytes per packed record
     const int sys = a.sys0 + (int)blockIdx.y;
@@ -84,17 +162,16 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
     unsigned char *gout = reinterpret_cast<unsigned char *>(a.spins_out) + sd.spin_off * RB;
     const int nbox = t.nbox, nb = t.nb;
     R *sx = reinterpret_cast<R *>(smem);
-    const SpinsSoA<R> X{sx, sx + (nbox + 1), sx + 2 * (nbox + 1)};
+    const BoxView<R> X = box_view<R>(sx, nbox + 1);
     // run bases in LDS after the spins: runbase[r * nb + b]
     int *sbase = reinterpret_cast<int *>(sx + 3 * (nbox + 1));
     const int32_t *eb = a.tab + t.tab_off, *runbase = eb + nb, *delta = runbase + t.p * nb;
     const int32_t *rect = delta + nb * t.K2, *rawc = rect + t.nsw * t.ncol * nb * 6;
-    for (int k = threadIdx.x; k < t.p * nb; k += blockDim.x) sbase[k] = runbase[k];
-    if (threadIdx.x == 0) X.st(nbox, spin4<R>{R(0), R(0), R(0), R(0)});
-    const float r_B1 = 1.0f / (float)t.B[1], r_B2 = 1.0f / (float)t.B[2];
-    const float r_T1 = 1.0f / (float)t.T[1], r_T2 = 1.0f / (float)t.T[2];
-    const int cells_row = t.B[1] * t.B[2], tcells = t.tcells, trow = t.T[1] * t.T[2];
-    const float r_cr = 1.0f / (float)cells_row, r_tr = 1.0f / (float)trow;
+    for (int k = threadIdx.x; k < t.p * nb; k += NT) sbase[k] = runbase[k];
+    constexpr int W = box_w<R>();  // view index units per site
+    if (threadIdx.x == 0) X.st(nbox * W, spin4<R>{R(0), R(0), R(0), R(0)});
+    const float r_p = 1.0f / (float)t.p;
+    const int l2p = t.cz ? 0 : t.L[2] / t.p;  // cells of one raw colour per row
     int acc = 0;
     __syncthreads();  // sbase
     for (int tile = blockIdx.x; tile < t.ntiles; tile += gridDim.x) {
@@ -107,33 +184,37 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
                 r = q;
             }
         }
-        // device index of basis b at global cell x
-        auto dev = [&](int b, int x0, int x1, int x2) {
-            const int ci = (x0 * t.L[1] + x1) * t.L[2] + x2;
-            if (t.cz) return sbase[ldc(eb + b) * nb + b] + ci;
-            const int r = imod(ldc(eb + b) + t.c[0] * x0 + t.c[1] * x1 + t.c[2] * x2, t.p);
-            return sbase[r * nb + b] + (ci / t.L[2]) * (t.L[2] / t.p) + x2 / t.p;
+        // device index of basis b (wave-uniform) at global cell x (cz: base =
+        // the run base of the basis, read once per basis plane)
+        auto dev = [&](int b, int base, int x0, int x1, int x2) {
+            const int row = mulu24(x0, t.L[1]) + x1;
+            if (t.cz) return base + mulu24(row, t.L[2]) + x2;
+            const int r = fmodu(ldc(eb + b) + mulu24(t.c[0], x0) + mulu24(t.c[1], x1) + mulu24(t.c[2], x2),
+                                t.p, r_p);
+            return sbase[r * nb + b] + mulu24(row, l2p) + fdiv(x2, t.p, r_p);
         };
         // 1. the box, one basis plane at a time, four loads in flight
         for (int b = 0; b < nb; ++b) {
             const int bo = b * t.boxcells;
-            for (int c0 = threadIdx.x; c0 < t.boxcells; c0 += 4 * blockDim.x) {
+            const int base = t.cz ? sbase[ldc(eb + b) * nb + b] : 0;
+            Walk3 wk((int)threadIdx.x, NT, t.B[1], t.B[2]);
+            for (int c0 = threadIdx.x; c0 < t.boxcells; c0 += 4 * NT) {
                 spin4<R> v[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int c = min(c0 + q * (int)blockDim.x, t.boxcells - 1);
-                    const int u0 = fdiv(c, cells_row, r_cr), rc = c - u0 * cells_row;
-                    const int u1 = fdiv(rc, t.B[2], r_B2), u2 = rc - u1 * t.B[2];
-                    const int d = dev(b, wrap1(g0[0] + u0, t.L[0]), wrap1(g0[1] + u1, t.L[1]),
+                    // a cell past the box loads box cell 0 (not stored)
+                    const bool in = c0 + q * NT < t.boxcells;
+                    const int u0 = in ? wk.u0 : 0, u1 = in ? wk.u1 : 0, u2 = in ? wk.u2 : 0;
+                    const int d = dev(b, base, wrap1(g0[0] + u0, t.L[0]), wrap1(g0[1] + u1, t.L[1]),
                                       wrap1(g0[2] + u2, t.L[2]));
                     v[q] = rec_ld<R>(gin, (uint32_t)d * RB);
+                    wk.step();
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    if (c0 + q * (int)blockDim.x < t.boxcells) X.st(bo + c0 + q * (int)blockDim.x, v[q]);
+                    if (c0 + q * NT < t.boxcells) X.st((bo + c0 + q * NT) * W, v[q]);
             }
         }
-        (void)r_B1;
         __syncthreads();
         // 2. the colour steps
         const bool metro = KIND == 2;
@@ -150,27 +231,26 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
                 const int lo0 = ldc(rc), hi0 = ldc(rc + 1), lo1 = ldc(rc + 2), hi1 = ldc(rc + 3);
                 const int lo2 = ldc(rc + 4), hi2 = ldc(rc + 5);
                 if (lo0 >= hi0 || lo1 >= hi1 || lo2 >= hi2) continue;
-                const NbrD nbr{delta + b * t.K2, t.K2, nbox};
+                const NbrD<W> nbr{delta + b * t.K2, t.K2, nbox * W};
                 // cells per row: all of them (colour by basis), or every p-th
                 const int w2 = t.cz ? hi2 - lo2 : (hi2 - lo2 + t.p - 1) / t.p;
                 const int n1 = hi1 - lo1;
                 const int cnt = (hi0 - lo0) * n1 * w2;
                 const int e0 = ldc(eb + b);
-                const float r_w2 = 1.0f / (float)w2, r_n1 = 1.0f / (float)n1;
-                for (int w = threadIdx.x; w < cnt; w += blockDim.x) {
-                    const int row = fdiv(w, w2, r_w2), j = w - row * w2;
-                    const int q1 = fdiv(row, n1, r_n1);
-                    const int u0 = lo0 + q1, u1 = lo1 + (row - q1 * n1);
-                    int u2 = lo2 + j;
+                const int bo = b * t.boxcells;
+                Walk3 wk((int)threadIdx.x, NT, n1, w2);
+                for (int w = threadIdx.x; w < cnt; w += NT, wk.step()) {
+                    const int u0 = lo0 + wk.u0, u1 = lo1 + wk.u1;
+                    int u2 = lo2 + wk.u2;
                     if (!t.cz) {
                         // the cells of raw colour r0 in this row: c2 (lo2b + u2)
                         // = r0 - e_b - c0 (lo0b + u0) - c1 (lo1b + u1) mod p
                         // (tile origins are multiples of the pattern's period)
                         const int s = r0 - e0 - t.c[0] * (t.lo[0] + u0) - t.c[1] * (t.lo[1] + u1);
-                        u2 = lo2 + imod(s * t.cinv - t.lo[2] - lo2, t.p) + j * t.p;
+                        u2 = lo2 + imod(s * t.cinv - t.lo[2] - lo2, t.p) + wk.u2 * t.p;
                         if (u2 >= hi2) continue;
                     }
-                    const int li = b * t.boxcells + (u0 * t.B[1] + u1) * t.B[2] + u2;
+                    const int li = mulu24(bo + mulu24(mulu24(u0, t.B[1]) + u1, t.B[2]) + u2, W);
                     if constexpr (KIND == 0) {
                         or_update<R, false, 0>(L, nbr, X, b, li);
                     } else if constexpr (KIND == 1) {
@@ -198,17 +278,16 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
         }
         // 3. the tile's own sites, one basis plane at a time
         for (int b = 0; b < nb; ++b) {
-            for (int c = threadIdx.x; c < tcells; c += blockDim.x) {
-                const int v0 = fdiv(c, trow, r_tr), rc = c - v0 * trow;
-                const int v1 = fdiv(rc, t.T[2], r_T2), v2 = rc - v1 * t.T[2];
-                const int u0 = v0 - t.lo[0], u1 = v1 - t.lo[1], u2 = v2 - t.lo[2];
-                const int li = b * t.boxcells + (u0 * t.B[1] + u1) * t.B[2] + u2;
+            const int base = t.cz ? sbase[ldc(eb + b) * nb + b] : 0;
+            Walk3 wk((int)threadIdx.x, NT, t.T[1], t.T[2]);
+            for (int c = threadIdx.x; c < t.tcells; c += NT, wk.step()) {
+                const int u0 = wk.u0 - t.lo[0], u1 = wk.u1 - t.lo[1], u2 = wk.u2 - t.lo[2];
+                const int li = b * t.boxcells + mulu24(mulu24(u0, t.B[1]) + u1, t.B[2]) + u2;
                 // g0 + u is inside the lattice for the tile's own cells
-                const int d = dev(b, g0[0] + u0, g0[1] + u1, g0[2] + u2);
-                rec_st<R>(gout, (uint32_t)d * RB, X.ld(li));
+                const int d = dev(b, base, g0[0] + u0, g0[1] + u1, g0[2] + u2);
+                rec_st<R>(gout, (uint32_t)d * RB, X.ld(li * W));
             }
         }
-        (void)r_T1;
         __syncthreads();  // the next tile's box overwrites LDS
     }
     if constexpr (KIND == 2) {
@@ -217,14 +296,25 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
     }
 }
 
-template <class R, bool SELF, int KIND>
-static hipError_t launch_tile_t(dim3 grid, size_t lds, const TileArgs &a, hipStream_t st) {
+template <class R, bool SELF, int KIND, int NT>
+static hipError_t launch_tile_n(dim3 grid, size_t lds, const TileArgs &a, hipStream_t st) {
     if (lds > 65536) {
-        const hipError_t e = allow_large_lds((const void *)k_tile<R, SELF, KIND>);
+        const hipError_t e = allow_large_lds((const void *)k_tile<R, SELF, KIND, NT>);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_tile<R, SELF, KIND>), grid, dim3(256), lds, st, a);
+    hipLaunchKernelGGL((k_tile<R, SELF, KIND, NT>), grid, dim3(NT), lds, st, a);
     return hipGetLastError();
+}
+
+template <class R, bool SELF, int KIND>
+static hipError_t launch_tile_t(dim3 grid, size_t lds, const TileArgs &a, hipStream_t st) {
+    static const int nt = [] {
+        const char *e = getenv("CSMC_TILE_THREADS");
+        return e ? atoi(e) : 256;
+    }();
+    if (nt == 512) return launch_tile_n<R, SELF, KIND, 512>(grid, lds, a, st);
+    if (nt == 1024) return launch_tile_n<R, SELF, KIND, 1024>(grid, lds, a, st);
+    return launch_tile_n<R, SELF, KIND, 256>(grid, lds, a, st);
 }
 
 // The field uses the slot loop of the colour-pass kernel (slots in groups of

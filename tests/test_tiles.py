@@ -196,6 +196,40 @@ def replay_tiles(t, G, N, sweeps):
     return G
 
 
+def fdiv(a, d):
+    """k_tile's quotient from a float reciprocal: trunc(float32(a) *
+    float32(1 / d)), then one correction from the remainder"""
+    rd = np.float32(1.0) / np.float32(d)
+    q = int(np.float32(a) * rd)
+    r = a - q * d
+    return q + (-1 if r < 0 else (1 if r >= d else 0))
+
+
+def walk3(c, stride, n1, n2, count):
+    """the cells k_tile's Walk3 visits from cell c by `stride`, as
+    (u0, u1, u2), for the first `count` steps"""
+    n12 = n1 * n2
+    u0 = fdiv(c, n12)
+    rc = c - u0 * n12
+    u1 = fdiv(rc, n2)
+    u2 = rc - u1 * n2
+    d0 = fdiv(stride, n12)
+    rs = stride - d0 * n12
+    d1 = fdiv(rs, n2)
+    d2 = rs - d1 * n2
+    out = []
+    for _ in range(count):
+        out.append((u0, u1, u2))
+        u2 += d2
+        c2 = 1 if u2 >= n2 else 0
+        u2 -= n2 if c2 else 0
+        u1 += d1 + c2
+        c1 = 1 if u1 >= n1 else 0
+        u1 -= n1 if c1 else 0
+        u0 += d0 + c1
+    return out
+
+
 def pyrochlore_heisenberg():
     uc = Pyrochlore()
     for b1, b2, o in W.PYRO_BONDS:
@@ -289,3 +323,40 @@ def test_no_plan_reasons():
     assert tile_plan(W.honeycomb_kitaev(1.0, -2.0), (64, 64), periodic=0)[1] == "open boundaries"
     # too small for a box that does not wrap onto itself
     assert tile_plan(W.honeycomb_kitaev(1.0, -2.0), (2, 2))[0] is None
+
+
+@pytest.mark.parametrize("nsw", [1, 2])
+@pytest.mark.parametrize("name,make,shape", CASES, ids=[c[0] for c in CASES])
+def test_kernel_walks_equal_divmod(name, make, shape, nsw):
+    """the strided cell walks of k_tile (box staging, each colour step's
+    region, the tile store) visit the cells that row-major division gives,
+    for workgroups of 256 and 512 threads"""
+    t, _ = tile_plan(make(), shape, budget=(16 << 10) * nsw * nsw, nsw=nsw)
+    if t is None:
+        return
+    nb, p, cz = t["nb"], t["p"], t["cz"]
+    blocks = [(t["B"], t["boxcells"]), (t["T"], t["tcells"])]
+    for st in range(nsw * t["ncol"]):
+        for b in range(nb):
+            lo0, hi0, lo1, hi1, lo2, hi2 = t["rect"][(st * nb + b) * 6:(st * nb + b) * 6 + 6]
+            if lo0 >= hi0 or lo1 >= hi1 or lo2 >= hi2:
+                continue
+            w2 = hi2 - lo2 if cz else (hi2 - lo2 + p - 1) // p
+            blocks.append(([hi0 - lo0, hi1 - lo1, w2], (hi0 - lo0) * (hi1 - lo1) * w2))
+    for (n0, n1, n2), cnt in blocks:
+        for nt in (256, 512):
+            for tid in (0, 1, 37, nt - 1):
+                if tid >= cnt:
+                    continue
+                k = (cnt - tid + nt - 1) // nt
+                got = walk3(tid, nt, n1, n2, k)
+                want = [(c // (n1 * n2), (c // n2) % n1, c % n2) for c in range(tid, cnt, nt)]
+                assert got == want, (name, n0, n1, n2, nt, tid)
+
+
+def test_fdiv_exact():
+    """the float-reciprocal quotient is exact over the ranges k_tile uses"""
+    rng = np.random.default_rng(5)
+    for d in list(range(1, 300)) + [1023, 4096, 65535, (1 << 24) - 1]:
+        for a in list(rng.integers(0, 1 << 24, size=200)) + [0, d - 1, d, (1 << 24) - 1]:
+            assert fdiv(int(a), d) == int(a) // d, (a, d)
