@@ -404,23 +404,46 @@ __device__ __forceinline__ void interaction_field(const LatView<R> &L, const NB 
         for (int k = 0; k < K2C; ++k) sj[k] = ld(j[k]);
         bilinear_field<R, A, K2C, false>(L, b, sj, F);
     }
-    for (int k0 = 0; K2C == 0 && PART != 2 && k0 < L.K2P; k0 += 4) {
-        int ku = 4;
-        if constexpr (nb_knows_k2<NB>::value) ku = min(4, nb.K2 - k0);
-        int j[4];
+    // the bilinear slots in groups of four; DIAG: every coupling matrix is
+    // diagonal, and each row below rounds to m_rr s_r (its other terms are
+    // +-0), so only those products are formed, each rounded before the sum
+    // (a product that cannot be fused into the addition)
+    auto slots = [&](auto diag) {
+        constexpr bool DIAG = decltype(diag)::value;
+        for (int k0 = 0; k0 < L.K2P; k0 += 4) {
+            int ku = 4;
+            if constexpr (nb_knows_k2<NB>::value) ku = min(4, nb.K2 - k0);
+            int j[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) j[u] = nb(k0 + u, i);
-        spin4<R> sj[4];
+            for (int u = 0; u < 4; ++u) j[u] = nb(k0 + u, i);
+            spin4<R> sj[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) sj[u] = ld(j[u]);
+            for (int u = 0; u < 4; ++u) sj[u] = ld(j[u]);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (u >= ku) break;
-            const R *M = c2 + (b * L.K2P + k0 + u) * 9;
-            F[0] += (A)ldc(M + 0) * sj[u].x + (A)ldc(M + 1) * sj[u].y + (A)ldc(M + 2) * sj[u].z;
-            F[1] += (A)ldc(M + 3) * sj[u].x + (A)ldc(M + 4) * sj[u].y + (A)ldc(M + 5) * sj[u].z;
-            F[2] += (A)ldc(M + 6) * sj[u].x + (A)ldc(M + 7) * sj[u].y + (A)ldc(M + 8) * sj[u].z;
+            for (int u = 0; u < 4; ++u) {
+                if (u >= ku) break;
+                const R *M = c2 + (b * L.K2P + k0 + u) * 9;
+                if constexpr (DIAG) {
+#pragma clang fp contract(off)
+                    F[0] += (A)ldc(M + 0) * sj[u].x;
+                    F[1] += (A)ldc(M + 4) * sj[u].y;
+                    F[2] += (A)ldc(M + 8) * sj[u].z;
+                } else {
+                    // each row as fma(m2, z, fma(m0, x, m1 y)), written out
+                    // so that every kernel rounds it alike
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+                        F[r] += fma((A)ldc(M + 3 * r + 2), (A)sj[u].z,
+                                    fma((A)ldc(M + 3 * r), (A)sj[u].x, (A)ldc(M + 3 * r + 1) * (A)sj[u].y));
+                }
+            }
         }
+    };
+    if constexpr (K2C == 0 && PART != 2) {
+        if (nb_knows_k2<NB>::value && (L.flags & LAT_DIAG))  // wave-uniform
+            slots(std::true_type{});
+        else
+            slots(std::false_type{});
     }
     if (!GEN) return;
     // GS slots at a time: their patterns, partner indices and partner spins

@@ -37,6 +37,15 @@ def pyrochlore_heisenberg():
     return uc
 
 
+def square_xxz():
+    """diagonal couplings whose products round (the tiles' diagonal form
+    must round them as the colour passes do)"""
+    uc = W.square_heisenberg(J=1.0)
+    for d in ((1, 0), (0, 1)):
+        addBilinear(uc, 1, 1, np.diag([0.7, 0.7, -1.3]), d)
+    return uc
+
+
 def square_onsite():
     uc = W.square_heisenberg()
     addOnSite(uc, 1, np.diag([0.1, -0.05, 0.2]))
@@ -46,6 +55,7 @@ def square_onsite():
 CASES = [
     ("honeycomb_64", lambda: [(W.honeycomb_kitaev(1.0, -2.0), (64, 64))] * 4),
     ("square_onsite_64", lambda: [(square_onsite(), (64, 64))] * 3),
+    ("square_xxz_64", lambda: [(square_xxz(), (64, 64))] * 2),
     ("triangular_dm_48", lambda: [(W.triangular_dm(-1.0, 0.1, (0.0, 0.0, 0.2)), (48, 48))] * 3),
     ("pyrochlore_8", lambda: [(pyrochlore_heisenberg(), (8, 8, 8))] * 2),
     ("pyrochlore_12", lambda: [(pyrochlore_heisenberg(), (12, 12, 12))] * 2),
