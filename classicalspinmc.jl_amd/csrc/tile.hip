@@ -36,14 +36,29 @@ struct NbrD {
     __device__ __forceinline__ int operator()(int k, int i) const {
         if (k >= K2) return pad;
         const int v = ldc(d + k);
-        return v == TILE_PAD ? pad : i + v * W;
+        // the scaled step stays a scalar operand of one vector add
+        int vw = v * W;
+        asm("" : "+s"(vw));
+        return v == TILE_PAD ? pad : i + vw;
     }
     __device__ __forceinline__ int at(int k, int i) const { return (*this)(k, i); }
 };
 
 // a b for 0 <= a, b < 2^24 (a full-rate 24-bit multiply; the product must
 // fit 32 bits)
-__device__ __forceinline__ int mulu24(int a, int b) { return (int)__umul24((unsigned)a, (unsigned)b); }
+// (written as the instructions: the compiler otherwise forms some of these
+// products with the quarter-rate 32-bit multiplies)
+__device__ __forceinline__ int mulu24(int a, int b) {
+    int r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// a b + c for 0 <= a, b < 2^24
+__device__ __forceinline__ int madu24(int a, int b, int c) {
+    int r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
 
 // a / d for 0 <= a < 2^24 and 1 <= d < 2^24 from a float reciprocal rd of
 // d: the float quotient is within one of the true one (exact for d = 1),
@@ -58,6 +73,16 @@ __device__ __forceinline__ int fdiv(int a, int d, float rd) {
 __device__ __forceinline__ int imod(int a, int m) {
     const int r = a % m;
     return r < 0 ? r + m : r;
+}
+
+// byte offset of packed record d (3 x 4 or 3 x 8 bytes) as a shift and a
+// shift-add (the compiler would otherwise multiply with a quarter-rate
+// instruction)
+template <class R>
+__device__ __forceinline__ uint32_t rec_off(int d) {
+    uint32_t q = (uint32_t)d << (sizeof(R) == 4 ? 2 : 3);
+    asm("" : "+v"(q));
+    return q + (q << 1);
 }
 
 // the packed record at byte offset off from a wave-uniform base: a 32-bit
@@ -187,8 +212,8 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs a) {
         // device index of basis b (wave-uniform) at global cell x (cz: base =
         // the run base of the basis, read once per basis plane)
         auto dev = [&](int b, int base, int x0, int x1, int x2) {
-            const int row = mulu24(x0, t.L[1]) + x1;
-            if (t.cz) return base + mulu24(row, t.L[2]) + x2;
+            const int row = madu24(x0, t.L[1], x1);
+            if (t.cz) return base + madu24(row, t.L[2], x2);
             const int r = fmodu(ldc(eb + b) + mulu24(t.c[0], x0) + mulu24(t.c[1], x1) + mulu24(t.c[2], x2),
                                 t.p, r_p);
             return sbase[r * nb + b] + mulu24(row, l2p) + fdiv(x2, t.p, r_p);
@@ -207,7 +232,7 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs a) {
                     const int u0 = in ? wk.u0 : 0, u1 = in ? wk.u1 : 0, u2 = in ? wk.u2 : 0;
                     const int d = dev(b, base, wrap1(g0[0] + u0, t.L[0]), wrap1(g0[1] + u1, t.L[1]),
                                       wrap1(g0[2] + u2, t.L[2]));
-                    v[q] = rec_ld<R>(gin, (uint32_t)d * RB);
+                    v[q] = rec_ld<R>(gin, rec_off<R>(d));
                     wk.step();
                 }
 #pragma unroll
@@ -250,7 +275,7 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs a) {
                         u2 = lo2 + imod(s * t.cinv - t.lo[2] - lo2, t.p) + wk.u2 * t.p;
                         if (u2 >= hi2) continue;
                     }
-                    const int li = mulu24(bo + mulu24(mulu24(u0, t.B[1]) + u1, t.B[2]) + u2, W);
+                    const int li = mulu24(bo + madu24(madu24(u0, t.B[1], u1), t.B[2], u2), W);
                     if constexpr (KIND == 0) {
                         or_update<R, false, 0>(L, nbr, X, b, li);
                     } else if constexpr (KIND == 1) {
@@ -282,10 +307,10 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs a) {
             Walk3 wk((int)threadIdx.x, NT, t.T[1], t.T[2]);
             for (int c = threadIdx.x; c < t.tcells; c += NT, wk.step()) {
                 const int u0 = wk.u0 - t.lo[0], u1 = wk.u1 - t.lo[1], u2 = wk.u2 - t.lo[2];
-                const int li = b * t.boxcells + mulu24(mulu24(u0, t.B[1]) + u1, t.B[2]) + u2;
+                const int li = b * t.boxcells + madu24(madu24(u0, t.B[1], u1), t.B[2], u2);
                 // g0 + u is inside the lattice for the tile's own cells
                 const int d = dev(b, base, g0[0] + u0, g0[1] + u1, g0[2] + u2);
-                rec_st<R>(gout, (uint32_t)d * RB, X.ld(li * W));
+                rec_st<R>(gout, rec_off<R>(d), X.ld(mulu24(li, W)));
             }
         }
         __syncthreads();  // the next tile's box overwrites LDS
