@@ -35,8 +35,9 @@ fields:
   ssf             the equal-time structure factor of the 64 replicas of the
                   default workload at the 65 x 65 allowed momenta of one
                   zone copy, against the FP64 MFMA roofline
-  probe_L512      the default geometry at L=512 (537 MB of spin records,
-                  beyond the Infinity Cache) as batch-wide colour passes
+  probe_L512      the default geometry at L=512 (403 MB of spin records,
+                  beyond the Infinity Cache) as batch-wide tiled sweeps (two
+                  overrelaxation sweeps a launch)
 """
 import argparse
 import json
@@ -600,15 +601,20 @@ def bench_probe(args, csmc, W):
     if os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
-        traffic = tj["traffic_bytes_per_update"] * R * N
+        # HBM bytes per launch of the overrelaxation kernel (two sweeps of
+        # every replica from one staging of each tile)
+        traffic = tj["traffic_bytes_per_update"] * tj["updates_per_dispatch"]
         # the overrelaxation kernel's VALU issue at its own live rate (10 of
         # the 11 sweeps of a step)
         ms_or = tile_or_ms(csmc, b, ctx)
+        # LDS: a packed 12-byte record moves in two instructions (8 + 4 B)
         issue = issue_fractions(tj["sq_per_dispatch"], tj["updates_per_dispatch"], R * N / (ms_or * 1e-3),
-                                CHIP_CUS, lds_bytes_per_lane_inst=4)
+                                CHIP_CUS, lds_bytes_per_lane_inst=6)
         issue["issue_note"] = ("instruction counts of the overrelaxation tile kernel from its SQ counters "
                                "(profiles/), times its live updates/s (%.1f us per sweep): the kernel is "
-                               "VALU-issue bound" % (ms_or * 1e3))
+                               "bound by VALU issue; two sweeps a launch move %.1f measured HBM bytes per "
+                               "update, below the 24-byte algorithmic figure"
+                               % (ms_or * 1e3, tj["traffic_bytes_per_update"]))
     # the persistent kernel on the same state (storage mode 0: the packed
     # schedule splits each replica into parts on one XCD)
     b.sweep(csmc.OVERRELAX, n_sweeps=1)
@@ -628,10 +634,13 @@ def bench_probe(args, csmc, W):
                       "sites": N, "replicas": R, "storage_mode": b.storage_mode(), "tiles_per_replica": tiles},
            "roofline": dict({"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                             "kernel": "k_tile<float> over all replicas (one launch per sweep)" if tiles else
+                             "kernel": "k_tile<float> over all replicas (one launch per two overrelaxation "
+                                       "sweeps, one per Metropolis sweep)" if tiles else
                                        f"k_pass<float> over all replicas (one launch per colour pass: {why})",
-                             "note": "traffic = measured HBM bytes per sweep of all replicas (FETCH_SIZE + "
-                                     "WRITE_SIZE of the overrelaxation tile kernel, copy-kernel scaled)"},
+                             "note": "achieved = 24 algorithmic bytes per update x updates / kernel time; "
+                                     "traffic = measured HBM bytes per launch of the overrelaxation tile "
+                                     "kernel (two sweeps of all replicas; FETCH_SIZE + WRITE_SIZE, "
+                                     "copy-kernel scaled)"},
                             **issue),
            "persistent_kernel": {"value": persistent, "unit": "spin-updates/s",
                                  "form": "persistent kernel, storage mode 0, packed schedule (parts of a replica "

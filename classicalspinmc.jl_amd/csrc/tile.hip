@@ -18,7 +18,6 @@
 // neighbour.  The sweep reads one spin buffer and writes the other: a tile's
 // halo must hold the values from before the sweep, which a neighbouring tile
 // of the same launch may already have replaced in place.
-#include <cstdlib>
 #include <type_traits>
 
 #include "sweep.h"
@@ -331,14 +330,10 @@ static hipError_t launch_tile_n(dim3 grid, size_t lds, const TileArgs &a, hipStr
     return hipGetLastError();
 }
 
+// 256 threads a tile: 512 and 1024 measured 17 % and 60 % slower on the
+// honeycomb L=512 probe (more of each colour step's last pass idle)
 template <class R, bool SELF, int KIND>
 static hipError_t launch_tile_t(dim3 grid, size_t lds, const TileArgs &a, hipStream_t st) {
-    static const int nt = [] {
-        const char *e = getenv("CSMC_TILE_THREADS");
-        return e ? atoi(e) : 256;
-    }();
-    if (nt == 512) return launch_tile_n<R, SELF, KIND, 512>(grid, lds, a, st);
-    if (nt == 1024) return launch_tile_n<R, SELF, KIND, 1024>(grid, lds, a, st);
     return launch_tile_n<R, SELF, KIND, 256>(grid, lds, a, st);
 }
 
