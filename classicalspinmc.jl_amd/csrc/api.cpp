@@ -228,6 +228,7 @@ struct Batch {
     struct TileSet {
         size_t lds = 0;
         int max_tiles = 0;
+        bool flat = true;  // every plan's first position has extent 1
         DevBuf<TileDev> td;
         DevBuf<int32_t> tab;
     };
@@ -1832,6 +1833,7 @@ bool plan_tile_set(csmc_ctx *c, Batch *b, int nsw, Batch::TileSet &ts, std::stri
     std::vector<char> done(c->lats.size(), 0);
     size_t lds = 0;
     int max_tiles = 0;
+    bool flat = true;
     for (int s = 0; s < b->nsys; ++s) {
         const int id = b->lat[s];
         if (done[id]) continue;
@@ -1846,6 +1848,7 @@ bool plan_tile_set(csmc_ctx *c, Batch *b, int nsw, Batch::TileSet &ts, std::stri
         td[id] = P.d;
         lds = std::max(lds, (size_t)(P.d.nbox + 1) * rec + 4 * (size_t)P.d.p * P.d.nb);
         max_tiles = std::max(max_tiles, P.d.ntiles);
+        flat = flat && P.d.L[0] == 1;
         if (c->verbose)
             fprintf(stderr,
                     "csmc: lattice %d tiled, %d sweep(s) a launch: tile %dx%dx%d cells, box %dx%dx%d x %d bases "
@@ -1865,6 +1868,7 @@ bool plan_tile_set(csmc_ctx *c, Batch *b, int nsw, Batch::TileSet &ts, std::stri
     }
     ts.lds = lds;
     ts.max_tiles = max_tiles;
+    ts.flat = flat;
     return true;
 }
 
@@ -1958,6 +1962,7 @@ int sweep_passes(csmc_ctx *c, Batch *b, int32_t kind, const double *T, const dou
             done = many ? b->tiles.sweeps : 1;
             ta.td = ts.td.p;
             ta.tab = ts.tab.p;
+            ta.flat = ts.flat ? 1 : 0;
             ta.spins_in = b->spins.p;
             ta.spins_out = b->spins_alt.p;
             for (int s0 = 0; s0 < b->nsys; s0 += 65535) {

@@ -113,6 +113,7 @@ struct TileArgs {
     void *spins_out;         // the state after it (every site is written)
     const TileDev *td;       // [lattice id] tile plan (ntiles 0: not used)
     const int32_t *tab;      // tile table pool
+    int flat;                // every plan's first position has extent 1
     int sys0;
     int kind;
     uint32_t k0, k1;

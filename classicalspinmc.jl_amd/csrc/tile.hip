@@ -170,9 +170,11 @@ struct Walk3 {
 };
 
 // KIND 0: overrelaxation, 1: deterministic, 2: Metropolis (a.kind picks the
-// uniform or the cone proposal).  Index products use 24-bit multiplies: the
-// plan keeps every extent, the row count and the box below 2^24.
-template <class R, bool SELF, int KIND, int NT>
+// uniform or the cone proposal).  FLAT: the first of the three positions has
+// extent 1 for every lattice of the launch (two-dimensional lattices), so its
+// coordinate is 0 throughout.  Index products use 24-bit multiplies: the plan
+// keeps every extent, the row count and the box below 2^24.
+template <class R, bool SELF, int KIND, int NT, bool FLAT>
 __global__ __launch_bounds__(NT) void k_tile(TileArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr uint32_t RB = 3 * sizeof(R);  // bytes per packed record
@@ -211,33 +213,36 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs a) {
         // device index of basis b (wave-uniform) at global cell x (cz: base =
         // the run base of the basis, read once per basis plane)
         auto dev = [&](int b, int base, int x0, int x1, int x2) {
-            const int row = madu24(x0, t.L[1], x1);
+            const int row = FLAT ? x1 : madu24(x0, t.L[1], x1);
             if (t.cz) return base + madu24(row, t.L[2], x2);
-            const int r = fmodu(ldc(eb + b) + mulu24(t.c[0], x0) + mulu24(t.c[1], x1) + mulu24(t.c[2], x2),
+            const int r = fmodu(ldc(eb + b) + (FLAT ? 0 : mulu24(t.c[0], x0)) + mulu24(t.c[1], x1) +
+                                    mulu24(t.c[2], x2),
                                 t.p, r_p);
             return sbase[r * nb + b] + mulu24(row, l2p) + fdiv(x2, t.p, r_p);
         };
-        // 1. the box, one basis plane at a time, four loads in flight
+        // 1. the box, one basis plane at a time, four loads in flight (the
+        // plane's last cells one at a time)
         for (int b = 0; b < nb; ++b) {
             const int bo = b * t.boxcells;
             const int base = t.cz ? sbase[ldc(eb + b) * nb + b] : 0;
             Walk3 wk((int)threadIdx.x, NT, t.B[1], t.B[2]);
-            for (int c0 = threadIdx.x; c0 < t.boxcells; c0 += 4 * NT) {
+            // the record of the walk's cell, then the walk's next cell
+            auto load = [&]() {
+                const int x0 = FLAT ? 0 : wrap1(g0[0] + wk.u0, t.L[0]);
+                const int d = dev(b, base, x0, wrap1(g0[1] + wk.u1, t.L[1]), wrap1(g0[2] + wk.u2, t.L[2]));
+                const spin4<R> v = rec_ld<R>(gin, rec_off<R>(d));
+                wk.step();
+                return v;
+            };
+            int c0 = threadIdx.x;
+            for (; c0 + 3 * NT < t.boxcells; c0 += 4 * NT) {
                 spin4<R> v[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    // a cell past the box loads box cell 0 (not stored)
-                    const bool in = c0 + q * NT < t.boxcells;
-                    const int u0 = in ? wk.u0 : 0, u1 = in ? wk.u1 : 0, u2 = in ? wk.u2 : 0;
-                    const int d = dev(b, base, wrap1(g0[0] + u0, t.L[0]), wrap1(g0[1] + u1, t.L[1]),
-                                      wrap1(g0[2] + u2, t.L[2]));
-                    v[q] = rec_ld<R>(gin, rec_off<R>(d));
-                    wk.step();
-                }
+                for (int q = 0; q < 4; ++q) v[q] = load();
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (c0 + q * NT < t.boxcells) X.st((bo + c0 + q * NT) * W, v[q]);
+                for (int q = 0; q < 4; ++q) X.st((bo + c0 + q * NT) * W, v[q]);
             }
+            for (; c0 < t.boxcells; c0 += NT) X.st((bo + c0) * W, load());
         }
         __syncthreads();
         // 2. the colour steps
@@ -264,25 +269,26 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs a) {
                 const int bo = b * t.boxcells;
                 Walk3 wk((int)threadIdx.x, NT, n1, w2);
                 for (int w = threadIdx.x; w < cnt; w += NT, wk.step()) {
-                    const int u0 = lo0 + wk.u0, u1 = lo1 + wk.u1;
+                    const int u0 = FLAT ? 0 : lo0 + wk.u0, u1 = lo1 + wk.u1;
                     int u2 = lo2 + wk.u2;
                     if (!t.cz) {
                         // the cells of raw colour r0 in this row: c2 (lo2b + u2)
                         // = r0 - e_b - c0 (lo0b + u0) - c1 (lo1b + u1) mod p
                         // (tile origins are multiples of the pattern's period)
-                        const int s = r0 - e0 - t.c[0] * (t.lo[0] + u0) - t.c[1] * (t.lo[1] + u1);
+                        const int s = r0 - e0 - (FLAT ? 0 : t.c[0] * (t.lo[0] + u0)) - t.c[1] * (t.lo[1] + u1);
                         u2 = lo2 + imod(s * t.cinv - t.lo[2] - lo2, t.p) + wk.u2 * t.p;
                         if (u2 >= hi2) continue;
                     }
-                    const int li = mulu24(bo + madu24(madu24(u0, t.B[1], u1), t.B[2], u2), W);
+                    const int cell = FLAT ? madu24(u1, t.B[2], u2) : madu24(madu24(u0, t.B[1], u1), t.B[2], u2);
+                    const int li = mulu24(bo + cell, W);
                     if constexpr (KIND == 0) {
                         or_update<R, false, 0>(L, nbr, X, b, li);
                     } else if constexpr (KIND == 1) {
                         det_update<R, false, 0>(L, nbr, X, b, li);
                     } else {
-                        const int x0 = wrap1(g0[0] + u0, t.L[0]), x1 = wrap1(g0[1] + u1, t.L[1]);
+                        const int x0 = FLAT ? 0 : wrap1(g0[0] + u0, t.L[0]), x1 = wrap1(g0[1] + u1, t.L[1]);
                         const int x2 = wrap1(g0[2] + u2, t.L[2]);
-                        const bool own = (unsigned)(u0 + t.lo[0]) < (unsigned)t.T[0] &&
+                        const bool own = (FLAT || (unsigned)(u0 + t.lo[0]) < (unsigned)t.T[0]) &&
                                          (unsigned)(u1 + t.lo[1]) < (unsigned)t.T[1] &&
                                          (unsigned)(u2 + t.lo[2]) < (unsigned)t.T[2];
                         const uint32_t ref = (uint32_t)(((b * t.L[0] + x0) * t.L[1] + x1) * t.L[2] + x2);
@@ -305,10 +311,11 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs a) {
             const int base = t.cz ? sbase[ldc(eb + b) * nb + b] : 0;
             Walk3 wk((int)threadIdx.x, NT, t.T[1], t.T[2]);
             for (int c = threadIdx.x; c < t.tcells; c += NT, wk.step()) {
-                const int u0 = wk.u0 - t.lo[0], u1 = wk.u1 - t.lo[1], u2 = wk.u2 - t.lo[2];
-                const int li = b * t.boxcells + madu24(madu24(u0, t.B[1], u1), t.B[2], u2);
+                const int u0 = FLAT ? 0 : wk.u0 - t.lo[0], u1 = wk.u1 - t.lo[1], u2 = wk.u2 - t.lo[2];
+                const int li = b * t.boxcells +
+                               (FLAT ? madu24(u1, t.B[2], u2) : madu24(madu24(u0, t.B[1], u1), t.B[2], u2));
                 // g0 + u is inside the lattice for the tile's own cells
-                const int d = dev(b, base, g0[0] + u0, g0[1] + u1, g0[2] + u2);
+                const int d = dev(b, base, FLAT ? 0 : g0[0] + u0, g0[1] + u1, g0[2] + u2);
                 rec_st<R>(gout, rec_off<R>(d), X.ld(mulu24(li, W)));
             }
         }
@@ -320,13 +327,13 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs a) {
     }
 }
 
-template <class R, bool SELF, int KIND, int NT>
+template <class R, bool SELF, int KIND, int NT, bool FLAT>
 static hipError_t launch_tile_n(dim3 grid, size_t lds, const TileArgs &a, hipStream_t st) {
     if (lds > 65536) {
-        const hipError_t e = allow_large_lds((const void *)k_tile<R, SELF, KIND, NT>);
+        const hipError_t e = allow_large_lds((const void *)k_tile<R, SELF, KIND, NT, FLAT>);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_tile<R, SELF, KIND, NT>), grid, dim3(NT), lds, st, a);
+    hipLaunchKernelGGL((k_tile<R, SELF, KIND, NT, FLAT>), grid, dim3(NT), lds, st, a);
     return hipGetLastError();
 }
 
@@ -334,7 +341,8 @@ static hipError_t launch_tile_n(dim3 grid, size_t lds, const TileArgs &a, hipStr
 // honeycomb L=512 probe (more of each colour step's last pass idle)
 template <class R, bool SELF, int KIND>
 static hipError_t launch_tile_t(dim3 grid, size_t lds, const TileArgs &a, hipStream_t st) {
-    return launch_tile_n<R, SELF, KIND, 256>(grid, lds, a, st);
+    if (a.flat) return launch_tile_n<R, SELF, KIND, 256, true>(grid, lds, a, st);
+    return launch_tile_n<R, SELF, KIND, 256, false>(grid, lds, a, st);
 }
 
 // The field uses the slot loop of the colour-pass kernel (slots in groups of
