@@ -94,9 +94,10 @@ TilePlan plan_tiles(const HostLattice &L, size_t rec_bytes, size_t lds_budget, i
     }
     const long ncell = L.ncell, N = L.N;
     // the kernel's index products are 24-bit multiplies: the last extent,
-    // the row count and the raw colour's sum c . x + e below 2^24
+    // the row count over all bases and the raw colour's sum c . x + e below
+    // 2^24
     const long lim24 = 1L << 24;
-    if (Ls[2] >= lim24 || ncell / Ls[2] >= lim24 || (long)p * (Ls[0] + Ls[1] + Ls[2] + 1) >= lim24)
+    if (Ls[2] >= lim24 || N / Ls[2] >= lim24 || (long)p * (Ls[0] + Ls[1] + Ls[2] + 1) >= lim24)
         return no("lattice extents beyond 24-bit index arithmetic");
     auto cellidx = [&](const int *x) { return ((long)x[0] * Ls[1] + x[1]) * Ls[2] + x[2]; };
     // reference site index s = ((b L0 + x0) L1 + x1) L2 + x2

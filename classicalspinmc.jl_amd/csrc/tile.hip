@@ -291,7 +291,8 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs a) {
                         const bool own = (FLAT || (unsigned)(u0 + t.lo[0]) < (unsigned)t.T[0]) &&
                                          (unsigned)(u1 + t.lo[1]) < (unsigned)t.T[1] &&
                                          (unsigned)(u2 + t.lo[2]) < (unsigned)t.T[2];
-                        const uint32_t ref = (uint32_t)(((b * t.L[0] + x0) * t.L[1] + x1) * t.L[2] + x2);
+                        const uint32_t ref =
+                            (uint32_t)madu24(madu24(b * t.L[0] + x0, t.L[1], x1), t.L[2], x2);
                         const spin4<R> s = X.ld(li);
                         const uint4 r = philox4x32_10(make_uint4(ref, (uint32_t)sd.stream, ctr + (uint32_t)sw,
                                                                  TAG_METRO), a.k0,
