@@ -288,6 +288,11 @@ struct csmc_ctx {
     bool parttab = true;         // storage mode 0: partners of each workgroup's sites in LDS
                                  // (CSMC_PARTTAB=0: off)
     size_t tile_lds = 48 << 10;  // LDS budget of one tile (CSMC_TILE_LDS, KiB)
+    // ... of a tile of one sweep a launch (CSMC_TILE_LDS1, KiB): larger
+    // tiles, fewer workgroups per CU; on the honeycomb L=512 probe the
+    // Metropolis sweep took 568 us with 32 x 64-cell tiles (64 KB) against
+    // 620 with 32 x 32 (48 KB) and 1130 with 16 x 32 (24 KB)
+    size_t tile_lds1 = 64 << 10;
     int tile_sweeps = 2;         // sweeps per tiled launch where a plan allows (CSMC_TILE_SWEEPS, 1..8)
     int parts0 = 0;              // workgroups per system of the chained PT kernel in
                                  // storage mode 0: 0 chosen, 1 off, 2..8 forced
@@ -1175,7 +1180,9 @@ int32_t csmc_ctx_create(int32_t device, csmc_ctx **out) {
     const char *tl = getenv("CSMC_TILE");
     if (tl && tl[0] == '0') c->tile = false;
     const char *tb = getenv("CSMC_TILE_LDS");
-    if (tb && tb[0] >= '1' && tb[0] <= '9') c->tile_lds = (size_t)strtoul(tb, nullptr, 10) << 10;
+    if (tb && tb[0] >= '1' && tb[0] <= '9') c->tile_lds = c->tile_lds1 = (size_t)strtoul(tb, nullptr, 10) << 10;
+    const char *tb1 = getenv("CSMC_TILE_LDS1");
+    if (tb1 && tb1[0] >= '1' && tb1[0] <= '9') c->tile_lds1 = (size_t)strtoul(tb1, nullptr, 10) << 10;
     const char *tsw = getenv("CSMC_TILE_SWEEPS");
     if (tsw && tsw[0] >= '1' && tsw[0] <= '8' && tsw[1] == 0) c->tile_sweeps = tsw[0] - '0';
     const char *dt = getenv("CSMC_DENSE_TENSORS");
@@ -1826,7 +1833,7 @@ int constraint_run(csmc_ctx *c, Batch *b, uint64_t seed, int64_t n_sweeps) {
 
 // the tile plans of nsw sweeps per launch of every lattice of the batch;
 // false (and why) if some lattice has none
-bool plan_tile_set(csmc_ctx *c, Batch *b, int nsw, Batch::TileSet &ts, std::string &why) {
+bool plan_tile_set(csmc_ctx *c, Batch *b, int nsw, size_t budget, Batch::TileSet &ts, std::string &why) {
     const size_t rec = 3 * (b->prec == 32 ? 4 : 8);
     std::vector<TileDev> td(c->lats.size(), TileDev{});
     std::vector<int32_t> tab;
@@ -1838,7 +1845,7 @@ bool plan_tile_set(csmc_ctx *c, Batch *b, int nsw, Batch::TileSet &ts, std::stri
         const int id = b->lat[s];
         if (done[id]) continue;
         done[id] = 1;
-        TilePlan P = plan_tiles(*c->lats[id], rec, c->tile_lds, nsw);
+        TilePlan P = plan_tiles(*c->lats[id], rec, budget, nsw);
         if (!P.ok) {
             why = "lattice " + std::to_string(id) + ": " + P.why;
             return false;
@@ -1879,13 +1886,13 @@ bool tiles_ready(csmc_ctx *c, Batch *b) {
     Batch::Tiles &tl = b->tiles;
     if (tl.state != 0) return tl.state > 0;
     tl.state = -1;
-    if (!plan_tile_set(c, b, 1, tl.one, tl.why)) {
+    if (!plan_tile_set(c, b, 1, c->tile_lds1, tl.one, tl.why)) {
         if (c->verbose) fprintf(stderr, "csmc: batch-wide sweeps without tiles (%s)\n", tl.why.c_str());
         return false;
     }
     tl.sweeps = 1;
     std::string why;
-    if (c->tile_sweeps > 1 && plan_tile_set(c, b, c->tile_sweeps, tl.many, why))
+    if (c->tile_sweeps > 1 && plan_tile_set(c, b, c->tile_sweeps, c->tile_lds, tl.many, why))
         tl.sweeps = c->tile_sweeps;
     else if (c->tile_sweeps > 1 && c->verbose)
         fprintf(stderr, "csmc: tiles of one sweep a launch (%s)\n", why.c_str());
