@@ -11,7 +11,7 @@ import pytest
 
 import csmc
 from csmc import workloads as W
-from csmc.unitcell import Pyrochlore, addBilinear, addOnSite
+from csmc.unitcell import Pyrochlore, UnitCell, addBasisSite, addBilinear, addOnSite, addZeemanCoupling
 
 pytestmark = pytest.mark.gpu
 
@@ -46,6 +46,17 @@ def square_xxz():
     return uc
 
 
+def xxz_chain():
+    """a one-dimensional chain: two raw colours (x mod 2), diagonal couplings
+    whose products round, a field"""
+    uc = UnitCell([1.0])
+    addBasisSite(uc, np.zeros(1))
+    for d in ((1,), (-1,)):
+        addBilinear(uc, 1, 1, np.diag([0.7, 0.7, -1.3]), d)
+    addZeemanCoupling(uc, 1, np.array([0.0, 0.1, 0.2]))
+    return uc
+
+
 def square_onsite():
     uc = W.square_heisenberg()
     addOnSite(uc, 1, np.diag([0.1, -0.05, 0.2]))
@@ -56,6 +67,7 @@ CASES = [
     ("honeycomb_64", lambda: [(W.honeycomb_kitaev(1.0, -2.0), (64, 64))] * 4),
     ("square_onsite_64", lambda: [(square_onsite(), (64, 64))] * 3),
     ("square_xxz_64", lambda: [(square_xxz(), (64, 64))] * 2),
+    ("xxz_chain_4096", lambda: [(xxz_chain(), (4096,))] * 3),
     ("triangular_dm_48", lambda: [(W.triangular_dm(-1.0, 0.1, (0.0, 0.0, 0.2)), (48, 48))] * 3),
     ("pyrochlore_8", lambda: [(pyrochlore_heisenberg(), (8, 8, 8))] * 2),
     ("pyrochlore_12", lambda: [(pyrochlore_heisenberg(), (12, 12, 12))] * 2),

@@ -20,7 +20,7 @@ import pytest
 
 from csmc import _lib, workloads as W
 from csmc.lattice import Lattice
-from csmc.unitcell import Pyrochlore, addBasisSite, addBilinear
+from csmc.unitcell import Pyrochlore, UnitCell, addBasisSite, addBilinear, addZeemanCoupling
 
 HEAD = ["D", "nb", "K2", "p", "cz", "cinv", "ncol", "ntiles"]
 TILE_PAD = 0x7FFFFFFF
@@ -230,6 +230,17 @@ def walk3(c, stride, n1, n2, count):
     return out
 
 
+def xxz_chain():
+    """a one-dimensional chain (two raw colours, x mod 2), diagonal couplings
+    whose products round, and a field"""
+    uc = UnitCell([1.0])
+    addBasisSite(uc, np.zeros(1))
+    for d in ((1,), (-1,)):
+        addBilinear(uc, 1, 1, np.diag([0.7, 0.7, -1.3]), d)
+    addZeemanCoupling(uc, 1, np.array([0.0, 0.1, 0.2]))
+    return uc
+
+
 def pyrochlore_heisenberg():
     uc = Pyrochlore()
     for b1, b2, o in W.PYRO_BONDS:
@@ -245,6 +256,7 @@ CASES = [
     ("triangular_dm_48x48", lambda: W.triangular_dm(-1.0, 0.1, (0.0, 0.0, 0.2)), (48, 48)),
     ("pyrochlore_8", pyrochlore_heisenberg, (8, 8, 8)),
     ("pyrochlore_12", pyrochlore_heisenberg, (12, 12, 12)),
+    ("xxz_chain_512", xxz_chain, (512,)),
 ]
 
 
