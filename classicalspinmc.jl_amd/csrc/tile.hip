@@ -1,23 +1,25 @@
-// tile.hip -- one coloured sweep of every system from LDS tiles, for periodic
-// Bravais lattices whose neighbours and device order are arithmetic (the
-// host plan and its conditions are in tile.cpp).
+// tile.hip -- one or several coloured sweeps of every system from LDS tiles,
+// for periodic Bravais lattices whose neighbours and device order are
+// arithmetic (the host plan and its conditions are in tile.cpp).
 //
 // A workgroup sweeps one tile of one system at a time:
 //   1. stage the box (tile + halo) from the packed global records into LDS,
 //      each site's device index computed from its cell and basis;
-//   2. for each colour step, update every site of that colour in the step's
-//      region: the tile, plus the halo sites that later steps of this tile
-//      read.  A partner's box index is the site's index plus a per-(basis,
-//      slot) constant, so no neighbour index is loaded.  The updates are the
-//      ones of the colour-pass kernel (same functions, same Philox counter
-//      (reference site, stream, sweep counter, METR)), so a halo site ends
-//      the step with the value its own tile gives it;
+//   2. for each colour step of the launch's sweeps (nsw x ncol steps),
+//      update every site of that colour in the step's region: the tile, plus
+//      the halo sites that later steps of this tile read.  A partner's box
+//      index is the site's index plus a per-(basis, slot) constant, so no
+//      neighbour index is loaded.  The updates are the ones of the
+//      colour-pass kernel (same functions, same Philox counter (reference
+//      site, stream, sweep counter + the sweep's index in the launch,
+//      METR)), so a halo site ends each step with the value its own tile
+//      gives it;
 //   3. store the tile's own sites.
-// Every system's sweep thus reads each record once (plus the halo) and writes
-// it once, instead of once per colour pass with a gathered index per
-// neighbour.  The sweep reads one spin buffer and writes the other: a tile's
-// halo must hold the values from before the sweep, which a neighbouring tile
-// of the same launch may already have replaced in place.
+// Every launch thus reads each record once (plus the halo) and writes it
+// once for nsw sweeps, instead of once per colour pass with a gathered index
+// per neighbour.  The launch reads one spin buffer and writes the other: a
+// tile's halo must hold the values from before the launch, which a
+// neighbouring tile of the same launch may already have replaced in place.
 #include <type_traits>
 
 #include "sweep.h"
