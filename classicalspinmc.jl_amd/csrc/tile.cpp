@@ -8,9 +8,9 @@
 // e_b) mod p], the device order (colour, basis, reference index) is also
 // arithmetic: the run of (colour, basis) starts at a known index and a
 // cell's rank in it is its cell index (c = 0) or, when c[D-1] is invertible
-// mod p, row * (L[D-1] / p) + x[D-1] / p.  A sweep can then run tile by
-// tile from LDS: a tile of T cells is loaded with the halo that one sweep
-// of it reads, the halo sites that the tile's later colour steps read are
+// mod p, row * (L[D-1] / p) + x[D-1] / p.  Sweeps can then run tile by
+// tile from LDS: a tile of T cells is loaded with the halo that nsw sweeps
+// of it read, the halo sites that the tile's later colour steps read are
 // updated too (recomputed, with the same counter-based random numbers, so
 // they equal what their own tile computes), and only the tile is stored.
 #include "tile.h"
