@@ -611,8 +611,9 @@ def bench_probe(args, csmc, W):
         issue = issue_fractions(tj["sq_per_dispatch"], tj["updates_per_dispatch"], R * N / (ms_or * 1e-3),
                                 CHIP_CUS, lds_bytes_per_lane_inst=6)
         issue["issue_note"] = ("instruction counts of the overrelaxation tile kernel from its SQ counters "
-                               "(profiles/), times its live updates/s (%.1f us per sweep): the kernel is "
-                               "bound by VALU issue; two sweeps a launch move %.1f measured HBM bytes per "
+                               "(profiles/), times its live updates/s (%.1f us per sweep): VALU issue is "
+                               "the kernel's main limit (SALU, LDS latency and the barriers between colour "
+                               "steps the rest); two sweeps a launch move %.1f measured HBM bytes per "
                                "update, below the 24-byte algorithmic figure"
                                % (ms_or * 1e3, tj["traffic_bytes_per_update"]))
     # the persistent kernel on the same state (storage mode 0: the packed
