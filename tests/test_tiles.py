@@ -330,6 +330,33 @@ def test_plan_geometry():
     assert (t64["nbox"] + 1) * 24 <= 48 << 10 and t64["nbox"] < th["nbox"]
 
 
+def test_two_sweep_geometry():
+    """two sweeps a launch: the honeycomb box grows by one cell a side (the
+    halo of the first sweep's partners), and the regions shrink from the
+    first step to the last, which covers the tile alone"""
+    t, _ = tile_plan(W.honeycomb_kitaev(1.0, -2.0), (512, 512), nsw=2)
+    assert t["nsw"] == 2 and t["T"] == [1, 32, 32] and t["B"] == [1, 36, 36] and t["lo"] == [0, -2, -2]
+    nb, S = t["nb"], 2 * t["ncol"]
+    size = []
+    for st in range(S):
+        cells = 0
+        for b in range(nb):
+            lo0, hi0, lo1, hi1, lo2, hi2 = t["rect"][(st * nb + b) * 6:(st * nb + b) * 6 + 6]
+            cells += max(0, hi0 - lo0) * max(0, hi1 - lo1) * max(0, hi2 - lo2)
+        size.append(cells)
+    assert size == sorted(size, reverse=True) and size[-1] == 32 * 32
+    # the last step's region is the tile itself, box cells [2, 34)
+    last = [t["rect"][((S - 1) * nb + b) * 6:((S - 1) * nb + b) * 6 + 6] for b in range(nb)]
+    assert [0, 1, 2, 34, 2, 34] in last
+
+
+def test_sweeps_argument_checked():
+    """csmc_tile_plan takes 1..8 sweeps a launch"""
+    for bad in (0, 9):
+        with pytest.raises(_lib.CsmcError):
+            tile_plan(W.honeycomb_kitaev(1.0, -2.0), (64, 64), nsw=bad)
+
+
 def test_no_plan_reasons():
     assert tile_plan(W.pyrochlore_mixed(), (8, 8, 8))[1] == "cubic or quartic terms"
     assert tile_plan(W.honeycomb_kitaev(1.0, -2.0), (64, 64), periodic=0)[1] == "open boundaries"
