@@ -632,7 +632,8 @@ def bench_probe(args, csmc, W):
            "config": {"workload": f"roofline probe: honeycomb Kitaev-Heisenberg J=1 K=-2, L={L} ({N} spins), "
                                   f"{R} FP32 replicas ({R * (N + 1) * 12 / 1e6:.0f} MB of spin records), 10 "
                                   "overrelaxation + 1 Metropolis sweep per step, batch-wide tiled sweeps",
-                      "sites": N, "replicas": R, "storage_mode": b.storage_mode(), "tiles_per_replica": tiles},
+                      "sites": N, "replicas": R, "storage_mode": b.storage_mode(),
+                      "tiles_per_replica_one_sweep_plan": tiles, "sweeps_per_tiled_launch": b.tile_sweeps()},
            "roofline": dict({"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                              "kernel": "k_tile<float> over all replicas (one launch per two overrelaxation "
